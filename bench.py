@@ -1,0 +1,155 @@
+#!/usr/bin/env python
+"""Benchmark of the DUGKS time-step engine on one MI355X.
+
+Workload: the 3-D lid-driven cavity, 48^3 hexes x 24^3 velocities (Kn 0.1,
+Ma 0.16 lid, diffuse walls at 273 K), FP64 plain path, fixed dt at Courant 0.5.
+A step is one full DUGKS time step (fvDVM::evolution) over the whole mesh with
+the state resident in HBM.  value = cell-velocity updates per second (G/s).
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(case, storage_bytes: int):
+    """Bytes per launch the step's two kernels must move (bytes model):
+    k_cells reads and writes every resident distribution value once and reads the
+    lagged boundary gradient; k_faces reads every distribution value once and reads
+    and writes the boundary gradient.  h is not resident when K + 3 - D = 0."""
+    s = storage_bytes
+    nd = 1 if (case.K + 3 - case.dim) == 0 else 2
+    nc, nv = case.n_cells, case.nv
+    nw = sum(p["size"] for p in case.patches if p["type"] == 1)
+    cells = nd * (2 * s * nc * nv + s * nw * nv)
+    faces = nd * (s * nc * nv + 2 * s * nw * nv)
+    return faces, cells
+
+
+def traffic_from_profile(kernel_prefix: str, storage: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
+    recorded in profiles/counters_cavity3d.json, else None."""
+    p = os.path.join(HERE, "profiles", "counters_cavity3d.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get(storage, {}).get(kernel_prefix)
+    except Exception:
+        return None
+
+
+def run_gpu(case, storage: str, steps: int, warmup: int):
+    from dugksfoam_amd.engine import Engine
+    e = Engine(case, storage=storage)
+    e.init_equilibrium()
+    e.step(nsteps=warmup)
+    e.set_profiling(True)
+    t0 = time.perf_counter()
+    e.step(nsteps=steps)          # returns after the engine stream has drained
+    t1 = time.perf_counter()
+    faces_ms, cells_ms, n = e.profile()
+    m = e.macro()
+    ok = bool(np.all(np.isfinite(m["rho"])) and np.all(m["rho"] > 0))
+    e.close()
+    wall = t1 - t0
+    upd = case.n_cells * case.nv * steps
+    sb = 8 if storage == "fp64" else 4
+    bf, bc = algorithmic_bytes(case, sb)
+    kern = {"k_faces": (faces_ms / n, bf), "k_cells": (cells_ms / n, bc)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    avg_ms, bytes_ = kern[dom]
+    achieved = bytes_ / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(dom, storage),
+            "bytes_per_launch": bytes_, "avg_launch_ms": round(avg_ms, 4),
+            "other_kernel": {k: {"avg_launch_ms": round(v[0], 4), "bytes_per_launch": v[1],
+                                 "GBs": round(v[1] / (v[0] * 1e-3) / 1e9, 1)} for k, v in kern.items() if k != dom}}
+    return dict(value=upd / wall / 1e9, ms_per_step=wall / steps * 1e3, kernel_ms=(faces_ms + cells_ms) / n,
+                roofline=roof, finite=ok)
+
+
+def cpu_baseline(target_s: float = 12.0):
+    """The FP64 oracle (a CPU restatement of the reference's time step) on a bounded
+    sample of the workload: the same 48^3 mesh with a 12^3 velocity grid."""
+    from dugksfoam_amd import cases
+    from oracle.oracle import Oracle
+    threads = min(os.cpu_count() or 1, 16)
+    c = cases.cavity(48, 12, dim=3, kn=0.1)
+    o = Oracle(c, threads=threads)
+    o.init_equilibrium()
+    n, t = 0, 0.0
+    while t < target_s and n < 5:
+        t0 = time.perf_counter()
+        o.step(nsteps=1)
+        t += time.perf_counter() - t0
+        n += 1
+    o.close()
+    rate = c.n_cells * c.nv * n / t / 1e9
+    return {"value": rate, "unit": "G cell-velocity updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, 48^3 cells x 12^3 velocities, {n} step(s), {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage form")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the DUGKS engine on a single MI355X; "
+              "multi-GPU velocity-space sharding is not built (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from dugksfoam_amd import cases
+    case = cases.config("cavity3d")
+    log(f"[bench] cavity3d cells={case.n_cells} nv={case.nv} steps={a.steps} warmup={a.warmup}")
+    r64 = run_gpu(case, "fp64", a.steps, a.warmup)
+    log(f"[bench] fp64 {r64['value']:.2f} G upd/s, {r64['ms_per_step']:.2f} ms/step")
+    out = {
+        "metric": "G cell-velocity updates/s (FP64 plain path; FP32-storage form in fp32)",
+        "value": round(r64["value"], 3),
+        "unit": "G cell-velocity updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r64["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "cavity3d: 48^3 hexes x 24^3 velocities, Kn 0.1, lid Ma 0.16, diffuse walls 273 K",
+                   "cells": case.n_cells, "velocities": case.nv, "dt": case.dt,
+                   "resident": "g only (h is identically zero for D=3, KInner=0)"},
+        "roofline": r64["roofline"],
+    }
+    if not a.no_fp32:
+        r32 = run_gpu(case, "fp32", a.steps, a.warmup)
+        log(f"[bench] fp32 {r32['value']:.2f} G upd/s")
+        out["fp32"] = {"value": round(r32["value"], 3), "ms_per_step": round(r32["ms_per_step"], 3),
+                       "speedup_vs_fp64": round(r32["value"] / r64["value"], 3), "roofline": r32["roofline"]}
+    if not a.no_cpu:
+        cb = cpu_baseline()
+        log(f"[bench] cpu baseline {cb['value']:.4f} G upd/s on {cb['cores']} threads")
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/*
+ * dugks.h -- C-ABI of the MI355X (gfx950) DUGKS time-step engine.
+ *
+ * The engine advances the Shakhov-model discrete unified gas kinetic scheme
+ * (DUGKS) on an unstructured finite-volume mesh.  It replaces the body of
+ * Foam::fvDVM (src/fvDVM/fvDVM/fvDVM.H:78-377 in dugksFoam): an OpenFOAM-side
+ * shim keeps the fvDVM.H public API and forwards to the functions below.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; no C++ types cross the ABI, no exceptions.
+ *  - Every function returns an int status: 0 = ok, < 0 = dugks_status code.
+ *    dugks_last_error(h) returns a message for the last failure on h.
+ *  - Caller-owned host buffers are copied during the call and never retained.
+ *  - One engine = one device + one HIP stream.  Calls on one engine must be
+ *    serialised by the caller; get_* calls synchronise the engine's stream.
+ *  - Mesh numbering follows OpenFOAM's polyMesh: faces 0..n_internal_faces-1
+ *    are internal (owner < neighbour, Sf points owner -> neighbour), then the
+ *    faces of every non-empty boundary patch in patch order (Sf outward).
+ *    Empty patches (2-D front/back) are omitted, as fvPatch gives them size 0.
+ *  - Discrete velocities are the tensor product of the 1-D set in
+ *    constant/Xis + constant/weights, ordered k = (iz*nY + iy)*nX + ix with
+ *    xi = (Xis[ix], Xis[iy], Xis[iz]) and w = w[iz]*w[iy]*w[ix]
+ *    (fvDVM.C:140-221; z components are 0 in 2-D).
+ */
+#ifndef DUGKS_H
+#define DUGKS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DUGKS_ABI_VERSION 1
+
+/* status codes */
+enum dugks_status {
+    DUGKS_OK = 0,
+    DUGKS_ERR_INVALID = -1,     /* invalid argument or case description   */
+    DUGKS_ERR_DEVICE = -2,      /* HIP runtime error                       */
+    DUGKS_ERR_MEMORY = -3,      /* device allocation failed                */
+    DUGKS_ERR_STATE = -4,       /* call not valid in the engine's state    */
+    DUGKS_ERR_NONFINITE = -5,   /* step produced a non-finite or rho <= 0  */
+    DUGKS_ERR_UNSUPPORTED = -6  /* feature not supported by this build     */
+};
+
+/* Boundary patch kinds.  Each is named by the rho boundary type that selects
+ * it in dugksFoam (the DF-side type follows bcMap, discreteVelocity.C:274-281). */
+enum dugks_patch_type {
+    DUGKS_PATCH_WALL = 1,      /* rho calculatedMaxwell, U/T fixedValue: diffuse Maxwell wall */
+    DUGKS_PATCH_CYCLIC = 2     /* OpenFOAM cyclic (coupled) patch pair                      */
+};
+
+/* Storage precision of the resident distributions. */
+enum dugks_storage {
+    DUGKS_STORAGE_FP64 = 0,    /* plain FP64 path                                 */
+    DUGKS_STORAGE_FP32 = 1     /* g/h stored in FP32; equilibrium, moments in FP64 */
+};
+
+typedef struct dugks_patch {
+    int32_t type;              /* dugks_patch_type                                   */
+    int32_t start;             /* first face, global face numbering                  */
+    int32_t size;              /* number of faces                                    */
+    int32_t neighbour_patch;   /* cyclic partner patch index, else -1                */
+    const double* U;           /* [size][3] boundary U (wall velocity), may be NULL for cyclic */
+    const double* T;           /* [size]    boundary T (wall temperature), NULL for cyclic     */
+} dugks_patch;
+
+typedef struct dugks_case {
+    int32_t abi_version;       /* = DUGKS_ABI_VERSION                                */
+    int32_t dim;               /* mesh.nSolutionD(): 2 or 3                          */
+    int32_t n_cells;
+    int32_t n_internal_faces;
+    int32_t n_faces;           /* internal + faces of non-empty patches              */
+    const int32_t* owner;      /* [n_faces]  fvMesh::owner() / faceCells             */
+    const int32_t* neighbour;  /* [n_internal_faces]                                 */
+    const double* C;           /* [n_cells][3] cell centres                          */
+    const double* V;           /* [n_cells]    cell volumes                          */
+    const double* Sf;          /* [n_faces][3] face area vectors                     */
+    const double* Cf;          /* [n_faces][3] face centres                          */
+    const double* weights;     /* [n_faces] surfaceInterpolation::weights()          */
+    const double* delta;       /* [n_faces - n_internal_faces][3] fvPatch::delta()   */
+    int32_t n_patches;
+    const dugks_patch* patches;
+    /* velocity space (DVMProperties fvDVMparas + constant/Xis, constant/weights) */
+    int32_t n_xi;              /* nDV, points per axis                               */
+    const double* xi;          /* [n_xi] */
+    const double* xi_weights;  /* [n_xi] */
+    double xi_max;             /* xiMax, used by the Courant number                  */
+    /* gas (DVMProperties gasProperties) */
+    double R, omega, T_ref, mu_ref, Pr;
+    int32_t K;                 /* KInner                                             */
+    int32_t storage;           /* dugks_storage                                      */
+    int32_t device;            /* HIP device ordinal                                 */
+    /* Independent cases advanced together (batching).  n_cases <= 1 means one
+     * case.  cell_case[c] names the case of cell c; the mesh is the disjoint
+     * union of the cases' meshes (no face joins two cases). */
+    int32_t n_cases;
+    const int32_t* cell_case;  /* [n_cells] or NULL                                  */
+    const double* mu_ref_case; /* [n_cases] per-case muRef, or NULL (= mu_ref)       */
+} dugks_case;
+
+typedef struct dugks_engine dugks_engine;
+
+/* Builds the engine: validates the case, derives the least-squares vectors,
+ * uploads mesh and velocity set once, allocates resident state.
+ * Replaces the fvDVM constructor (fvDVM.C:886-1075). */
+int dugks_create(const dugks_case* cs, dugks_engine** out);
+
+/* Sets rho, U, T in every cell (q = 0) and the distributions to the Shakhov
+ * equilibrium with q = 0 (discreteVelocity::initDFtoEq, discreteVelocity.C:220-249;
+ * wall inComingByRho, fvDVM.C:263-309; tau, fvDVM.C:1073; Usurf, fvDVM.C:1074). */
+int dugks_init_equilibrium(dugks_engine* h, const double* rho, const double* U, const double* T);
+
+/* Advances nsteps time steps of size dt: fvDVM::evolution() (fvDVM.C:1086-1108)
+ * called nsteps times with a fixed deltaT. */
+int dugks_step(dugks_engine* h, double dt, int32_t nsteps);
+
+/* As dugks_step with one time step per case: dt_case[n_cases]. */
+int dugks_step_cases(dugks_engine* h, const double* dt_case, int32_t nsteps);
+
+/* Cell fields after the last step: rho[n_cells], U[n_cells][3], T[n_cells],
+ * q[n_cells][3] (fvDVM::rhoVol/Uvol/Tvol/qVol, fvDVM.C:597-728).  NULL skips. */
+int dugks_get_macro(dugks_engine* h, double* rho, double* U, double* T, double* q);
+
+/* Face fields of the last step (fvDVM.C:456-582): rho, U[3], T, q[3] on all
+ * n_faces faces.  NULL skips. */
+int dugks_get_face_macro(dugks_engine* h, double* rho, double* U, double* T, double* q);
+
+/* Wall data of patch `patch` after the last step: rho_w (calculatedMaxwell value,
+ * calculatedMaxwellFvPatchField.C:139-164), qWall[3], stressWall[9] (fvDVM.C:539-581).
+ * Arrays are sized by the patch; NULL skips. */
+int dugks_get_wall(dugks_engine* h, int32_t patch, double* rho_w, double* q_wall, double* stress_wall);
+
+/* Courant number of fvDVM::getCoNum (fvDVM.C:1111-1119) for time step dt. */
+int dugks_get_courant(dugks_engine* h, double dt, double* co_max, double* co_mean);
+
+/* Distribution functions g~ and h~ of n cells (cells[n]) for every velocity in
+ * reference order: g[n][nv], h[n][nv] (fvDVM::writeDFonCell, fvDVM.C:820-875).
+ * Either output may be NULL. */
+int dugks_get_df(dugks_engine* h, int32_t n, const int32_t* cells, double* g, double* h_out);
+
+/* Number of velocities, cells and faces. */
+int dugks_get_sizes(dugks_engine* h, int32_t* nv, int32_t* n_cells, int32_t* n_faces);
+
+/* Complete resident state as an opaque byte image: resuming from it continues
+ * bit-identically.  dugks_state_size gives the byte count. */
+int dugks_state_size(dugks_engine* h, int64_t* bytes);
+int dugks_save_state(dugks_engine* h, void* buf, int64_t bytes);
+int dugks_load_state(dugks_engine* h, const void* buf, int64_t bytes);
+
+/* Synchronises the stream and reports the device time of the kernels of the
+ * last dugks_step call (ms), measured with HIP events on the engine stream. */
+int dugks_last_step_ms(dugks_engine* h, double* ms);
+
+/* Per-kernel timing (no reference counterpart; for measurement).  With
+ * profiling on, every step records HIP events on the engine stream around its
+ * two kernels; dugks_get_profile returns the accumulated device time of the
+ * face kernel (S1-S6) and the cell kernel (S7-S9 + next S1) and the number of
+ * steps since profiling was switched on. */
+int dugks_set_profiling(dugks_engine* h, int32_t on);
+int dugks_get_profile(dugks_engine* h, double* faces_ms, double* cells_ms, int64_t* steps);
+
+void dugks_destroy(dugks_engine* h);
+const char* dugks_last_error(dugks_engine* h);
+int dugks_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DUGKS_H */

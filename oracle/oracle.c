@@ -1,0 +1,927 @@
+/*
+ * oracle.c -- FP64 CPU restatement of dugksFoam's time step, fvDVM::evolution().
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the GPU engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it.  The product (dugksfoam_amd/) never links, loads or calls it.
+ *
+ * It restates, phase by phase and with the reference's operation order
+ * (no FMA contraction: built with -ffp-contract=off):
+ *   S1 updateGHbarPvol          discreteVelocity.C:346-410
+ *   S2 updateGHbarSurf          discreteVelocity.C:412-691 (+ OpenFOAM
+ *      leastSquaresGrad/leastSquaresVectors, whose algorithm is
+ *      zeroBoundaryGrad.C:87-138 and zeroBoundaryVectors.C:90-230 with the
+ *      commented-out non-coupled boundary blocks enabled)
+ *   S3 updateMaxwellWallRho     fvDVM.C:347-367, calculatedMaxwellFvPatchField.C:139-164
+ *   S4 updateGHbarSurfMaxwellWallIn  discreteVelocity.C:693-731
+ *   S6 updateMacroSurf          fvDVM.C:456-582
+ *   S7 updateGHsurf             discreteVelocity.C:819-932
+ *   S8 updateGHtildeVol         discreteVelocity.C:934-978
+ *   S9 updateMacroVol           fvDVM.C:597-728 (macroFlux no)
+ * with the equilibria of discreteVelocity.C:1016-1075, tau of fvDVM.C:808-817,
+ * the wall inComingByRho of fvDVM.C:263-309 and initDFtoEq (discreteVelocity.C:220-249).
+ *
+ * Memory layout is DV-outer ([k][cell]) and the per-DV intermediates
+ * (gBarPvol, gradients, gSurf) are recomputed in each pass over k instead of
+ * being held for every DV at once; each recomputation is the same arithmetic
+ * on the same inputs, so the results equal those of the materialised form.
+ * All velocity-space sums run sequentially in k as the reference's
+ * forAll(DV_, dvi) loops do.  Inside one k, cells and faces are split across
+ * OpenMP threads; every cell accumulates its faces in face-index order, the
+ * order of the reference's face loops, so results do not depend on the
+ * thread count.
+ *
+ * OpenFOAM semantics restated here (OpenFOAM is not vendored; parity against
+ * it is unpinned): fixedGradient boundary evaluation phi_b = phi_P +
+ * gradient/deltaCoeffs with deltaCoeffs = 1/|delta|; symmTensor inverse by
+ * cofactors, 2-D tensors inverted with a unit entry in the empty direction;
+ * coupled (cyclic) boundary values w*phi_P + (1-w)*phi_N; zeroGradient
+ * gradient boundary values equal to the cell gradient.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/dugks.h"
+
+#define VSMALL 1.0e-300
+#define PI_ 3.14159265358979323846
+
+typedef struct oracle {
+    int dim, nc, nfi, nf, nb, nv, nxi, K, n_cases;
+    double R, omega, Tref, mu_ref, Pr, xi_max;
+    int32_t *owner, *nei;
+    double *C, *V, *Sf, *Cf, *w, *delta, *magSf;
+    /* boundary faces */
+    int *bkind;      /* [nb] patch type                      */
+    int *bpatch;     /* [nb] patch index                     */
+    int *bpartner;   /* [nb] cyclic partner boundary index   */
+    double *Ub, *Tb; /* [nb*3], [nb] wall values             */
+    double *nhat;    /* [nb*3] Sf/|Sf|                       */
+    double *dcoef;   /* [nb] 1/|delta|                       */
+    int n_patches; int *pstart, *psize, *ptype;
+    /* least-squares vectors */
+    double *pvec;    /* [nf*3] */
+    double *nvec;    /* [nfi*3] */
+    /* cell -> faces CSR (face index ascending) */
+    int *cstart, *cface;
+    /* velocities */
+    double *xi, *wv; /* [nv*3], [nv] */
+    /* per-case */
+    int *cell_case; double *mu_case; double *dt_case;
+    /* state */
+    double *gt, *ht;           /* [nv][nc] g~, h~             */
+    double *gradb_g, *gradb_h; /* [nv][nb] fixedGradient value */
+    double *rho, *U, *T, *q, *tau;
+    double *incoming;          /* [nb] inComingByRho           */
+    double *rho_w;             /* [nb]                         */
+    double *rho_f, *U_f, *T_f, *q_f, *tau_f; /* [nf]           */
+    double *qwall, *stresswall;              /* [nb*3], [nb*9]  */
+    int initialised;
+    int cache_faces;
+    int nthreads;
+    /* scratch */
+    double *gbar, *hbar, *phib_g, *phib_h, *grad_g, *grad_h, *gs, *hs;
+    double *bgrad_g, *bgrad_h; /* cyclic boundary gradient values [nb*3] */
+    double *newgb_g, *newgb_h; /* [nb] new fixedGradient values           */
+    double *cache_g, *cache_h; /* [nv][nf] cached pre-S4 face values      */
+    double *outgoing;          /* [nb] */
+    double *acc_rho, *acc_rhoU, *acc_rhoE; /* [nf] / [nf*3] */
+    double *cacc_rho, *cacc_rhoU, *cacc_rhoE; /* [nc] / [nc*3] */
+} oracle;
+
+static void *xmalloc(size_t n) { void *p = calloc(n ? n : 1, 1); return p; }
+
+static inline double dot3(const double *a, const double *b) { return a[0]*b[0] + a[1]*b[1] + a[2]*b[2]; }
+static inline double magsqr3(const double *a) { return a[0]*a[0] + a[1]*a[1] + a[2]*a[2]; }
+
+/* equilibriumShakhov, discreteVelocity.C:1016-1044 */
+static void shakhov(const oracle *o, double rho, const double *U, double T, const double *q,
+                    const double *xi, double *geq, double *heq)
+{
+    const double R = o->R, Pr = o->Pr;
+    const int D = o->dim, K = o->K;
+    double Uxi[3] = {U[0] - xi[0], U[1] - xi[1], U[2] - xi[2]};
+    double xiU[3] = {xi[0] - U[0], xi[1] - U[1], xi[2] - U[2]};
+    double cSqrByRT = magsqr3(Uxi) / (R*T);
+    double cqBy5pRT = dot3(xiU, q) / (5.0*rho*R*T*R*T);
+    double gEqBGK = rho / pow(sqrt(2.0*PI_*R*T), (double)D) * exp(-cSqrByRT/2.0) / 1.0;
+    *geq = (1.0 + (1.0 - Pr)*cqBy5pRT*(cSqrByRT - D - 2.0))*gEqBGK;
+    *heq = ((K + 3.0 - D) + (1.0 - Pr)*cqBy5pRT*((cSqrByRT - D)*(K + 3.0 - D) - 2*K))*gEqBGK*R*T;
+}
+
+/* equilibriumMaxwellByRho, discreteVelocity.C:1063-1075 */
+static double maxwell_by_rho(const oracle *o, const double *U, double T, const double *xi)
+{
+    const double Ri = o->R;
+    double Uxi[3] = {U[0] - xi[0], U[1] - xi[1], U[2] - xi[2]};
+    return 1.0/pow(sqrt(2.0*PI_*Ri*T), (double)o->dim)*exp(-magsqr3(Uxi)/(2.0*Ri*T));
+}
+
+/* fvDVM::updateTau, fvDVM.C:808-817 */
+static inline double tau_of(const oracle *o, double mu_ref, double T, double rho)
+{
+    return mu_ref*exp(o->omega*log(T/o->Tref))/rho/T/o->R;
+}
+
+static inline double mu_of_cell(const oracle *o, int c)
+{
+    return o->cell_case ? o->mu_case[o->cell_case[c]] : o->mu_ref;
+}
+
+static inline double dt_of_cell(const oracle *o, int c)
+{
+    return o->cell_case ? o->dt_case[o->cell_case[c]] : o->dt_case[0];
+}
+
+/* ---- symmetric tensor helpers: (xx, xy, xz, yy, yz, zz) ---- */
+static void symm_inv(const double *s, double *r)
+{
+    /* det and cofactor inverse of a symmetric 3x3 tensor (OpenFOAM SymmTensorI.H form) */
+    double det = s[0]*s[3]*s[5] + s[1]*s[4]*s[2] + s[2]*s[1]*s[4]
+               - s[0]*s[4]*s[4] - s[1]*s[1]*s[5] - s[2]*s[3]*s[2];
+    r[0] = (s[3]*s[5] - s[4]*s[4])/det;
+    r[1] = (s[2]*s[4] - s[1]*s[5])/det;
+    r[2] = (s[1]*s[4] - s[2]*s[3])/det;
+    r[3] = (s[0]*s[5] - s[2]*s[2])/det;
+    r[4] = (s[1]*s[2] - s[0]*s[4])/det;
+    r[5] = (s[0]*s[3] - s[1]*s[1])/det;
+}
+
+static inline void symm_dot(const double *s, const double *d, double *r)
+{
+    r[0] = s[0]*d[0] + s[1]*d[1] + s[2]*d[2];
+    r[1] = s[1]*d[0] + s[3]*d[1] + s[4]*d[2];
+    r[2] = s[2]*d[0] + s[4]*d[1] + s[5]*d[2];
+}
+
+/* leastSquaresVectors (zeroBoundaryVectors.C:90-230 with the non-coupled
+ * boundary contributions of lines 154-166 enabled). */
+static void compute_lsq(oracle *o)
+{
+    const int nc = o->nc, nfi = o->nfi;
+    double *dd = (double *)xmalloc(sizeof(double)*6*nc);
+    for (int f = 0; f < nfi; ++f) {
+        int own = o->owner[f], nei = o->nei[f];
+        double d[3] = {o->C[3*nei] - o->C[3*own], o->C[3*nei+1] - o->C[3*own+1], o->C[3*nei+2] - o->C[3*own+2]};
+        double m = o->magSf[f]/magsqr3(d);
+        double wdd[6] = {m*(d[0]*d[0]), m*(d[0]*d[1]), m*(d[0]*d[2]), m*(d[1]*d[1]), m*(d[1]*d[2]), m*(d[2]*d[2])};
+        for (int i = 0; i < 6; ++i) {
+            dd[6*own + i] += (1 - o->w[f])*wdd[i];
+            dd[6*nei + i] += o->w[f]*wdd[i];
+        }
+    }
+    for (int b = 0; b < o->nb; ++b) {
+        int f = nfi + b, P = o->owner[f];
+        const double *d = o->delta + 3*b;
+        double m = (o->bkind[b] == DUGKS_PATCH_CYCLIC) ? (1 - o->w[f])*o->magSf[f]/magsqr3(d)
+                                                       : o->magSf[f]/magsqr3(d);
+        double s[6] = {d[0]*d[0], d[0]*d[1], d[0]*d[2], d[1]*d[1], d[1]*d[2], d[2]*d[2]};
+        for (int i = 0; i < 6; ++i) dd[6*P + i] += m*s[i];
+    }
+    /* inv(symmTensorField): components that vanish on the first cell are
+     * replaced by 1 before inversion and removed afterwards (2-D meshes). */
+    double scale = 0;
+    for (int i = 0; i < 6; ++i) scale += dd[i]*dd[i]*((i == 1 || i == 2 || i == 4) ? 2.0 : 1.0);
+    int rm[3] = {dd[0]*dd[0]/scale < 1e-15, dd[3]*dd[3]/scale < 1e-15, dd[5]*dd[5]/scale < 1e-15};
+    double *inv = (double *)xmalloc(sizeof(double)*6*nc);
+    for (int c = 0; c < nc; ++c) {
+        double s[6];
+        memcpy(s, dd + 6*c, sizeof s);
+        if (rm[0]) s[0] += 1; if (rm[1]) s[3] += 1; if (rm[2]) s[5] += 1;
+        symm_inv(s, inv + 6*c);
+        if (rm[0]) inv[6*c + 0] -= 1; if (rm[1]) inv[6*c + 3] -= 1; if (rm[2]) inv[6*c + 5] -= 1;
+    }
+    for (int f = 0; f < nfi; ++f) {
+        int own = o->owner[f], nei = o->nei[f];
+        double d[3] = {o->C[3*nei] - o->C[3*own], o->C[3*nei+1] - o->C[3*own+1], o->C[3*nei+2] - o->C[3*own+2]};
+        double m = o->magSf[f]/magsqr3(d);
+        double a[3], b[3];
+        symm_dot(inv + 6*own, d, a);
+        symm_dot(inv + 6*nei, d, b);
+        double fo = (1 - o->w[f])*m, fn = -o->w[f]*m;
+        for (int i = 0; i < 3; ++i) { o->pvec[3*f + i] = fo*a[i]; o->nvec[3*f + i] = fn*b[i]; }
+    }
+    for (int b = 0; b < o->nb; ++b) {
+        int f = nfi + b, P = o->owner[f];
+        const double *d = o->delta + 3*b;
+        double a[3];
+        symm_dot(inv + 6*P, d, a);
+        double m = (o->bkind[b] == DUGKS_PATCH_CYCLIC) ? ((1 - o->w[f])*o->magSf[f]/magsqr3(d))
+                                                       : o->magSf[f]*(1.0/magsqr3(d));
+        for (int i = 0; i < 3; ++i) o->pvec[3*f + i] = m*a[i];
+    }
+    free(dd);
+    free(inv);
+}
+
+void oracle_destroy(oracle *o);
+
+oracle *oracle_create(const dugks_case *cs)
+{
+    if (!cs || cs->n_cells <= 0 || cs->n_faces < cs->n_internal_faces) return NULL;
+    oracle *o = (oracle *)xmalloc(sizeof(oracle));
+    o->dim = cs->dim; o->nc = cs->n_cells; o->nfi = cs->n_internal_faces; o->nf = cs->n_faces;
+    o->nb = o->nf - o->nfi; o->nxi = cs->n_xi;
+    o->nv = cs->dim == 3 ? cs->n_xi*cs->n_xi*cs->n_xi : cs->n_xi*cs->n_xi;
+    o->R = cs->R; o->omega = cs->omega; o->Tref = cs->T_ref; o->mu_ref = cs->mu_ref; o->Pr = cs->Pr;
+    o->K = cs->K; o->xi_max = cs->xi_max;
+    o->nthreads = 1;
+    const int nc = o->nc, nf = o->nf, nfi = o->nfi, nb = o->nb, nv = o->nv;
+    o->owner = (int32_t *)xmalloc(sizeof(int32_t)*nf); memcpy(o->owner, cs->owner, sizeof(int32_t)*nf);
+    o->nei = (int32_t *)xmalloc(sizeof(int32_t)*(nfi + 1)); memcpy(o->nei, cs->neighbour, sizeof(int32_t)*nfi);
+#define CP(dst, src, n) do { dst = (double *)xmalloc(sizeof(double)*(n)); memcpy(dst, src, sizeof(double)*(n)); } while (0)
+    CP(o->C, cs->C, 3*nc); CP(o->V, cs->V, nc); CP(o->Sf, cs->Sf, 3*nf); CP(o->Cf, cs->Cf, 3*nf);
+    CP(o->w, cs->weights, nf); CP(o->delta, cs->delta, 3*nb);
+    o->magSf = (double *)xmalloc(sizeof(double)*nf);
+    for (int f = 0; f < nf; ++f) o->magSf[f] = sqrt(magsqr3(o->Sf + 3*f));
+    o->bkind = (int *)xmalloc(sizeof(int)*(nb + 1)); o->bpatch = (int *)xmalloc(sizeof(int)*(nb + 1));
+    o->bpartner = (int *)xmalloc(sizeof(int)*(nb + 1));
+    o->Ub = (double *)xmalloc(sizeof(double)*3*(nb + 1)); o->Tb = (double *)xmalloc(sizeof(double)*(nb + 1));
+    o->nhat = (double *)xmalloc(sizeof(double)*3*(nb + 1)); o->dcoef = (double *)xmalloc(sizeof(double)*(nb + 1));
+    o->n_patches = cs->n_patches;
+    o->pstart = (int *)xmalloc(sizeof(int)*(cs->n_patches + 1));
+    o->psize = (int *)xmalloc(sizeof(int)*(cs->n_patches + 1));
+    o->ptype = (int *)xmalloc(sizeof(int)*(cs->n_patches + 1));
+    for (int b = 0; b < nb; ++b) o->bkind[b] = 0;
+    for (int p = 0; p < cs->n_patches; ++p) {
+        const dugks_patch *pp = cs->patches + p;
+        o->pstart[p] = pp->start; o->psize[p] = pp->size; o->ptype[p] = pp->type;
+        for (int i = 0; i < pp->size; ++i) {
+            int b = pp->start - nfi + i;
+            o->bkind[b] = pp->type;
+            o->bpatch[b] = p;
+            if (pp->type == DUGKS_PATCH_CYCLIC) {
+                o->bpartner[b] = cs->patches[pp->neighbour_patch].start - nfi + i;
+            } else {
+                o->bpartner[b] = -1;
+                for (int d = 0; d < 3; ++d) o->Ub[3*b + d] = pp->U ? pp->U[3*i + d] : 0.0;
+                o->Tb[b] = pp->T[i];
+            }
+        }
+    }
+    for (int b = 0; b < nb; ++b) {
+        if (o->bkind[b] == 0) { oracle_destroy(o); return NULL; }
+        int f = nfi + b;
+        for (int d = 0; d < 3; ++d) o->nhat[3*b + d] = o->Sf[3*f + d]/o->magSf[f];
+        o->dcoef[b] = 1.0/sqrt(magsqr3(o->delta + 3*b));
+    }
+    o->pvec = (double *)xmalloc(sizeof(double)*3*nf);
+    o->nvec = (double *)xmalloc(sizeof(double)*3*(nfi + 1));
+    compute_lsq(o);
+    /* cell CSR, faces ascending */
+    o->cstart = (int *)xmalloc(sizeof(int)*(nc + 1));
+    for (int f = 0; f < nf; ++f) o->cstart[o->owner[f] + 1]++;
+    for (int f = 0; f < nfi; ++f) o->cstart[o->nei[f] + 1]++;
+    for (int c = 0; c < nc; ++c) o->cstart[c + 1] += o->cstart[c];
+    o->cface = (int *)xmalloc(sizeof(int)*(o->cstart[nc] + 1));
+    int *fill = (int *)xmalloc(sizeof(int)*nc);
+    for (int f = 0; f < nf; ++f) {
+        int a = o->owner[f];
+        o->cface[o->cstart[a] + fill[a]++] = f;
+        if (f < nfi) { int b = o->nei[f]; o->cface[o->cstart[b] + fill[b]++] = f; }
+    }
+    free(fill);
+    /* velocities, fvDVM.C:140-221 */
+    o->xi = (double *)xmalloc(sizeof(double)*3*nv); o->wv = (double *)xmalloc(sizeof(double)*nv);
+    {
+        const int n = o->nxi; const double *X = cs->xi, *W = cs->xi_weights;
+        int i = 0;
+        if (o->dim == 3) {
+            for (int iz = 0; iz < n; ++iz) for (int iy = 0; iy < n; ++iy) for (int ix = 0; ix < n; ++ix) {
+                o->wv[i] = W[iz]*W[iy]*W[ix];
+                o->xi[3*i] = X[ix]; o->xi[3*i + 1] = X[iy]; o->xi[3*i + 2] = X[iz];
+                ++i;
+            }
+        } else {
+            for (int iy = 0; iy < n; ++iy) for (int ix = 0; ix < n; ++ix) {
+                o->wv[i] = W[iy]*W[ix]*1;
+                o->xi[3*i] = X[ix]; o->xi[3*i + 1] = X[iy]; o->xi[3*i + 2] = 0.0;
+                ++i;
+            }
+        }
+    }
+    o->n_cases = cs->n_cases > 1 ? cs->n_cases : 1;
+    o->dt_case = (double *)xmalloc(sizeof(double)*o->n_cases);
+    if (cs->n_cases > 1 && cs->cell_case) {
+        o->cell_case = (int *)xmalloc(sizeof(int)*nc);
+        for (int c = 0; c < nc; ++c) o->cell_case[c] = cs->cell_case[c];
+        o->mu_case = (double *)xmalloc(sizeof(double)*o->n_cases);
+        for (int i = 0; i < o->n_cases; ++i) o->mu_case[i] = cs->mu_ref_case ? cs->mu_ref_case[i] : cs->mu_ref;
+    }
+    /* state */
+    size_t nvc = (size_t)nv*nc, nvb = (size_t)nv*(nb ? nb : 1);
+    o->gt = (double *)xmalloc(sizeof(double)*nvc); o->ht = (double *)xmalloc(sizeof(double)*nvc);
+    o->gradb_g = (double *)xmalloc(sizeof(double)*nvb); o->gradb_h = (double *)xmalloc(sizeof(double)*nvb);
+    if (!o->gt || !o->ht || !o->gradb_g || !o->gradb_h) { oracle_destroy(o); return NULL; }
+    o->rho = (double *)xmalloc(sizeof(double)*nc); o->U = (double *)xmalloc(sizeof(double)*3*nc);
+    o->T = (double *)xmalloc(sizeof(double)*nc); o->q = (double *)xmalloc(sizeof(double)*3*nc);
+    o->tau = (double *)xmalloc(sizeof(double)*nc);
+    o->incoming = (double *)xmalloc(sizeof(double)*(nb + 1)); o->rho_w = (double *)xmalloc(sizeof(double)*(nb + 1));
+    o->rho_f = (double *)xmalloc(sizeof(double)*nf); o->U_f = (double *)xmalloc(sizeof(double)*3*nf);
+    o->T_f = (double *)xmalloc(sizeof(double)*nf); o->q_f = (double *)xmalloc(sizeof(double)*3*nf);
+    o->tau_f = (double *)xmalloc(sizeof(double)*nf);
+    o->qwall = (double *)xmalloc(sizeof(double)*3*(nb + 1)); o->stresswall = (double *)xmalloc(sizeof(double)*9*(nb + 1));
+    o->gbar = (double *)xmalloc(sizeof(double)*nc); o->hbar = (double *)xmalloc(sizeof(double)*nc);
+    o->phib_g = (double *)xmalloc(sizeof(double)*(nb + 1)); o->phib_h = (double *)xmalloc(sizeof(double)*(nb + 1));
+    o->grad_g = (double *)xmalloc(sizeof(double)*3*nc); o->grad_h = (double *)xmalloc(sizeof(double)*3*nc);
+    o->bgrad_g = (double *)xmalloc(sizeof(double)*3*(nb + 1)); o->bgrad_h = (double *)xmalloc(sizeof(double)*3*(nb + 1));
+    o->gs = (double *)xmalloc(sizeof(double)*nf); o->hs = (double *)xmalloc(sizeof(double)*nf);
+    o->newgb_g = (double *)xmalloc(sizeof(double)*(nb + 1)); o->newgb_h = (double *)xmalloc(sizeof(double)*(nb + 1));
+    o->outgoing = (double *)xmalloc(sizeof(double)*(nb + 1));
+    o->acc_rho = (double *)xmalloc(sizeof(double)*nf); o->acc_rhoU = (double *)xmalloc(sizeof(double)*3*nf);
+    o->acc_rhoE = (double *)xmalloc(sizeof(double)*nf);
+    o->cacc_rho = (double *)xmalloc(sizeof(double)*nc); o->cacc_rhoU = (double *)xmalloc(sizeof(double)*3*nc);
+    o->cacc_rhoE = (double *)xmalloc(sizeof(double)*nc);
+    return o;
+}
+
+void oracle_destroy(oracle *o)
+{
+    if (!o) return;
+    void *ptrs[] = {o->owner, o->nei, o->C, o->V, o->Sf, o->Cf, o->w, o->delta, o->magSf, o->bkind, o->bpatch,
+                    o->bpartner, o->Ub, o->Tb, o->nhat, o->dcoef, o->pstart, o->psize, o->ptype, o->pvec, o->nvec,
+                    o->cstart, o->cface, o->xi, o->wv, o->cell_case, o->mu_case, o->dt_case, o->gt, o->ht,
+                    o->gradb_g, o->gradb_h, o->rho, o->U, o->T, o->q, o->tau, o->incoming, o->rho_w, o->rho_f,
+                    o->U_f, o->T_f, o->q_f, o->tau_f, o->qwall, o->stresswall, o->gbar, o->hbar, o->phib_g,
+                    o->phib_h, o->grad_g, o->grad_h, o->gs, o->hs, o->bgrad_g, o->bgrad_h, o->newgb_g,
+                    o->newgb_h, o->cache_g, o->cache_h, o->outgoing, o->acc_rho, o->acc_rhoU, o->acc_rhoE,
+                    o->cacc_rho, o->cacc_rhoU, o->cacc_rhoE};
+    for (size_t i = 0; i < sizeof ptrs/sizeof ptrs[0]; ++i) free(ptrs[i]);
+    free(o);
+}
+
+void oracle_set_threads(oracle *o, int n) { o->nthreads = n > 0 ? n : 1; }
+
+/* Holds the pre-S4 face values of every k between passes instead of
+ * recomputing them (same arithmetic; more memory, less time). */
+int oracle_set_cache_faces(oracle *o, int on)
+{
+    free(o->cache_g); free(o->cache_h); o->cache_g = o->cache_h = NULL; o->cache_faces = 0;
+    if (!on) return 0;
+    size_t n = (size_t)o->nv*o->nf;
+    o->cache_g = (double *)malloc(sizeof(double)*n);
+    o->cache_h = (double *)malloc(sizeof(double)*n);
+    if (!o->cache_g || !o->cache_h) { free(o->cache_g); free(o->cache_h); o->cache_g = o->cache_h = NULL; return -1; }
+    o->cache_faces = 1;
+    return 0;
+}
+
+int oracle_init_equilibrium(oracle *o, const double *rho, const double *U, const double *T)
+{
+    const int nc = o->nc, nv = o->nv, nb = o->nb;
+    for (int c = 0; c < nc; ++c) {
+        o->rho[c] = rho[c]; o->T[c] = T[c];
+        for (int d = 0; d < 3; ++d) { o->U[3*c + d] = U[3*c + d]; o->q[3*c + d] = 0.0; }
+        o->tau[c] = tau_of(o, mu_of_cell(o, c), o->T[c], o->rho[c]);
+    }
+    /* initDFtoEq: Shakhov with q = 0 */
+    const double q0[3] = {0, 0, 0};
+    for (int k = 0; k < nv; ++k)
+        for (int c = 0; c < nc; ++c)
+            shakhov(o, o->rho[c], o->U + 3*c, o->T[c], q0, o->xi + 3*k,
+                    o->gt + (size_t)k*nc + c, o->ht + (size_t)k*nc + c);
+    memset(o->gradb_g, 0, sizeof(double)*(size_t)nv*(nb ? nb : 1));
+    memset(o->gradb_h, 0, sizeof(double)*(size_t)nv*(nb ? nb : 1));
+    /* setCalculatedMaxwellRhoBC, fvDVM.C:263-309 */
+    for (int b = 0; b < nb; ++b) {
+        o->incoming[b] = 0; o->rho_w[b] = 1.0;
+        if (o->bkind[b] != DUGKS_PATCH_WALL) continue;
+        const double *S = o->Sf + 3*(o->nfi + b);
+        for (int k = 0; k < nv; ++k) {
+            const double *xi = o->xi + 3*k;
+            double s = dot3(xi, S);
+            if (s < 0)
+                o->incoming[b] += -o->wv[k]*s*maxwell_by_rho(o, o->Ub + 3*b, o->Tb[b], xi);
+        }
+    }
+    /* Usurf = fvc::interpolate(U, "linear") for the first Courant number */
+    for (int f = 0; f < o->nfi; ++f)
+        for (int d = 0; d < 3; ++d)
+            o->U_f[3*f + d] = o->w[f]*o->U[3*o->owner[f] + d] + (1 - o->w[f])*o->U[3*o->nei[f] + d];
+    for (int b = 0; b < nb; ++b) {
+        int f = o->nfi + b;
+        for (int d = 0; d < 3; ++d) {
+            if (o->bkind[b] == DUGKS_PATCH_CYCLIC) {
+                int N = o->owner[o->nfi + o->bpartner[b]];
+                o->U_f[3*f + d] = o->w[f]*o->U[3*o->owner[f] + d] + (1 - o->w[f])*o->U[3*N + d];
+            } else {
+                o->U_f[3*f + d] = o->Ub[3*b + d];
+            }
+        }
+    }
+    o->initialised = 1;
+    return 0;
+}
+
+/* One k: S1 (gBarPvol) for all cells, boundary values, S2 gradient and the
+ * pre-S4 face values.  Writes gbar/hbar, grad, gs/hs and the new
+ * fixedGradient values newgb_*. */
+static void compute_k(oracle *o, int k, int want_faces)
+{
+    const int nc = o->nc, nfi = o->nfi, nf = o->nf, nb = o->nb;
+    const double *xi = o->xi + 3*k;
+    const double *gt = o->gt + (size_t)k*nc, *ht = o->ht + (size_t)k*nc;
+    const double *gbk = o->gradb_g + (size_t)k*nb, *hbk = o->gradb_h + (size_t)k*nb;
+
+    /* S1, discreteVelocity.C:393-406 */
+    #pragma omp for schedule(static)
+    for (int c = 0; c < nc; ++c) {
+        double dt = dt_of_cell(o, c);
+        double rf = 1.5*dt/(2.0*o->tau[c] + dt);
+        double geq, heq;
+        shakhov(o, o->rho[c], o->U + 3*c, o->T[c], o->q + 3*c, xi, &geq, &heq);
+        o->gbar[c] = (1.0 - rf)*gt[c] + rf*geq;
+        o->hbar[c] = (1.0 - rf)*ht[c] + rf*heq;
+    }
+    /* correctBoundaryConditions: fixedGradient walls, coupled cyclic */
+    #pragma omp for schedule(static)
+    for (int b = 0; b < nb; ++b) {
+        int f = nfi + b, P = o->owner[f];
+        if (o->bkind[b] == DUGKS_PATCH_WALL) {
+            o->phib_g[b] = o->gbar[P] + gbk[b]/o->dcoef[b];
+            o->phib_h[b] = o->hbar[P] + hbk[b]/o->dcoef[b];
+        } else { /* cyclic */
+            int N = o->owner[nfi + o->bpartner[b]];
+            o->phib_g[b] = o->w[f]*o->gbar[P] + (1.0 - o->w[f])*o->gbar[N];
+            o->phib_h[b] = o->w[f]*o->hbar[P] + (1.0 - o->w[f])*o->hbar[N];
+        }
+    }
+    /* S2 gradient: leastSquaresGrad; per cell in face order */
+    #pragma omp for schedule(static)
+    for (int c = 0; c < nc; ++c) {
+        double g[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+        for (int e = o->cstart[c]; e < o->cstart[c + 1]; ++e) {
+            int f = o->cface[e];
+            if (f < nfi) {
+                int own = o->owner[f], nei = o->nei[f];
+                double dg = o->gbar[nei] - o->gbar[own];
+                double dh = o->hbar[nei] - o->hbar[own];
+                if (own == c) {
+                    const double *L = o->pvec + 3*f;
+                    for (int d = 0; d < 3; ++d) { g[d] += L[d]*dg; h[d] += L[d]*dh; }
+                } else {
+                    const double *L = o->nvec + 3*f;
+                    for (int d = 0; d < 3; ++d) { g[d] -= L[d]*dg; h[d] -= L[d]*dh; }
+                }
+            } else {
+                int b = f - nfi;
+                const double *L = o->pvec + 3*f;
+                double dg, dh;
+                if (o->bkind[b] == DUGKS_PATCH_CYCLIC) {
+                    int N = o->owner[nfi + o->bpartner[b]];
+                    dg = o->gbar[N] - o->gbar[c];
+                    dh = o->hbar[N] - o->hbar[c];
+                } else {
+                    dg = o->phib_g[b] - o->gbar[c];
+                    dh = o->phib_h[b] - o->hbar[c];
+                }
+                for (int d = 0; d < 3; ++d) { g[d] += L[d]*dg; h[d] += L[d]*dh; }
+            }
+        }
+        for (int d = 0; d < 3; ++d) { o->grad_g[3*c + d] = g[d]; o->grad_h[3*c + d] = h[d]; }
+    }
+    /* gradient boundary values and the new fixedGradient gradient
+     * (discreteVelocity.C:427-470) */
+    #pragma omp for schedule(static)
+    for (int b = 0; b < nb; ++b) {
+        int f = nfi + b, P = o->owner[f];
+        if (o->bkind[b] == DUGKS_PATCH_WALL) {
+            o->newgb_g[b] = dot3(o->grad_g + 3*P, o->nhat + 3*b);
+            o->newgb_h[b] = dot3(o->grad_h + 3*P, o->nhat + 3*b);
+        } else {
+            int N = o->owner[nfi + o->bpartner[b]];
+            for (int d = 0; d < 3; ++d) {
+                o->bgrad_g[3*b + d] = o->w[f]*o->grad_g[3*P + d] + (1.0 - o->w[f])*o->grad_g[3*N + d];
+                o->bgrad_h[3*b + d] = o->w[f]*o->grad_h[3*P + d] + (1.0 - o->w[f])*o->grad_h[3*N + d];
+            }
+        }
+    }
+    if (!want_faces) return;
+    const double dt0 = o->dt_case[0];
+    /* S2 face values, discreteVelocity.C:490-690 */
+    #pragma omp for schedule(static)
+    for (int f = 0; f < nf; ++f) {
+        int own = o->owner[f];
+        double dt = o->cell_case ? o->dt_case[o->cell_case[own]] : dt0;
+        double hx[3] = {0.5*xi[0]*dt, 0.5*xi[1]*dt, 0.5*xi[2]*dt};
+        const double *Cf = o->Cf + 3*f;
+        double s = dot3(xi, o->Sf + 3*f);
+        double dO[3] = {Cf[0] - o->C[3*own] - hx[0], Cf[1] - o->C[3*own + 1] - hx[1], Cf[2] - o->C[3*own + 2] - hx[2]};
+        if (f < nfi) {
+            int nei = o->nei[f];
+            double dN[3] = {Cf[0] - o->C[3*nei] - hx[0], Cf[1] - o->C[3*nei + 1] - hx[1], Cf[2] - o->C[3*nei + 2] - hx[2]};
+            if (s >= VSMALL) {
+                o->gs[f] = o->gbar[own] + dot3(o->grad_g + 3*own, dO);
+                o->hs[f] = o->hbar[own] + dot3(o->grad_h + 3*own, dO);
+            } else if (s < -VSMALL) {
+                o->gs[f] = o->gbar[nei] + dot3(o->grad_g + 3*nei, dN);
+                o->hs[f] = o->hbar[nei] + dot3(o->grad_h + 3*nei, dN);
+            } else {
+                o->gs[f] = 0.5*(o->gbar[nei] + dot3(o->grad_g + 3*nei, dN) + o->gbar[own] + dot3(o->grad_g + 3*own, dO));
+                o->hs[f] = 0.5*(o->hbar[nei] + dot3(o->grad_h + 3*nei, dN) + o->hbar[own] + dot3(o->grad_h + 3*own, dO));
+            }
+        } else {
+            int b = f - nfi;
+            if (o->bkind[b] == DUGKS_PATCH_WALL) {
+                if (s > 0) {
+                    o->gs[f] = o->gbar[own] + dot3(o->grad_g + 3*own, dO);
+                    o->hs[f] = o->hbar[own] + dot3(o->grad_h + 3*own, dO);
+                } else {
+                    o->gs[f] = 0.0; o->hs[f] = 0.0; /* set by S4 */
+                }
+            } else { /* cyclic, discreteVelocity.C:628-672 */
+                /* mesh C boundary values are the face centres */
+                double dB[3] = {Cf[0] - Cf[0] - hx[0], Cf[1] - Cf[1] - hx[1], Cf[2] - Cf[2] - hx[2]};
+                if (s > VSMALL) {
+                    o->gs[f] = o->gbar[own] + dot3(o->grad_g + 3*own, dO);
+                    o->hs[f] = o->hbar[own] + dot3(o->grad_h + 3*own, dO);
+                } else if (s < -VSMALL) {
+                    o->gs[f] = o->phib_g[b] + dot3(o->bgrad_g + 3*b, dB);
+                    o->hs[f] = o->phib_h[b] + dot3(o->bgrad_h + 3*b, dB);
+                } else {
+                    o->gs[f] = 0.5*(o->gbar[own] + dot3(o->grad_g + 3*own, dO) + o->phib_g[b] + dot3(o->bgrad_g + 3*b, dB));
+                    o->hs[f] = 0.5*(o->hbar[own] + dot3(o->grad_h + 3*own, dO) + o->phib_h[b] + dot3(o->bgrad_h + 3*b, dB));
+                }
+            }
+        }
+    }
+}
+
+/* S4 on wall faces (discreteVelocity.C:693-731) applied to the face values of k. */
+static void apply_wall_in(oracle *o, int k, double *gs, double *hs)
+{
+    const int nfi = o->nfi;
+    const double *xi = o->xi + 3*k;
+    #pragma omp for schedule(static)
+    for (int b = 0; b < o->nb; ++b) {
+        if (o->bkind[b] != DUGKS_PATCH_WALL) continue;
+        int f = nfi + b;
+        if (dot3(xi, o->Sf + 3*f) <= 0) {
+            gs[f] = o->rho_w[b]*maxwell_by_rho(o, o->Ub + 3*b, o->Tb[b], xi);
+            hs[f] = gs[f]*(o->R*o->Tb[b])*(o->K + 3 - o->dim);
+        }
+    }
+}
+
+/* face values of k after S4, recomputed or from the cache */
+static void faces_k(oracle *o, int k, double **pg, double **ph)
+{
+    if (o->cache_faces) {
+        double *cg = o->cache_g + (size_t)k*o->nf, *ch = o->cache_h + (size_t)k*o->nf;
+        #pragma omp for schedule(static)
+        for (int f = 0; f < o->nf; ++f) { o->gs[f] = cg[f]; o->hs[f] = ch[f]; }
+    } else {
+        compute_k(o, k, 1);
+    }
+    apply_wall_in(o, k, o->gs, o->hs);
+    *pg = o->gs; *ph = o->hs;
+}
+
+static int do_step(oracle *o)
+{
+    const int nc = o->nc, nfi = o->nfi, nf = o->nf, nb = o->nb, nv = o->nv;
+    const int D = o->dim, K = o->K;
+    int bad = 0;
+    int nth = o->nthreads;
+    (void)nth;
+    #pragma omp parallel num_threads(nth)
+    {
+        /* ---- pass A: S1, S2 and the wall outgoing flux (discreteVelocity.C:605-627) ---- */
+        #pragma omp for schedule(static)
+        for (int b = 0; b < nb; ++b) o->outgoing[b] = 0;
+        for (int k = 0; k < nv; ++k) {
+            compute_k(o, k, 1);
+            const double *xi = o->xi + 3*k;
+            #pragma omp for schedule(static)
+            for (int b = 0; b < nb; ++b) {
+                if (o->bkind[b] != DUGKS_PATCH_WALL) continue;
+                int f = nfi + b;
+                double s = dot3(xi, o->Sf + 3*f);
+                if (s > 0) o->outgoing[b] += o->wv[k]*s*o->gs[f];
+            }
+            if (o->cache_faces) {
+                double *cg = o->cache_g + (size_t)k*nf, *ch = o->cache_h + (size_t)k*nf;
+                #pragma omp for schedule(static)
+                for (int f = 0; f < nf; ++f) { cg[f] = o->gs[f]; ch[f] = o->hs[f]; }
+            }
+        }
+        /* ---- S3: calculatedMaxwell evaluate ---- */
+        #pragma omp for schedule(static)
+        for (int b = 0; b < nb; ++b)
+            if (o->bkind[b] == DUGKS_PATCH_WALL) o->rho_w[b] = o->outgoing[b]/fabs(o->incoming[b]);
+        /* ---- pass B: S4 + conserved face moments (fvDVM.C:473-483) ---- */
+        #pragma omp for schedule(static)
+        for (int f = 0; f < nf; ++f) {
+            o->acc_rho[f] = 0; o->acc_rhoE[f] = 0;
+            o->acc_rhoU[3*f] = o->acc_rhoU[3*f + 1] = o->acc_rhoU[3*f + 2] = 0;
+        }
+        for (int k = 0; k < nv; ++k) {
+            double *gs, *hs;
+            faces_k(o, k, &gs, &hs);
+            const double *xi = o->xi + 3*k;
+            const double wk = o->wv[k];
+            const double xi2 = magsqr3(xi);
+            #pragma omp for schedule(static)
+            for (int f = 0; f < nf; ++f) {
+                o->acc_rho[f] += wk*gs[f];
+                for (int d = 0; d < 3; ++d) o->acc_rhoU[3*f + d] += wk*gs[f]*xi[d];
+                o->acc_rhoE[f] += 0.5*wk*(gs[f]*xi2 + hs[f]);
+            }
+        }
+        /* ---- S6a: Usurf, Tsurf, tauSurf (fvDVM.C:493-497) ---- */
+        #pragma omp for schedule(static)
+        for (int f = 0; f < nf; ++f) {
+            double r = o->acc_rho[f];
+            o->rho_f[f] = r;
+            for (int d = 0; d < 3; ++d) o->U_f[3*f + d] = o->acc_rhoU[3*f + d]/r;
+            o->T_f[f] = (o->acc_rhoE[f] - 0.5*r*magsqr3(o->U_f + 3*f))/((K + 3)/2.0*o->R*r);
+            int own = o->owner[f];
+            o->tau_f[f] = tau_of(o, mu_of_cell(o, own), o->T_f[f], r);
+            o->q_f[3*f] = o->q_f[3*f + 1] = o->q_f[3*f + 2] = 0;
+        }
+        #pragma omp for schedule(static)
+        for (int b = 0; b < nb; ++b) {
+            for (int d = 0; d < 3; ++d) o->qwall[3*b + d] = 0;
+            for (int d = 0; d < 9; ++d) o->stresswall[9*b + d] = 0;
+        }
+        /* ---- pass C: face heat flux (fvDVM.C:503-516) and wall diagnostics (555-570) ---- */
+        for (int k = 0; k < nv; ++k) {
+            double *gs, *hs;
+            faces_k(o, k, &gs, &hs);
+            const double *xi = o->xi + 3*k;
+            const double wk = o->wv[k];
+            #pragma omp for schedule(static)
+            for (int f = 0; f < nf; ++f) {
+                double c[3] = {xi[0] - o->U_f[3*f], xi[1] - o->U_f[3*f + 1], xi[2] - o->U_f[3*f + 2]};
+                double a = magsqr3(c)*gs[f] + hs[f];
+                for (int d = 0; d < 3; ++d) o->q_f[3*f + d] += 0.5*wk*c[d]*a;
+                if (f >= nfi && o->bkind[f - nfi] == DUGKS_PATCH_WALL) {
+                    int b = f - nfi;
+                    double cw[3] = {xi[0] - o->Ub[3*b], xi[1] - o->Ub[3*b + 1], xi[2] - o->Ub[3*b + 2]};
+                    double aw = magsqr3(cw)*gs[f] + hs[f];
+                    for (int d = 0; d < 3; ++d) o->qwall[3*b + d] += 0.5*1.0*wk*cw[d]*aw;
+                    double wg = 1.0*wk*gs[f];
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) o->stresswall[9*b + 3*i + j] += (wg*xi[i])*xi[j];
+                }
+            }
+        }
+        /* ---- S6b: q correction (fvDVM.C:522, 571-573) ---- */
+        #pragma omp for schedule(static)
+        for (int f = 0; f < nf; ++f) {
+            int own = o->owner[f];
+            double dt = dt_of_cell(o, own);
+            double t2 = 2.0*o->tau_f[f];
+            double fac = t2/(t2 + 0.5*dt*o->Pr);
+            for (int d = 0; d < 3; ++d) o->q_f[3*f + d] = fac*o->q_f[3*f + d];
+            if (f >= nfi && o->bkind[f - nfi] == DUGKS_PATCH_WALL) {
+                int b = f - nfi;
+                double tp = o->tau_f[f];
+                for (int d = 0; d < 3; ++d) o->qwall[3*b + d] = 2.0*tp/(2.0*tp + 0.5*dt*o->Pr)*o->qwall[3*b + d];
+                for (int d = 0; d < 9; ++d) o->stresswall[9*b + d] = 2.0*tp/(2.0*tp + 0.5*dt)*o->stresswall[9*b + d];
+            }
+        }
+        /* ---- pass D: S7 relaxation, S8 update of g~, conserved cell moments ---- */
+        #pragma omp for schedule(static)
+        for (int c = 0; c < nc; ++c) {
+            o->cacc_rho[c] = 0; o->cacc_rhoE[c] = 0;
+            o->cacc_rhoU[3*c] = o->cacc_rhoU[3*c + 1] = o->cacc_rhoU[3*c + 2] = 0;
+        }
+        for (int k = 0; k < nv; ++k) {
+            double *gs, *hs;
+            if (o->cache_faces) compute_k(o, k, 0);   /* gbar, new fixedGradient */
+            faces_k(o, k, &gs, &hs);
+            const double *xi = o->xi + 3*k;
+            const double wk = o->wv[k];
+            const double xi2 = magsqr3(xi);
+            /* S7, discreteVelocity.C:867-920 */
+            #pragma omp for schedule(static)
+            for (int f = 0; f < nf; ++f) {
+                int own = o->owner[f];
+                double dt = dt_of_cell(o, own);
+                double h = 0.5*dt;
+                double rf = h/(2*o->tau_f[f] + h);
+                double s = dot3(xi, o->Sf + 3*f);
+                int relax = (f < nfi) || o->bkind[f - nfi] == DUGKS_PATCH_CYCLIC || s > 0;
+                if (relax) {
+                    double geq, heq;
+                    shakhov(o, o->rho_f[f], o->U_f + 3*f, o->T_f[f], o->q_f + 3*f, xi, &geq, &heq);
+                    gs[f] = (1.0 - rf)*gs[f] + rf*geq;
+                    hs[f] = (1.0 - rf)*hs[f] + rf*heq;
+                }
+            }
+            /* S8, discreteVelocity.C:937-975; per cell in face order */
+            double *gt = o->gt + (size_t)k*nc, *ht = o->ht + (size_t)k*nc;
+            #pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                double dt = dt_of_cell(o, c);
+                double g = -1.0/3*gt[c] + 4.0/3*o->gbar[c];
+                double hh = -1.0/3*ht[c] + 4.0/3*o->hbar[c];
+                for (int e = o->cstart[c]; e < o->cstart[c + 1]; ++e) {
+                    int f = o->cface[e];
+                    double s = dot3(xi, o->Sf + 3*f);
+                    if (o->owner[f] == c) {
+                        g -= (s*gs[f]*dt/o->V[c]);
+                        hh -= (s*hs[f]*dt/o->V[c]);
+                    } else {
+                        g += (s*gs[f]*dt/o->V[c]);
+                        hh += (s*hs[f]*dt/o->V[c]);
+                    }
+                }
+                gt[c] = g; ht[c] = hh;
+                /* S9 conserved moments (fvDVM.C:612-622), k order */
+                o->cacc_rho[c] += wk*g;
+                for (int d = 0; d < 3; ++d) o->cacc_rhoU[3*c + d] += wk*g*xi[d];
+                o->cacc_rhoE[c] += 0.5*wk*(xi2*g + hh);
+            }
+            /* commit the new fixedGradient values of k */
+            double *gbk = o->gradb_g + (size_t)k*nb, *hbk = o->gradb_h + (size_t)k*nb;
+            #pragma omp for schedule(static)
+            for (int b = 0; b < nb; ++b) {
+                if (o->bkind[b] == DUGKS_PATCH_WALL) { gbk[b] = o->newgb_g[b]; hbk[b] = o->newgb_h[b]; }
+            }
+        }
+        /* ---- S9: primitive variables, tau (fvDVM.C:694-707) ---- */
+        #pragma omp for schedule(static) reduction(|:bad)
+        for (int c = 0; c < nc; ++c) {
+            double r = o->cacc_rho[c];
+            double rE = o->cacc_rhoE[c];
+            o->rho[c] = r;
+            for (int d = 0; d < 3; ++d) o->U[3*c + d] = o->cacc_rhoU[3*c + d]/r;
+            o->T[c] = (rE - 0.5*r*magsqr3(o->U + 3*c))/((K + 3)/2.0*o->R*r);
+            o->tau[c] = tau_of(o, mu_of_cell(o, c), o->T[c], r);
+            o->q[3*c] = o->q[3*c + 1] = o->q[3*c + 2] = 0;
+            if (!(r > 0) || !isfinite(o->T[c])) bad = 1;
+        }
+        /* ---- pass E: cell heat flux (fvDVM.C:712-721) ---- */
+        for (int k = 0; k < nv; ++k) {
+            const double *xi = o->xi + 3*k;
+            const double wk = o->wv[k];
+            const double *gt = o->gt + (size_t)k*nc, *ht = o->ht + (size_t)k*nc;
+            #pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                double cc[3] = {xi[0] - o->U[3*c], xi[1] - o->U[3*c + 1], xi[2] - o->U[3*c + 2]};
+                double a = magsqr3(cc)*gt[c] + ht[c];
+                for (int d = 0; d < 3; ++d) o->q[3*c + d] += 0.5*wk*cc[d]*a;
+            }
+        }
+        #pragma omp for schedule(static)
+        for (int c = 0; c < nc; ++c) {
+            double dt = dt_of_cell(o, c);
+            double fac = 2.0*o->tau[c]/(2.0*o->tau[c] + dt*o->Pr);
+            for (int d = 0; d < 3; ++d) o->q[3*c + d] = fac*o->q[3*c + d];
+        }
+    }
+    (void)D;
+    return bad ? DUGKS_ERR_NONFINITE : 0;
+}
+
+int oracle_step_cases(oracle *o, const double *dt_case, int nsteps)
+{
+    if (!o->initialised) return DUGKS_ERR_STATE;
+    for (int i = 0; i < o->n_cases; ++i) o->dt_case[i] = dt_case[i];
+    for (int s = 0; s < nsteps; ++s) {
+        int r = do_step(o);
+        if (r) return r;
+    }
+    return 0;
+}
+
+int oracle_step(oracle *o, double dt, int nsteps)
+{
+    if (!o->initialised) return DUGKS_ERR_STATE;
+    for (int i = 0; i < o->n_cases; ++i) o->dt_case[i] = dt;
+    for (int s = 0; s < nsteps; ++s) {
+        int r = do_step(o);
+        if (r) return r;
+    }
+    return 0;
+}
+
+int oracle_get_macro(oracle *o, double *rho, double *U, double *T, double *q)
+{
+    const int nc = o->nc;
+    if (rho) memcpy(rho, o->rho, sizeof(double)*nc);
+    if (U) memcpy(U, o->U, sizeof(double)*3*nc);
+    if (T) memcpy(T, o->T, sizeof(double)*nc);
+    if (q) memcpy(q, o->q, sizeof(double)*3*nc);
+    return 0;
+}
+
+int oracle_get_face_macro(oracle *o, double *rho, double *U, double *T, double *q)
+{
+    const int nf = o->nf;
+    if (rho) memcpy(rho, o->rho_f, sizeof(double)*nf);
+    if (U) memcpy(U, o->U_f, sizeof(double)*3*nf);
+    if (T) memcpy(T, o->T_f, sizeof(double)*nf);
+    if (q) memcpy(q, o->q_f, sizeof(double)*3*nf);
+    return 0;
+}
+
+int oracle_get_wall(oracle *o, int patch, double *rho_w, double *qw, double *sw)
+{
+    if (patch < 0 || patch >= o->n_patches) return DUGKS_ERR_INVALID;
+    int b0 = o->pstart[patch] - o->nfi, n = o->psize[patch];
+    if (rho_w) memcpy(rho_w, o->rho_w + b0, sizeof(double)*n);
+    if (qw) memcpy(qw, o->qwall + 3*b0, sizeof(double)*3*n);
+    if (sw) memcpy(sw, o->stresswall + 9*b0, sizeof(double)*9*n);
+    return 0;
+}
+
+int oracle_get_df(oracle *o, int n, const int32_t *cells, double *g, double *h)
+{
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < o->nv; ++k) {
+            if (g) g[(size_t)i*o->nv + k] = o->gt[(size_t)k*o->nc + cells[i]];
+            if (h) h[(size_t)i*o->nv + k] = o->ht[(size_t)k*o->nc + cells[i]];
+        }
+    return 0;
+}
+
+/* fvDVM::getCoNum, fvDVM.C:1111-1119 (internal faces) */
+int oracle_get_courant(oracle *o, double dt, double *co_max, double *co_mean)
+{
+    double mx = -1e300, sum = 0;
+    for (int f = 0; f < o->nfi; ++f) {
+        int own = o->owner[f], nei = o->nei[f];
+        double d[3] = {o->C[3*nei] - o->C[3*own], o->C[3*nei + 1] - o->C[3*own + 1], o->C[3*nei + 2] - o->C[3*own + 2]};
+        double dc = 1.0/sqrt(magsqr3(d));
+        double u = dc*(sqrt(magsqr3(o->U_f + 3*f)) + sqrt((double)o->dim)*o->xi_max);
+        if (u > mx) mx = u;
+        sum += u;
+    }
+    *co_max = mx*dt;
+    *co_mean = sum/o->nfi*dt;
+    return 0;
+}
+
+/* ---- exports for unit tests ---- */
+
+int oracle_lsq_vectors(oracle *o, double *pvec, double *nvec)
+{
+    if (pvec) memcpy(pvec, o->pvec, sizeof(double)*3*o->nf);
+    if (nvec) memcpy(nvec, o->nvec, sizeof(double)*3*o->nfi);
+    return 0;
+}
+
+/* Least-squares gradient of a cell field phi with the given values phib on
+ * every boundary face (cyclic faces use the neighbour cell value). */
+int oracle_lsq_grad(oracle *o, const double *phi, const double *phib, double *grad)
+{
+    for (int c = 0; c < o->nc; ++c) {
+        double g[3] = {0, 0, 0};
+        for (int e = o->cstart[c]; e < o->cstart[c + 1]; ++e) {
+            int f = o->cface[e];
+            if (f < o->nfi) {
+                double dphi = phi[o->nei[f]] - phi[o->owner[f]];
+                if (o->owner[f] == c) for (int d = 0; d < 3; ++d) g[d] += o->pvec[3*f + d]*dphi;
+                else for (int d = 0; d < 3; ++d) g[d] -= o->nvec[3*f + d]*dphi;
+            } else {
+                int b = f - o->nfi;
+                double pb = o->bkind[b] == DUGKS_PATCH_CYCLIC ? phi[o->owner[o->nfi + o->bpartner[b]]] : phib[b];
+                for (int d = 0; d < 3; ++d) g[d] += o->pvec[3*f + d]*(pb - phi[c]);
+            }
+        }
+        for (int d = 0; d < 3; ++d) grad[3*c + d] = g[d];
+    }
+    return 0;
+}
+
+void oracle_shakhov(oracle *o, double rho, const double *U, double T, const double *q, const double *xi,
+                    double *geq, double *heq)
+{
+    shakhov(o, rho, U, T, q, xi, geq, heq);
+}
+
+double oracle_maxwell_by_rho(oracle *o, const double *U, double T, const double *xi)
+{
+    return maxwell_by_rho(o, U, T, xi);
+}
+
+double oracle_tau(oracle *o, double T, double rho) { return tau_of(o, o->mu_ref, T, rho); }
+
+int oracle_velocities(oracle *o, double *xi, double *w)
+{
+    if (xi) memcpy(xi, o->xi, sizeof(double)*3*o->nv);
+    if (w) memcpy(w, o->wv, sizeof(double)*o->nv);
+    return o->nv;
+}
+
+int oracle_incoming(oracle *o, double *incoming)
+{
+    memcpy(incoming, o->incoming, sizeof(double)*o->nb);
+    return 0;
+}
+
+/* Sum over cells of V * rho: total mass. */
+double oracle_total_mass(oracle *o)
+{
+    double m = 0;
+    for (int c = 0; c < o->nc; ++c) m += o->V[c]*o->rho[c];
+    return m;
+}

@@ -1,0 +1,234 @@
+"""Second FP64 restatement of fvDVM::evolution(), vectorised with numpy.
+
+Written directly from the reference (src/fvDVM/fvDVM/fvDVM.C and
+src/fvDVM/discreteVelocity/discreteVelocity.C) in its own materialised form:
+every per-velocity field ([Nv, cells] / [Nv, faces]) is held at once, as the
+reference's DV_ list holds them.  Used only to cross-check the C oracle on
+tiny cases; structure, loops and reductions are deliberately different.
+"""
+import math
+
+import numpy as np
+
+from dugksfoam_amd.cases import PATCH_CYCLIC, PATCH_WALL
+
+
+class NumpyDUGKS:
+    def __init__(self, c):
+        self.c = c
+        self.D = c.dim
+        nfi, nf = c.n_internal_faces, c.n_faces
+        self.nfi, self.nf, self.nb = nfi, nf, nf - nfi
+        self.own = c.owner.astype(np.int64)
+        self.nei = c.neighbour.astype(np.int64)
+        self.magSf = np.linalg.norm(c.Sf, axis=1)
+        self.xi, self.w = c.velocities()
+        # boundary classification
+        self.kind = np.zeros(self.nb, np.int64)
+        self.partner = np.full(self.nb, -1, np.int64)
+        self.Ub = np.zeros((self.nb, 3))
+        self.Tb = np.ones(self.nb)
+        for p in c.patches:
+            b = np.arange(p["start"], p["start"] + p["size"]) - nfi
+            self.kind[b] = p["type"]
+            if p["type"] == PATCH_WALL:
+                self.Ub[b] = p["U"]
+                self.Tb[b] = p["T"]
+            else:
+                q = c.patches[p["neighbour_patch"]]
+                self.partner[b] = np.arange(q["start"], q["start"] + q["size"]) - nfi
+        self.wall = np.nonzero(self.kind == PATCH_WALL)[0]
+        self.cyc = np.nonzero(self.kind == PATCH_CYCLIC)[0]
+        self.bP = self.own[nfi:]
+        self.bN = np.where(self.kind == PATCH_CYCLIC, self.own[nfi + np.maximum(self.partner, 0)], -1)
+        self.nhat = c.Sf[nfi:] / self.magSf[nfi:, None]
+        self.dcoef = 1.0 / np.linalg.norm(c.delta, axis=1)
+        self._lsq()
+        # state
+        nv = self.w.size
+        U0, T0, rho0 = c.U0, c.T0, c.rho0
+        self.rho, self.U, self.T = rho0.copy(), U0.copy(), T0.copy()
+        self.q = np.zeros_like(U0)
+        self.mu_c = c.mu_ref_case[c.cell_case] if c.n_cases > 1 else np.full(c.n_cells, c.mu_ref)
+        self.tau = self._tau(self.T, self.rho, self.mu_c)
+        self.gt, self.ht = self._shakhov(self.rho, self.U, self.T, self.q)
+        self.gradB_g = np.zeros((nv, self.nb))
+        self.gradB_h = np.zeros((nv, self.nb))
+        M = self._maxwell_by_rho(self.Ub, self.Tb)          # [nv, nb]
+        s = self.xi @ c.Sf[nfi:].T                           # [nv, nb]
+        self.incoming = np.sum(np.where(s < 0, -self.w[:, None] * s * M, 0.0), axis=0)
+
+    # ---- gas ----
+    def _tau(self, T, rho, mu):
+        c = self.c
+        return mu * np.exp(c.omega * np.log(T / c.T_ref)) / rho / T / c.R
+
+    def _shakhov(self, rho, U, T, q):
+        c, D = self.c, self.D
+        R, Pr, K = c.R, c.Pr, c.K
+        cu = U[None, :, :] - self.xi[:, None, :]                    # [nv, n, 3]
+        c2 = np.sum(cu * cu, axis=2) / (R * T)[None, :]
+        cq = np.sum(-cu * q[None, :, :], axis=2) / (5.0 * rho * R * T * R * T)[None, :]
+        gB = (rho / (2 * math.pi * R * T) ** (D / 2))[None, :] * np.exp(-c2 / 2)
+        g = (1 + (1 - Pr) * cq * (c2 - D - 2)) * gB
+        h = ((K + 3 - D) + (1 - Pr) * cq * ((c2 - D) * (K + 3 - D) - 2 * K)) * gB * (R * T)[None, :]
+        return g, h
+
+    def _maxwell_by_rho(self, U, T):
+        c, D = self.c, self.D
+        cu = U[None, :, :] - self.xi[:, None, :]
+        return (1.0 / (2 * math.pi * c.R * T) ** (D / 2))[None, :] * np.exp(
+            -np.sum(cu * cu, axis=2) / (2 * c.R * T)[None, :])
+
+    # ---- geometry: leastSquaresVectors ----
+    def _lsq(self):
+        c, nfi = self.c, self.nfi
+        nc = c.n_cells
+        d = c.C[self.nei] - c.C[self.own[:nfi]]
+        m = self.magSf[:nfi] / np.sum(d * d, axis=1)
+        dd = np.zeros((nc, 3, 3))
+        wdd = m[:, None, None] * np.einsum("fi,fj->fij", d, d)
+        np.add.at(dd, self.own[:nfi], (1 - c.weights[:nfi])[:, None, None] * wdd)
+        np.add.at(dd, self.nei, c.weights[:nfi, None, None] * wdd)
+        db = c.delta
+        wb = c.weights[nfi:]
+        mb = np.where(self.kind == PATCH_CYCLIC, (1 - wb), 1.0) * self.magSf[nfi:] / np.sum(db * db, axis=1)
+        np.add.at(dd, self.bP, mb[:, None, None] * np.einsum("fi,fj->fij", db, db))
+        if self.D == 2:
+            dd[:, 2, 2] += 1.0
+        inv = np.linalg.inv(dd)
+        if self.D == 2:
+            inv[:, 2, 2] -= 1.0
+        self.pvec = np.zeros((self.nf, 3))
+        self.nvec = np.zeros((nfi, 3))
+        self.pvec[:nfi] = ((1 - c.weights[:nfi]) * m)[:, None] * np.einsum("fij,fj->fi", inv[self.own[:nfi]], d)
+        self.nvec = (-c.weights[:nfi] * m)[:, None] * np.einsum("fij,fj->fi", inv[self.nei], d)
+        self.pvec[nfi:] = mb[:, None] * np.einsum("fij,fj->fi", inv[self.bP], db)
+
+    def _grad(self, phi, phib):
+        """phi [nv, nc], phib [nv, nb] boundary values (cyclic: neighbour cell values)."""
+        nfi = self.nfi
+        nv, nc = phi.shape
+        g = np.zeros((nc, nv, 3))
+        dphi = phi[:, self.nei] - phi[:, self.own[:nfi]]          # [nv, nfi]
+        np.add.at(g, self.own[:nfi], (dphi.T[:, :, None] * self.pvec[:nfi, None, :]))
+        np.add.at(g, self.nei, -(dphi.T[:, :, None] * self.nvec[:, None, :]))
+        db = phib - phi[:, self.bP]
+        np.add.at(g, self.bP, db.T[:, :, None] * self.pvec[nfi:, None, :])
+        return np.transpose(g, (1, 0, 2))                         # [nv, nc, 3]
+
+    # ---- one step ----
+    def step(self, dt):
+        c, D, nfi = self.c, self.D, self.nfi
+        xi, w = self.xi, self.w
+        R, Pr, K = c.R, c.Pr, c.K
+        # S1
+        r = 1.5 * dt / (2 * self.tau + dt)
+        geq, heq = self._shakhov(self.rho, self.U, self.T, self.q)
+        gbar = (1 - r) * self.gt + r * geq
+        hbar = (1 - r) * self.ht + r * heq
+        # boundary values of gBarPvol: fixedGradient on walls, coupled on cyclics
+        wb = c.weights[nfi:]
+        phib_g = np.empty((xi.shape[0], self.nb))
+        phib_h = np.empty_like(phib_g)
+        wl, cy = self.wall, self.cyc
+        phib_g[:, wl] = gbar[:, self.bP[wl]] + self.gradB_g[:, wl] / self.dcoef[wl]
+        phib_h[:, wl] = hbar[:, self.bP[wl]] + self.gradB_h[:, wl] / self.dcoef[wl]
+        lsq_b_g = phib_g.copy()
+        lsq_b_h = phib_h.copy()
+        lsq_b_g[:, cy] = gbar[:, self.bN[cy]]
+        lsq_b_h[:, cy] = hbar[:, self.bN[cy]]
+        phib_g[:, cy] = wb[cy] * gbar[:, self.bP[cy]] + (1 - wb[cy]) * gbar[:, self.bN[cy]]
+        phib_h[:, cy] = wb[cy] * hbar[:, self.bP[cy]] + (1 - wb[cy]) * hbar[:, self.bN[cy]]
+        # S2
+        Gg = self._grad(gbar, lsq_b_g)
+        Gh = self._grad(hbar, lsq_b_h)
+        newB_g = np.einsum("kbi,bi->kb", Gg[:, self.bP], self.nhat)
+        newB_h = np.einsum("kbi,bi->kb", Gh[:, self.bP], self.nhat)
+        hx = 0.5 * xi * dt                                        # [nv, 3]
+        s = xi @ c.Sf.T                                           # [nv, nf]
+
+        def recon(phi, G, cells, faces):
+            dC = c.Cf[faces][None, :, :] - c.C[cells][None, :, :] - hx[:, None, :]
+            return phi[:, cells] + np.sum(G[:, cells] * dC, axis=2)
+
+        o, n = self.own[:nfi], self.nei
+        fi = np.arange(nfi)
+        gs = np.zeros((xi.shape[0], self.nf))
+        hs = np.zeros_like(gs)
+        go, gn = recon(gbar, Gg, o, fi), recon(gbar, Gg, n, fi)
+        ho, hn = recon(hbar, Gh, o, fi), recon(hbar, Gh, n, fi)
+        si = s[:, :nfi]
+        gs[:, :nfi] = np.where(si >= 1e-300, go, np.where(si < -1e-300, gn, 0.5 * (gn + go)))
+        hs[:, :nfi] = np.where(si >= 1e-300, ho, np.where(si < -1e-300, hn, 0.5 * (hn + ho)))
+        bf = nfi + np.arange(self.nb)
+        gP, hP = recon(gbar, Gg, self.bP, bf), recon(hbar, Gh, self.bP, bf)
+        sb = s[:, nfi:]
+        outgoing = np.sum(np.where(sb > 0, w[:, None] * sb * gP, 0.0), axis=0)
+        rho_w = outgoing / np.abs(self.incoming)
+        M = self._maxwell_by_rho(self.Ub, self.Tb)
+        gin = rho_w[None, :] * M
+        hin = gin * (R * self.Tb)[None, :] * (K + 3 - D)
+        gb = np.where(sb > 0, gP, gin)
+        hb = np.where(sb > 0, hP, hin)
+        if cy.size:
+            Gb_g = wb[cy][None, :, None] * Gg[:, self.bP[cy]] + (1 - wb[cy])[None, :, None] * Gg[:, self.bN[cy]]
+            Gb_h = wb[cy][None, :, None] * Gh[:, self.bP[cy]] + (1 - wb[cy])[None, :, None] * Gh[:, self.bN[cy]]
+            inc_g = phib_g[:, cy] + np.sum(Gb_g * (-hx[:, None, :]), axis=2)
+            inc_h = phib_h[:, cy] + np.sum(Gb_h * (-hx[:, None, :]), axis=2)
+            sc = sb[:, cy]
+            gb[:, cy] = np.where(sc > 1e-300, gP[:, cy], np.where(sc < -1e-300, inc_g, 0.5 * (gP[:, cy] + inc_g)))
+            hb[:, cy] = np.where(sc > 1e-300, hP[:, cy], np.where(sc < -1e-300, inc_h, 0.5 * (hP[:, cy] + inc_h)))
+        gs[:, nfi:] = gb
+        hs[:, nfi:] = hb
+        # S6
+        xi2 = np.sum(xi * xi, axis=1)
+        rho_f = w @ gs
+        rhoU_f = np.einsum("k,kf,kd->fd", w, gs, xi)
+        rhoE_f = 0.5 * (w @ (gs * xi2[:, None] + hs))
+        U_f = rhoU_f / rho_f[:, None]
+        T_f = (rhoE_f - 0.5 * rho_f * np.sum(U_f ** 2, axis=1)) / ((K + 3) / 2 * R * rho_f)
+        mu_f = self.mu_c[self.own]
+        tau_f = self._tau(T_f, rho_f, mu_f)
+        cf = xi[:, None, :] - U_f[None, :, :]
+        q_f = 0.5 * np.einsum("k,kfd,kf->fd", w, cf, np.sum(cf ** 2, axis=2) * gs + hs)
+        q_f = (2 * tau_f / (2 * tau_f + 0.5 * dt * Pr))[:, None] * q_f
+        self.f_rho, self.f_U, self.f_T, self.f_q = rho_f, U_f, T_f, q_f
+        # S7
+        rf = 0.5 * dt / (2 * tau_f + 0.5 * dt)
+        gq, hq = self._shakhov(rho_f, U_f, T_f, q_f)
+        relax = np.ones_like(s, dtype=bool)
+        relax[:, nfi:] = (sb > 0) | (self.kind == PATCH_CYCLIC)[None, :]
+        gs = np.where(relax, (1 - rf) * gs + rf * gq, gs)
+        hs = np.where(relax, (1 - rf) * hs + rf * hq, hs)
+        # S8
+        gt = -1.0 / 3 * self.gt + 4.0 / 3 * gbar
+        ht = -1.0 / 3 * self.ht + 4.0 / 3 * hbar
+        V = c.V
+        fg = s * gs * dt
+        fh = s * hs * dt
+        gtT, htT = gt.T.copy(), ht.T.copy()
+        np.add.at(gtT, self.own, -(fg / V[self.own][None, :]).T)
+        np.add.at(gtT, self.nei, (fg[:, :nfi] / V[self.nei][None, :]).T)
+        np.add.at(htT, self.own, -(fh / V[self.own][None, :]).T)
+        np.add.at(htT, self.nei, (fh[:, :nfi] / V[self.nei][None, :]).T)
+        self.gt, self.ht = gtT.T, htT.T
+        self.gradB_g[:, wl] = newB_g[:, wl]
+        self.gradB_h[:, wl] = newB_h[:, wl]
+        # S9
+        rho = w @ self.gt
+        rhoU = np.einsum("k,kc,kd->cd", w, self.gt, xi)
+        rhoE = 0.5 * (w @ (xi2[:, None] * self.gt + self.ht))
+        U = rhoU / rho[:, None]
+        T = (rhoE - 0.5 * rho * np.sum(U ** 2, axis=1)) / ((K + 3) / 2 * R * rho)
+        tau = self._tau(T, rho, self.mu_c)
+        cc = xi[:, None, :] - U[None, :, :]
+        q = 0.5 * np.einsum("k,kcd,kc->cd", w, cc, np.sum(cc ** 2, axis=2) * self.gt + self.ht)
+        q = (2 * tau / (2 * tau + dt * Pr))[:, None] * q
+        self.rho, self.U, self.T, self.q, self.tau = rho, U, T, q, tau
+
+    def macro(self):
+        return dict(rho=self.rho, U=self.U, T=self.T, q=self.q)
+
+    def face_macro(self):
+        return dict(rho=self.f_rho, U=self.f_U, T=self.f_T, q=self.f_q)

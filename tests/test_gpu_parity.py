@@ -1,0 +1,273 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the FP64 oracle.
+
+FP64 bar: relative L2 <= 1e-12 on rho, U, T, q (cells and faces) and the wall
+data; the total-mass history equal to the oracle's to round-off.  FP32-storage
+bar: the tolerances stated per test (storage rounding of g/h ~ 6e-8).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_l2
+from dugksfoam_amd import cases
+from dugksfoam_amd.engine import DugksError, Engine
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle(c):
+    from oracle.oracle import Oracle
+    o = Oracle(c)
+    o.init_equilibrium()
+    return o
+
+
+def _small_cases():
+    return {
+        "cavity2d_12x21": (lambda: cases.cavity(12, 21), 12),
+        "cavity3d_6x12": (lambda: cases.cavity(6, 12, dim=3, kn=0.1), 4),
+        "cavity3d_K2": (lambda: cases.cavity(5, 8, dim=3, K=2), 3),
+        "channel_20x8": (lambda: cases.channel(20, 8, 1.25, 0.5, nv=12), 10),
+        "delaunay_300": (lambda: cases.delaunay_cavity(300, 12, nv=12), 6),
+        "batch_4": (lambda: cases.batch(n_cases=4, nmin=4, nmax=8, nv=8, seed=5), 5),
+    }
+
+
+@pytest.fixture(scope="module", params=list(_small_cases()))
+def pair(request):
+    mk, n = _small_cases()[request.param]
+    c = mk()
+    _say(f"[parity] case {request.param} cells={c.n_cells} nv={c.nv} steps={n}")
+    o = _oracle(c)
+    e = Engine(c)
+    e.init_equilibrium()
+    mass_o, mass_e = [], []
+    for _ in range(n):
+        o.step()
+        e.step()
+        mass_o.append(o.total_mass())
+        mass_e.append(float(np.sum(c.V * e.macro()["rho"])))
+    yield request.param, c, o, e, np.array(mass_o), np.array(mass_e)
+    e.close()
+    o.close()
+
+
+def test_cell_fields_fp64(pair):
+    name, c, o, e, _, _ = pair
+    mo, me = o.macro(), e.macro()
+    for k in ("rho", "U", "T", "q"):
+        r = rel_l2(me[k], mo[k])
+        _say(f"  {name} cell {k}: {r:.2e}")
+        assert r <= FP64_TOL, (name, k, r)
+
+
+def test_face_fields_fp64(pair):
+    name, c, o, e, _, _ = pair
+    fo, fe = o.face_macro(), e.face_macro()
+    for k in ("rho", "U", "T", "q"):
+        r = rel_l2(fe[k], fo[k])
+        assert r <= FP64_TOL, (name, k, r)
+
+
+def test_wall_data_fp64(pair):
+    name, c, o, e, _, _ = pair
+    for i, p in enumerate(c.patches):
+        if p["type"] != cases.PATCH_WALL:
+            continue
+        wo, we = o.wall(i), e.wall(i)
+        for k in ("rho_w", "q_wall", "stress_wall"):
+            r = rel_l2(we[k], wo[k])
+            assert r <= FP64_TOL, (name, p["name"], k, r)
+
+
+def test_mass_history(pair):
+    name, c, o, e, mo, me = pair
+    np.testing.assert_allclose(me, mo, rtol=1e-13)
+
+
+def test_distribution_functions(pair):
+    name, c, o, e, _, _ = pair
+    cells = np.unique(np.linspace(0, c.n_cells - 1, 7).astype(np.int32))
+    go, ho = o.df(cells)
+    ge, he = e.df(cells)
+    assert np.abs(ge - go).max() <= 1e-12 * np.abs(go).max()
+    if np.abs(ho).max() > 0:
+        assert np.abs(he - ho).max() <= 1e-12 * np.abs(ho).max()
+    else:
+        assert np.abs(he).max() == 0.0
+
+
+def test_courant(pair):
+    name, c, o, e, _, _ = pair
+    a, b = o.courant(), e.courant()
+    assert b[0] == pytest.approx(a[0], rel=1e-12)
+    assert b[1] == pytest.approx(a[1], rel=1e-12)
+
+
+@pytest.mark.parametrize("which", ["cavity2d_12x21", "cavity3d_6x12", "delaunay_300", "channel_20x8"])
+def test_fp32_storage(which):
+    mk, n = _small_cases()[which]
+    c = mk()
+    _say(f"[fp32] case {which}")
+    o = _oracle(c)
+    o.step(nsteps=n)
+    e = Engine(c, storage="fp32")
+    e.init_equilibrium()
+    e.step(nsteps=n)
+    mo, me = o.macro(), e.macro()
+    errs = {k: rel_l2(me[k], mo[k]) for k in mo}
+    _say("  ", errs)
+    assert errs["rho"] < 1e-6 and errs["T"] < 1e-6
+    assert errs["U"] < 1e-3 and errs["q"] < 1e-3
+    e.close()
+
+
+def test_dt_change_and_steps_split():
+    """dt changes between calls (adjustTimeStep): the resident state is re-prepared."""
+    c = cases.cavity(8, 13)
+    o = _oracle(c)
+    e = Engine(c)
+    e.init_equilibrium()
+    for dt, n in ((c.dt, 3), (0.7 * c.dt, 2), (c.dt, 1)):
+        o.step(dt, n)
+        e.step(dt, n)
+    mo, me = o.macro(), e.macro()
+    for k in mo:
+        assert rel_l2(me[k], mo[k]) <= FP64_TOL, k
+
+
+def test_save_load_resume_bit_exact():
+    c = cases.cavity(10, 13)
+    e = Engine(c)
+    e.init_equilibrium()
+    e.step(nsteps=4)
+    img = e.save_state()
+    e.step(nsteps=3)
+    a = e.macro()
+    e.load_state(img)
+    e.step(nsteps=3)
+    b = e.macro()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    # a fresh engine resumes from the image too
+    e2 = Engine(c)
+    e2.init_equilibrium()
+    e2.load_state(img)
+    e2.step(nsteps=3)
+    d = e2.macro()
+    for k in a:
+        assert np.array_equal(a[k], d[k]), k
+
+
+def test_error_paths():
+    c = cases.cavity(4, 5)
+    e = Engine(c)
+    with pytest.raises(DugksError) as x:
+        e.step(nsteps=1)
+    assert x.value.code == -4
+    bad = c.rho0.copy()
+    bad[3] = -1.0
+    with pytest.raises(DugksError) as x:
+        e.init_equilibrium(rho=bad)
+    assert x.value.code == -1
+    T = c.T0.copy()
+    T[0] = np.nan
+    with pytest.raises(DugksError):
+        e.init_equilibrium(T=T)
+    e.init_equilibrium()
+    with pytest.raises(DugksError):
+        e.step(dt=-1.0)
+    with pytest.raises(DugksError):
+        e.wall(5)
+    with pytest.raises(DugksError):
+        e.df([c.n_cells])
+    with pytest.raises(DugksError):
+        e.load_state(b"\0" * 64)
+    e.step(nsteps=1)
+
+
+def test_nonfinite_is_reported():
+    """A time step far beyond stability drives rho/T non-finite; the step reports it."""
+    c = cases.cavity(4, 9)
+    e = Engine(c)
+    e.init_equilibrium()
+    with pytest.raises(DugksError) as x:
+        e.step(dt=c.dt * 1e4, nsteps=40)
+    assert x.value.code == -5
+
+
+def _golden(name):
+    p = os.path.join(GOLDEN, name + ".npz")
+    if not os.path.exists(p):
+        pytest.skip(f"fixture {name}.npz not generated")
+    return np.load(p)
+
+
+@pytest.mark.parametrize("name", ["cavity2d", "channel"])
+def test_golden_full_size(name):
+    """Full configuration, N = 2000 steps, against the committed oracle fixture."""
+    g = _golden(name)
+    c = cases.config(name)
+    _say(f"[golden] {name} cells={c.n_cells} nv={c.nv}")
+    e = Engine(c)
+    e.init_equilibrium()
+    n = 0
+    mass = [float(np.sum(c.V * c.rho0))]
+    for s in g["snaps"]:
+        while n < s:
+            e.step(nsteps=1)
+            n += 1
+            mass.append(float(np.sum(c.V * e.macro()["rho"])))
+        m = e.macro()
+        errs = {k: rel_l2(m[k], g[f"{k}_{s}"]) for k in ("rho", "U", "T", "q")}
+        _say(f"  step {s}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
+        for k, v in errs.items():
+            assert v <= FP64_TOL, (s, k, v)
+    np.testing.assert_allclose(np.array(mass), g["mass"], rtol=1e-12)
+
+
+def test_golden_cavity3d():
+    g = _golden("cavity3d")
+    c = cases.config("cavity3d")
+    e = Engine(c)
+    e.init_equilibrium()
+    n = 0
+    for s in g["snaps"]:
+        e.step(nsteps=int(s) - n)
+        n = int(s)
+        m = e.macro()
+        for k in ("rho", "U", "T", "q"):
+            assert rel_l2(m[k], g[f"{k}_{s}"]) <= FP64_TOL, (s, k)
+
+
+def test_cavity3d_full_size_invariants():
+    """48^3 x 24^3: the lid moves along x, so the flow is mirror-symmetric in z;
+    the pre-relaxation wall mass flux vanishes at every wall face."""
+    c = cases.config("cavity3d")
+    _say(f"[invariants] cavity3d cells={c.n_cells} nv={c.nv}")
+    e = Engine(c)
+    e.init_equilibrium()
+    e.step(nsteps=3)
+    m = e.macro()
+    n = 48
+    r = m["rho"].reshape(n, n, n)          # [z][y][x]
+    uz = m["U"][:, 2].reshape(n, n, n)
+    ux = m["U"][:, 0].reshape(n, n, n)
+    assert np.abs(ux).max() > 1e-3
+    assert np.abs(r - r[::-1]).max() <= 1e-13 * np.abs(r).max()
+    assert np.abs(uz + uz[::-1]).max() <= 1e-11 * np.abs(m["U"]).max() + 1e-12
+    f = e.face_macro()
+    nfi = c.n_internal_faces
+    nrm = c.Sf[nfi:] / np.linalg.norm(c.Sf[nfi:], axis=1)[:, None]
+    un = np.einsum("ij,ij->i", f["U"][nfi:], nrm)
+    assert np.abs(un).max() <= 1e-12 * max(1.0, np.abs(f["U"]).max())
+    e.close()
