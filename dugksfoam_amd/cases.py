@@ -419,15 +419,72 @@ def channel(nx: int = 200, ny: int = 40, Lx: float = 5.0, Ly: float = 1.0, nv: i
                    meta=dict(kn=kn, a=a, nx=nx, ny=ny, Lx=Lx))
 
 
+def hard_core_points(n: int, r: float, fixed: np.ndarray, rng, margin: float) -> np.ndarray:
+    """n seeded uniform-random points in [margin, 1-margin]^2 at distance >= r from each other
+    and from `fixed` (random sequential addition: uniform candidates, rejected when closer
+    than r to an accepted point)."""
+    h = r / math.sqrt(2.0)
+    ng = int(math.ceil(1.0 / h)) + 1
+    grid = -np.ones((ng, ng), dtype=np.int64)
+    pts = [tuple(p) for p in fixed]
+    for i, (x, y) in enumerate(pts):
+        grid[min(int(x / h), ng - 1), min(int(y / h), ng - 1)] = i
+    out = []
+    r2 = r * r
+    tries = 0
+    while len(out) < n:
+        cand = rng.uniform(margin, 1 - margin, size=(4096, 2))
+        for x, y in cand:
+            tries += 1
+            gx, gy = int(x / h), int(y / h)
+            ok = True
+            for i in range(max(gx - 2, 0), min(gx + 3, ng)):
+                for j in range(max(gy - 2, 0), min(gy + 3, ng)):
+                    q = grid[i, j]
+                    if q >= 0:
+                        px, py = pts[q]
+                        if (px - x) ** 2 + (py - y) ** 2 < r2:
+                            ok = False
+                            break
+                if not ok:
+                    break
+            if ok:
+                grid[gx, gy] = len(pts)
+                pts.append((x, y))
+                out.append((x, y))
+                if len(out) == n:
+                    break
+        if tries > 200 * n:
+            raise RuntimeError("hard-core sampling did not reach the requested point count")
+    return np.array(out)
+
+
+def fv_courant_dt(c_V, owner, neighbour, Sf, nfi, speed, co):
+    """dt for which sum_f |Sf| speed dt / V <= co in every cell (finite-volume CFL)."""
+    tot = np.zeros(c_V.shape[0])
+    a = np.linalg.norm(Sf, axis=1)
+    np.add.at(tot, owner, a)
+    np.add.at(tot, neighbour, a[:nfi])
+    return float(co * np.min(c_V / (tot * speed)))
+
+
 def delaunay_cavity(n_interior: int = 99400, n_side: int = 400, nv: int = 64, kn: float = 1.0,
                     seed: int = 0, R: float = 208.0, T0: float = 273.0, rho0: float = 1.0,
                     xi_sigma: float = 5.0, lid_mach: float = 0.16, Pr: float = 2.0 / 3.0,
-                    omega: float = 0.5, K: int = 0, co: float = 0.5, dz: float = 0.01) -> Case:
-    """Cavity on a Delaunay triangulation of seeded uniform interior points plus
+                    omega: float = 0.5, K: int = 0, co: float = 0.5, dz: float = 0.01,
+                    min_sep: float = 0.5, smooth: int = 8) -> Case:
+    """Cavity on a Delaunay triangulation of seeded uniform-random interior points plus
     n_side uniform points per side of the unit square (one prism layer thick).
 
-    Cells are renumbered along a Hilbert curve of their centroids for locality;
-    faces follow OpenFOAM's upper-triangular order."""
+    min_sep > 0 keeps interior points at least min_sep / sqrt(n_interior) apart (a
+    hard-core random process of uniform density): plain uniform points produce sliver
+    triangles on which the scheme is unstable at the reference's Courant number, whose
+    deltaCoeffs ignore the cell volume.  `smooth` Laplacian sweeps (each interior point
+    moved to the mean of its Delaunay neighbours) then remove the nearly flat triangles
+    a hard-core process still allows.  dt is the stricter of that Courant number and the
+    finite-volume bound sum_f |Sf| sqrt(2) xi_max dt / V <= 2 co (at most about half of
+    a cell's faces are outflow faces for any one velocity).  Cells are renumbered along a Hilbert curve of
+    their centroids for locality; faces follow OpenFOAM's upper-triangular order."""
     from scipy.spatial import Delaunay
     rng = np.random.default_rng(seed)
     t = np.arange(n_side) / n_side
@@ -435,9 +492,25 @@ def delaunay_cavity(n_interior: int = 99400, n_side: int = 400, nv: int = 64, kn
                           np.stack([np.ones_like(t), t], 1),
                           np.stack([1 - t, np.ones_like(t)], 1),
                           np.stack([np.zeros_like(t), 1 - t], 1)])
-    margin = 0.25 / n_side
-    pin = rng.uniform(margin, 1 - margin, size=(n_interior, 2))
+    if min_sep > 0:
+        r = min_sep / math.sqrt(n_interior)
+        pin = hard_core_points(n_interior, r, bnd, rng, margin=0.5 * r)
+    else:
+        margin = 0.25 / n_side
+        pin = rng.uniform(margin, 1 - margin, size=(n_interior, 2))
     pts = np.concatenate([bnd, pin])
+    nb_ = bnd.shape[0]
+    for _ in range(smooth):
+        sim = Delaunay(pts).simplices
+        e = np.concatenate([sim[:, [0, 1]], sim[:, [1, 2]], sim[:, [2, 0]]])
+        e = np.unique(np.sort(e, axis=1), axis=0)
+        acc = np.zeros_like(pts)
+        cnt = np.zeros(pts.shape[0])
+        np.add.at(acc, e[:, 0], pts[e[:, 1]])
+        np.add.at(acc, e[:, 1], pts[e[:, 0]])
+        np.add.at(cnt, e[:, 0], 1)
+        np.add.at(cnt, e[:, 1], 1)
+        pts[nb_:] = acc[nb_:] / cnt[nb_:, None]
     tri = Delaunay(pts).simplices.astype(np.int64)
     P = pts[tri]                                   # [nt,3,2]
     area2 = ((P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1])
@@ -508,7 +581,8 @@ def delaunay_cavity(n_interior: int = 99400, n_side: int = 400, nv: int = 64, kn
     mu = mu_from_kn(kn, R, T0, rho0)
     nf = geo["neighbour"].shape[0]
     dcoef = 1.0 / np.linalg.norm(geo["C"][geo["neighbour"]] - geo["C"][geo["owner"][:nf]], axis=1)
-    dt = courant_dt(float(dcoef.max()), 2, xi_max, co)
+    dt = min(courant_dt(float(dcoef.max()), 2, xi_max, co),
+             fv_courant_dt(geo["V"], geo["owner"], geo["neighbour"], geo["Sf"], nf, math.sqrt(2) * xi_max, 2 * co))
     return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
                    meta=dict(kn=kn, seed=seed, n_interior=n_interior, n_side=n_side))
 

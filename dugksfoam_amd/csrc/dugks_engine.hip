@@ -3,12 +3,14 @@
 // One time step of dugksFoam's fvDVM::evolution() (src/fvDVM/fvDVM/fvDVM.C:1086-1108) as two
 // kernels over resident device state:
 //
-//   k_faces  (one workgroup per (cell, <=3 faces it owns), all velocities):
-//            S1 gBarPvol from the resident state, S2 least-squares gradient and upwind face
-//            reconstruction, S3/S4 Maxwell wall (outGoing, rho_w, incoming Maxwellian), S6 face
-//            moments (rho, rhoU, rhoE and the heat flux) reduced across the velocity lanes.
-//            Writes face macros, per-face equilibrium constants and exp tables, and the lagged
-//            fixedGradient boundary gradient of the next step.
+//   k_recon  (one workgroup per cell, all velocities):
+//            S2 least-squares gradient of the resident gBarPvol and upwind face reconstruction;
+//            each face value is written once, by the upwind cell.  Also the lagged fixedGradient
+//            boundary gradient of the next step.
+//   k_fmom   (one workgroup per face, all velocities):
+//            S3/S4 Maxwell wall (outGoing, rho_w, incoming Maxwellian), S6 face moments (rho, rhoU,
+//            rhoE and the heat flux) reduced across the velocity lanes; face macros, equilibrium
+//            constants and exp tables.
 //   k_cells  (one workgroup per cell, all velocities):
 //            S7 face relaxation, S8 flux divergence and g~ update in the reference's per-cell face
 //            order, S9 cell moments and macros, then S1 of the next step (gBarPvol = (1-r) g~ + r geq),
@@ -54,7 +56,6 @@ constexpr int KIND_NEI = 1;   // internal face, this cell is the neighbour
 constexpr int KIND_WALL = 2;  // Maxwell wall (fixedGradient gBarPvol, maxwellWall gSurf)
 constexpr int KIND_CYC = 3;   // cyclic (coupled) patch face
 constexpr int MAXS = 8;       // faces per cell handled by k_cells
-constexpr int GMAX = 3;       // faces per k_faces item
 constexpr int NAXMAX = 192;   // per-axis exp table entries (3 axes x 64 points)
 constexpr int MAXNT = 512;     // workgroup size cap (256 VGPRs per lane available)
 constexpr int MAXWAVES = MAXNT / 64;
@@ -102,12 +103,25 @@ struct EqC {
     double invRT, pre, qc[3], RT;
 };
 
+// One discrete velocity: xi, weight, |xi|^2 and the packed axis indices ix | iy << 8 | iz << 16.
+struct alignas(16) VelRec {
+    double x, y, z, w, xi2;
+    int32_t idx, pad;
+};
+static_assert(sizeof(VelRec) == 48, "velrec size");
+
+// Per-face data of k_fmom.
+struct alignas(16) FaceInfo {
+    double S[3];
+    int32_t own, kind, bidx, pad;
+};
+static_assert(sizeof(FaceInfo) == 48, "faceinfo size");
+
 template <typename TS>
 struct P {
     int D, nc, nf, nvp, nxi, nax, K, nt, niter, nw;
     double R, Pr, omega, Tref, kc;  // kc = K + 3 - D
-    const double *__restrict__ vx, *__restrict__ vy, *__restrict__ vz, *__restrict__ vw, *__restrict__ vxi2;
-    const int *__restrict__ vidx;
+    const VelRec *__restrict__ vel;
     const double *__restrict__ axis;
     const int *__restrict__ cs;
     const Slot *__restrict__ slots;
@@ -118,11 +132,11 @@ struct P {
     double *crho, *cU, *cT, *cq, *ctau;
     double *frho, *fU, *fT, *fq;
     FaceEq *feq;
-    double *fexp;
     WallC *wall;
     double *wexp;
     double *qwall, *swall;
-    const int4 *__restrict__ items;
+    const FaceInfo *__restrict__ finfo;
+    TS *fg, *fh;        // face values [face][velocity] (S2/S4 result, before S7)
     const TS *gb_in, *hb_in;
     TS *gb_out, *hb_out;
     const TS *gBg_in, *gBh_in;
@@ -138,10 +152,20 @@ struct NMom {
     static constexpr int v = D + 2 + D + D * (D + 1) / 2;  // + Q + P
 };
 
-// xi . S without contraction, so the upwind test sees the same product on every path.
+// xi . S = (x Sx + y Sy) + z Sz without FMA contraction, as the reference computes it: on faces
+// with |Sx| = |Sy| and velocities with |xi_x| = |xi_y| the two products cancel exactly to 0, which
+// selects the averaging branch of the upwind test; a contracted product would leave the rounding
+// error of one product and pick an upwind side instead.
 DEVI double dotnc(double ax, double ay, double az, const double *b)
 {
-    return __dadd_rn(__dadd_rn(__dmul_rn(ax, b[0]), __dmul_rn(ay, b[1])), __dmul_rn(az, b[2]));
+#pragma clang fp contract(off)
+    return (ax * b[0] + ay * b[1]) + az * b[2];
+}
+
+template <typename TS>
+DEVI double ld(const TS *a, size_t i)
+{
+    return (double)a[i];
 }
 
 struct Vel {
@@ -152,16 +176,16 @@ struct Vel {
 template <typename TS>
 DEVI Vel load_vel(const P<TS> &p, int k)
 {
+    const VelRec r = p.vel[k];
     Vel v;
-    v.x = p.vx[k];
-    v.y = p.vy[k];
-    v.z = p.vz[k];
-    v.w = p.vw[k];
-    v.xi2 = p.vxi2[k];
-    int id = p.vidx[k];
-    v.ix = id & 0xff;
-    v.iy = (id >> 8) & 0xff;
-    v.iz = (id >> 16) & 0xff;
+    v.x = r.x;
+    v.y = r.y;
+    v.z = r.z;
+    v.w = r.w;
+    v.xi2 = r.xi2;
+    v.ix = r.idx & 0xff;
+    v.iy = (r.idx >> 8) & 0xff;
+    v.iz = (r.idx >> 16) & 0xff;
     return v;
 }
 
@@ -200,6 +224,10 @@ DEVI void eq_eval(const EqC &e, const double *tab, int nxi, const Vel &v, double
 // Least-squares gradient of gbar (and hbar) at one cell for velocity k
 // (leastSquaresGrad: owner += pVec (phi_N - phi_P), neighbour -= nVec (phi_N - phi_P),
 // non-coupled boundary += pVec_b (phi_b - phi_P) with phi_b = phi_P + gradient_b / deltaCoeffs_b).
+// The stencil values of up to MAXG faces are loaded before any is used, so their
+// memory latencies overlap.
+constexpr int MAXG = 6;
+
 template <int D, bool H, typename TS>
 DEVI void grad_cell(const P<TS> &p, int cell, int k, double pg, double ph, double *G, double *Hg)
 {
@@ -208,23 +236,47 @@ DEVI void grad_cell(const P<TS> &p, int cell, int k, double pg, double ph, doubl
         G[d] = 0.0;
         Hg[d] = 0.0;
     }
-    const int s0 = p.cs[cell], s1 = p.cs[cell + 1];
-    for (int s = s0; s < s1; ++s) {
-        const Slot &sl = p.slots[s];
-        double dg, dh = 0.0;
-        if (sl.kind == KIND_WALL) {
-            size_t bi = (size_t)sl.bidx * p.nvp + k;
-            double bg = (double)p.gBg_in[bi];
-            dg = (pg + bg * sl.aux) - pg;
-            if (H) {
-                double bh = (double)p.gBh_in[bi];
-                dh = (ph + bh * sl.aux) - ph;
-            }
-        } else {
-            size_t oi = (size_t)sl.other * p.nvp + k;
-            dg = (double)p.gb_in[oi] - pg;
-            if (H) dh = (double)p.hb_in[oi] - ph;
+    const int s0 = p.cs[cell], ns = p.cs[cell + 1] - s0;
+    double vg[MAXG], vh[MAXG];
+#pragma unroll
+    for (int j = 0; j < MAXG; ++j) {
+        vg[j] = 0.0;
+        vh[j] = 0.0;
+        if (j < ns) {
+            const Slot &sl = p.slots[s0 + j];
+            const bool wall = sl.kind == KIND_WALL;
+            const size_t row = (size_t)(wall ? sl.bidx : sl.other) * p.nvp + k;
+            vg[j] = ld(wall ? p.gBg_in : p.gb_in, row);
+            if (H) vh[j] = ld(wall ? p.gBh_in : p.hb_in, row);
         }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXG; ++j) {
+        if (j < ns) {
+            const Slot &sl = p.slots[s0 + j];
+            double dg, dh = 0.0;
+            if (sl.kind == KIND_WALL) {
+                dg = (pg + vg[j] * sl.aux) - pg;
+                if (H) dh = (ph + vh[j] * sl.aux) - ph;
+            } else {
+                dg = vg[j] - pg;
+                if (H) dh = vh[j] - ph;
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                G[d] = fma(sl.L[d], dg, G[d]);
+                if (H) Hg[d] = fma(sl.L[d], dh, Hg[d]);
+            }
+        }
+    }
+    for (int s = s0 + MAXG; s < s0 + ns; ++s) {
+        const Slot &sl = p.slots[s];
+        const bool wall = sl.kind == KIND_WALL;
+        const size_t row = (size_t)(wall ? sl.bidx : sl.other) * p.nvp + k;
+        const double a = ld(wall ? p.gBg_in : p.gb_in, row);
+        const double b = H ? ld(wall ? p.gBh_in : p.hb_in, row) : 0.0;
+        const double dg = wall ? (pg + a * sl.aux) - pg : a - pg;
+        const double dh = wall ? (ph + b * sl.aux) - ph : b - ph;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             G[d] = fma(sl.L[d], dg, G[d]);
@@ -352,49 +404,21 @@ DEVI void block_sum(double *v, double *lds, double *out)
     __syncthreads();
 }
 
-template <typename TS>
-DEVI double ld(const TS *a, size_t i)
-{
-    return (double)a[i];
-}
 
 // ---------------------------------------------------------------------------------------------
-// k_faces: S1-S6 for the faces owned by one cell
+// k_recon: S1 state, S2 gradient and upwind face reconstruction (discreteVelocity.C:412-691)
+// One workgroup per cell, all velocities.  For every face and velocity the upwind cell writes the
+// face value: the owner for xi.S >= VSMALL, the neighbour for xi.S < -VSMALL; the owner writes the
+// average branch (|xi.S| < VSMALL) and the cyclic incoming value, which need the other cell's
+// gradient.  Wall faces get their outgoing values; incoming wall values come from rho_w later.
+// Also writes the next step's fixedGradient gradient n . grad at wall faces.
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, typename TS>
-__global__ __launch_bounds__(MAXNT) void k_faces(P<TS> p)
+__global__ __launch_bounds__(MAXNT) void k_recon(P<TS> p)
 {
-    constexpr int NM = NMom<D>::v + 1;  // + wall outGoing
-    __shared__ double red[MAXWAVES * GMAX * NM];
-    __shared__ double tot[GMAX * NM];
-    __shared__ double fin[GMAX][8];     // U[3], invRT per face for the exp tables
-
-    const int4 item = p.items[blockIdx.x];
-    const int c = item.x;
-    const int sl[GMAX] = {item.y, item.z, item.w};
-    const int cas = p.ccase[c];
-    const double dt = p.dtc[cas];
-
-    double Uh[GMAX][3];
-#pragma unroll
-    for (int j = 0; j < GMAX; ++j) {
-        Uh[j][0] = Uh[j][1] = Uh[j][2] = 0.0;
-        if (sl[j] >= 0) {
-            const Slot &S = p.slots[sl[j]];
-            if (S.kind == KIND_WALL) {
-                const WallC &W = p.wall[S.bidx];
-                Uh[j][0] = W.U[0]; Uh[j][1] = W.U[1]; Uh[j][2] = W.U[2];
-            } else {
-                Uh[j][0] = p.fU[3 * S.face]; Uh[j][1] = p.fU[3 * S.face + 1]; Uh[j][2] = p.fU[3 * S.face + 2];
-            }
-        }
-    }
-    double acc[GMAX][NM];
-#pragma unroll
-    for (int j = 0; j < GMAX; ++j)
-#pragma unroll
-        for (int m = 0; m < NM; ++m) acc[j][m] = 0.0;
-
+    const int c = blockIdx.x;
+    const double dt = p.dtc[p.ccase[c]];
+    const int s0 = p.cs[c], ns = p.cs[c + 1] - s0;
     for (int it = 0; it < p.niter; ++it) {
         const int k = threadIdx.x + it * p.nt;
         const Vel v = load_vel(p, k);
@@ -404,23 +428,11 @@ __global__ __launch_bounds__(MAXNT) void k_faces(P<TS> p)
         const double hc = H ? ld(p.hb_in, ck) : 0.0;
         double Gc[3] = {0, 0, 0}, Hc[3] = {0, 0, 0};
         grad_cell<D, H>(p, c, k, gc, hc, Gc, Hc);
-#pragma unroll
-        for (int j = 0; j < GMAX; ++j) {
-            if (sl[j] < 0) continue;
-            const Slot &S = p.slots[sl[j]];
+        for (int j = 0; j < ns; ++j) {
+            const Slot &S = p.slots[s0 + j];
             const double s = dotnc(v.x, v.y, v.z, S.S);
-            double gf = 0.0, hf = 0.0;
-            bool on = true;
+            const size_t fk = (size_t)S.face * p.nvp + k;
             if (S.kind == KIND_WALL) {
-                // S2 outgoing reconstruction + outGoing (discreteVelocity.C:605-627)
-                if (s > 0) {
-                    gf = gc + dotg<D>(Gc, S.dCs, hx);
-                    if (H) hf = hc + dotg<D>(Hc, S.dCs, hx);
-                    acc[j][NM - 1] = fma(v.w * s, gf, acc[j][NM - 1]);
-                } else {
-                    on = false;
-                }
-                // next step's fixedGradient gradient: n . grad (discreteVelocity.C:436-470)
                 const size_t bi = (size_t)S.bidx * p.nvp + k;
                 double ng = Gc[0] * S.dCo[0] + Gc[1] * S.dCo[1];
                 if (D == 3) ng += Gc[2] * S.dCo[2];
@@ -430,23 +442,34 @@ __global__ __launch_bounds__(MAXNT) void k_faces(P<TS> p)
                     if (D == 3) nh += Hc[2] * S.dCo[2];
                     p.gBh_out[bi] = (TS)nh;
                 }
+                if (s > 0) {
+                    p.fg[fk] = (TS)(gc + dotg<D>(Gc, S.dCs, hx));
+                    if (H) p.fh[fk] = (TS)(hc + dotg<D>(Hc, S.dCs, hx));
+                }
+            } else if (S.kind == KIND_NEI) {
+                if (s < -VSMALL) {
+                    p.fg[fk] = (TS)(gc + dotg<D>(Gc, S.dCs, hx));
+                    if (H) p.fh[fk] = (TS)(hc + dotg<D>(Hc, S.dCs, hx));
+                }
             } else {
                 const bool cyc = S.kind == KIND_CYC;
-                const bool up_self = cyc ? (s > VSMALL) : (s >= VSMALL);
-                const bool up_other = s < -VSMALL;
+                const bool up = cyc ? (s > VSMALL) : (s >= VSMALL);
+                const bool need = !up && (cyc || !(s < -VSMALL));
                 const double dcg = dotg<D>(Gc, S.dCs, hx);
                 const double dch = H ? dotg<D>(Hc, S.dCs, hx) : 0.0;
-                double go = 0.0, ho = 0.0, dog = 0.0, doh = 0.0;
-                if (__any(!up_self)) {
+                double vg = gc + dcg, vh = hc + dch;
+                if (__any(need)) {
                     const size_t ok = (size_t)S.other * p.nvp + k;
-                    go = ld(p.gb_in, ok);
-                    ho = H ? ld(p.hb_in, ok) : 0.0;
+                    double go = ld(p.gb_in, ok);
+                    double ho = H ? ld(p.hb_in, ok) : 0.0;
                     double Go[3] = {0, 0, 0}, Ho[3] = {0, 0, 0};
                     grad_cell<D, H>(p, S.other, k, go, ho, Go, Ho);
+                    double dog, doh = 0.0;
                     if (cyc) {
-                        // coupled boundary values w phi_P + (1-w) phi_N, same for the gradient
+                        // coupled boundary values w phi_P + (1-w) phi_N, same for the gradient,
+                        // at distance Cf - Cf - xi dt/2 (discreteVelocity.C:645-667)
                         const double wb = S.aux;
-                        double Gb[3], Hb[3];
+                        double Gb[3] = {0, 0, 0}, Hb[3] = {0, 0, 0};
                         for (int d = 0; d < D; ++d) {
                             Gb[d] = wb * Gc[d] + (1.0 - wb) * Go[d];
                             Hb[d] = wb * Hc[d] + (1.0 - wb) * Ho[d];
@@ -462,62 +485,102 @@ __global__ __launch_bounds__(MAXNT) void k_faces(P<TS> p)
                         dog = dotg<D>(Go, S.dCo, hx);
                         if (H) doh = dotg<D>(Ho, S.dCo, hx);
                     }
+                    if (need) {
+                        if (cyc && s < -VSMALL) {
+                            vg = go + dog;
+                            vh = ho + doh;
+                        } else if (cyc) {
+                            // 0.5*(phi_P + grad_P.d_P + phi_b + grad_b.d_b)
+                            vg = 0.5 * (((gc + dcg) + go) + dog);
+                            vh = 0.5 * (((hc + dch) + ho) + doh);
+                        } else {
+                            // 0.5*(phi_N + grad_N.d_N + phi_P + grad_P.d_P) (discreteVelocity.C:515-528)
+                            vg = 0.5 * (((go + dog) + gc) + dcg);
+                            vh = 0.5 * (((ho + doh) + hc) + dch);
+                        }
+                    }
                 }
-                if (up_self) {
-                    gf = gc + dcg;
-                    hf = hc + dch;
-                } else if (up_other) {
-                    gf = go + dog;
-                    hf = ho + doh;
-                } else if (cyc) {
-                    // 0.5*(phi_P + grad_P.d_P + phi_b + grad_b.d_b) (discreteVelocity.C:656-667)
-                    gf = 0.5 * (((gc + dcg) + go) + dog);
-                    hf = 0.5 * (((hc + dch) + ho) + doh);
-                } else {
-                    // 0.5*(phi_N + grad_N.d_N + phi_P + grad_P.d_P) (discreteVelocity.C:515-528)
-                    gf = 0.5 * (((go + dog) + gc) + dcg);
-                    hf = 0.5 * (((ho + doh) + hc) + dch);
+                if (up || need) {
+                    p.fg[fk] = (TS)vg;
+                    if (H) p.fh[fk] = (TS)vh;
                 }
             }
-            if (on) acc_moments<D, H>(acc[j], gf, hf, v, Uh[j]);
         }
     }
+}
 
-    // reduce across the block
-    {
-        double flat[GMAX * NM];
-#pragma unroll
-        for (int j = 0; j < GMAX; ++j)
-#pragma unroll
-            for (int m = 0; m < NM; ++m) flat[j * NM + m] = acc[j][m];
-        block_sum<GMAX * NM>(flat, red, tot);
+// ---------------------------------------------------------------------------------------------
+// k_fmom: S3 wall density, S4 incoming wall Maxwellian, S6 face moments (fvDVM.C:347-582) and
+// S7 face relaxation (discreteVelocity.C:819-932).  One workgroup per face, all velocities:
+// the first pass reduces the moments of the stored face values, the second rewrites them
+// relaxed towards the face equilibrium (incoming wall values: rho_w times the wall Maxwellian,
+// unrelaxed).  With NIT > 0 a thread keeps its face values in registers between the passes.
+// ---------------------------------------------------------------------------------------------
+template <int D, bool H, typename TS, int NIT>
+__global__ __launch_bounds__(MAXNT) void k_fmom(P<TS> p)
+{
+    constexpr int NM = NMom<D>::v + 1;  // + wall outGoing
+    constexpr int NR = NIT > 0 ? NIT : 1;
+    constexpr int NL = NIT > 0 ? NIT : 64;
+    __shared__ double red[MAXWAVES * NM];
+    __shared__ double tot[NM];
+    __shared__ double tabF[NAXMAX];
+    __shared__ double tabW[NAXMAX];
+    __shared__ double fin[16];
+    const int f = blockIdx.x;
+    const FaceInfo fi = p.finfo[f];
+    const int cas = p.ccase[fi.own];
+    const double dt = p.dtc[cas];
+    const bool wall = fi.kind == KIND_WALL;
+    const TS *__restrict__ fg = p.fg;
+    const TS *__restrict__ fh = p.fh;
+    double Uh[3];
+    if (wall) {
+        const WallC &W = p.wall[fi.bidx];
+        Uh[0] = W.U[0]; Uh[1] = W.U[1]; Uh[2] = W.U[2];
+        for (int t = threadIdx.x; t < p.nax; t += blockDim.x) tabW[t] = p.wexp[(size_t)fi.bidx * p.nax + t];
+    } else {
+        Uh[0] = p.fU[3 * f]; Uh[1] = p.fU[3 * f + 1]; Uh[2] = p.fU[3 * f + 2];
     }
-
-    // finish each owned face (threads 0..GMAX-1)
-    const int j = threadIdx.x;
-    const int slj = j == 0 ? item.y : (j == 1 ? item.z : item.w);
-    if (j < GMAX && slj >= 0) {
-        const Slot &S = p.slots[slj];
-        const int f = S.face;
-        double Uhj[3];
-        if (S.kind == KIND_WALL) {
-            for (int d = 0; d < 3; ++d) Uhj[d] = p.wall[S.bidx].U[d];
-        } else {
-            for (int d = 0; d < 3; ++d) Uhj[d] = p.fU[3 * f + d];
+    double acc[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) acc[m] = 0.0;
+    double vg[NR], vh[NR];
+#pragma unroll(NIT > 0 ? NIT : 2)
+    for (int it = 0; it < NL; ++it) {
+        if (it < p.niter) {
+            const int k = threadIdx.x + it * p.nt;
+            const Vel v = load_vel(p, k);
+            const double s = dotnc(v.x, v.y, v.z, fi.S);
+            const bool in = wall && !(s > 0);   // incoming wall values: rho_w times the wall Maxwellian
+            const size_t fk = (size_t)f * p.nvp + k;
+            const double gf = in ? 0.0 : ld(fg, fk);
+            const double hf = (H && !in) ? ld(fh, fk) : 0.0;
+            if (NIT > 0) {
+                vg[NIT > 0 ? it : 0] = gf;
+                vh[NIT > 0 ? it : 0] = hf;
+            }
+            if (!in) {
+                acc_moments<D, H>(acc, gf, hf, v, Uh);
+                if (wall) acc[NM - 1] = fma(v.w * s, gf, acc[NM - 1]);
+            }
         }
+    }
+    block_sum<NM>(acc, red, tot);
+    if (threadIdx.x == 0) {
         double a[NM];
-        for (int m = 0; m < NM; ++m) a[m] = tot[j * NM + m];
+        for (int m = 0; m < NM; ++m) a[m] = tot[m];
         double rho_w = 0.0;
-        if (S.kind == KIND_WALL) {
-            const WallC &W = p.wall[S.bidx];
+        if (wall) {
+            const WallC &W = p.wall[fi.bidx];
             // S3: rho_w = outGoing / |inComingByRho| (calculatedMaxwellFvPatchField.C:139-164)
             rho_w = a[NM - 1] / fabs(W.incoming);
             // S4: incoming DFs are rho_w times the wall Maxwellian; add their moments
             for (int m = 0; m < NM - 1; ++m) a[m] = fma(rho_w, W.A[m], a[m]);
         }
         double rho, U[3], T, tau, q[3];
-        const double mu = p.muc[p.ccase[c]];
-        finish_moments<D>(a, Uhj, p.R, p.K, mu, p.omega, p.Tref, rho, U, T, tau, q);
+        const double mu = p.muc[cas];
+        finish_moments<D>(a, Uh, p.R, p.K, mu, p.omega, p.Tref, rho, U, T, tau, q);
         const double t2 = 2.0 * tau;
         const double fq = t2 / (t2 + 0.5 * dt * p.Pr);
         for (int d = 0; d < 3; ++d) q[d] = fq * q[d];
@@ -527,9 +590,10 @@ __global__ __launch_bounds__(MAXNT) void k_faces(P<TS> p)
             p.fU[3 * f + d] = U[d];
             p.fq[3 * f + d] = q[d];
         }
-        FaceEq fe;
         EqC e;
         eq_consts(rho, U, T, q, p.R, p.Pr, D, e);
+        const double hh = 0.5 * dt;
+        FaceEq fe;
         for (int d = 0; d < 3; ++d) {
             fe.U[d] = e.U[d];
             fe.qc[d] = e.qc[d];
@@ -537,14 +601,13 @@ __global__ __launch_bounds__(MAXNT) void k_faces(P<TS> p)
         fe.invRT = e.invRT;
         fe.pre = e.pre;
         fe.RT = e.RT;
-        const double hh = 0.5 * dt;
         fe.r = hh / (2 * tau + hh);
         fe.rho_w = rho_w;
         for (int d = 0; d < 5; ++d) fe.pad[d] = 0.0;
         p.feq[f] = fe;
-        if (S.kind == KIND_WALL) {
+        if (wall) {
             // qWall, stressWall (fvDVM.C:539-581): moments about the wall velocity
-            const WallC &W = p.wall[S.bidx];
+            const WallC &W = p.wall[fi.bidx];
             const double fw = t2 / (t2 + 0.5 * dt * p.Pr);
             const double fs = t2 / (t2 + 0.5 * dt);
             double Pm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -557,43 +620,78 @@ __global__ __launch_bounds__(MAXNT) void k_faces(P<TS> p)
                 }
             double rU[3] = {0, 0, 0};
             for (int d = 0; d < D; ++d) rU[d] = a[1 + d];
-            for (int d = 0; d < 3; ++d) p.qwall[3 * S.bidx + d] = d < D ? fw * a[D + 2 + d] : 0.0;
+            for (int d = 0; d < 3; ++d) p.qwall[3 * fi.bidx + d] = d < D ? fw * a[D + 2 + d] : 0.0;
             for (int i = 0; i < 3; ++i)
                 for (int jj = 0; jj < 3; ++jj) {
                     double sv = 2.0 * Pm[i][jj] + W.U[i] * rU[jj] + W.U[jj] * rU[i] - W.U[i] * W.U[jj] * rho;
-                    p.swall[9 * S.bidx + 3 * i + jj] = fs * sv;
+                    p.swall[9 * fi.bidx + 3 * i + jj] = fs * sv;
                 }
         }
-        fin[j][0] = U[0];
-        fin[j][1] = U[1];
-        fin[j][2] = U[2];
-        fin[j][3] = e.invRT;
+        fin[0] = e.U[0]; fin[1] = e.U[1]; fin[2] = e.U[2];
+        fin[3] = e.invRT; fin[4] = e.pre; fin[5] = e.qc[0]; fin[6] = e.qc[1]; fin[7] = e.qc[2];
+        fin[8] = e.RT; fin[9] = fe.r; fin[10] = rho_w;
     }
     __syncthreads();
-    // per-axis exp tables of the face equilibria
-    for (int jj = 0; jj < GMAX; ++jj) {
-        if (sl[jj] < 0) continue;
-        const int f = p.slots[sl[jj]].face;
-        for (int t = threadIdx.x; t < p.nax; t += blockDim.x) {
-            const int ax = t / p.nxi;
-            const double u = fin[jj][ax] - p.axis[t - ax * p.nxi];
-            p.fexp[(size_t)f * p.nax + t] = exp(-(u * u) * (0.5 * fin[jj][3]));
+    // per-axis exp table of the face equilibrium
+    for (int t = threadIdx.x; t < p.nax; t += blockDim.x) {
+        const int ax = t / p.nxi;
+        const double u = fin[ax] - p.axis[t - ax * p.nxi];
+        tabF[t] = exp(-(u * u) * (0.5 * fin[3]));
+    }
+    __syncthreads();
+    EqC e;
+    e.U[0] = fin[0]; e.U[1] = fin[1]; e.U[2] = fin[2];
+    e.invRT = fin[3]; e.pre = fin[4]; e.qc[0] = fin[5]; e.qc[1] = fin[6]; e.qc[2] = fin[7]; e.RT = fin[8];
+    const double r = fin[9], rho_w = fin[10];
+    double Tw = 1.0, prew = 0.0;
+    if (wall) {
+        Tw = p.wall[fi.bidx].T;
+        prew = p.wall[fi.bidx].pre;
+    }
+    TS *__restrict__ fgo = p.fg;
+    TS *__restrict__ fho = p.fh;
+#pragma unroll(NIT > 0 ? NIT : 2)
+    for (int it = 0; it < NL; ++it) {
+        if (it < p.niter) {
+            const int k = threadIdx.x + it * p.nt;
+            const Vel v = load_vel(p, k);
+            const double s = dotnc(v.x, v.y, v.z, fi.S);
+            const size_t fk = (size_t)f * p.nvp + k;
+            double gf, hf = 0.0;
+            if (wall && !(s > 0)) {
+                // S4 incoming Maxwellian, left unrelaxed by S7 (maxwellWall operators are no-ops)
+                double m = prew * tabW[v.ix] * tabW[p.nxi + v.iy];
+                if (D == 3) m *= tabW[2 * p.nxi + v.iz];
+                gf = rho_w * m;
+                if (H) hf = gf * (p.R * Tw) * p.kc;
+            } else {
+                gf = NIT > 0 ? vg[NIT > 0 ? it : 0] : ld(fgo, fk);
+                if (H) hf = NIT > 0 ? vh[NIT > 0 ? it : 0] : ld(fho, fk);
+                double gq, hq;
+                eq_eval<D, H>(e, tabF, p.nxi, v, p.kc, p.K, gq, hq);
+                gf = (1.0 - r) * gf + r * gq;
+                if (H) hf = (1.0 - r) * hf + r * hq;
+            }
+            fgo[fk] = (TS)gf;
+            if (H) fho[fk] = (TS)hf;
         }
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_cells: S7-S9 for one cell, then S1 of the next step
+// k_cells: S8 g~ update with the relaxed face values, S9 cell moments, then S1 of the next step
+// One workgroup per cell, all velocities.  With NIT > 0 each thread keeps its NIT new g~ values
+// in registers until the cell moments are known; with NIT = 0 it writes them to its own slots of
+// the output row and rewrites them there as gbar.
 // ---------------------------------------------------------------------------------------------
-
 template <int D, bool H, typename TS, int NIT>
 __global__ __launch_bounds__(MAXNT) void k_cells(P<TS> p)
 {
+    constexpr int NR = NIT > 0 ? NIT : 1;
+    constexpr int NL = NIT > 0 ? NIT : 64;
     constexpr int NC = NMom<D>::v;
     __shared__ double tabC[NAXMAX];
     __shared__ double tabN[NAXMAX];
-    __shared__ double tabF[MAXS][NAXMAX];
-    __shared__ double tabW[MAXS][NAXMAX];
     __shared__ double red[MAXWAVES * NC];
     __shared__ double tot[NC];
     __shared__ double newm[16];
@@ -604,6 +702,10 @@ __global__ __launch_bounds__(MAXNT) void k_cells(P<TS> p)
     const double V = p.cV[c];
     const int s0 = p.cs[c];
     const int ns = p.cs[c + 1] - s0;
+    const TS *__restrict__ fg = p.fg;
+    const TS *__restrict__ fh = p.fh;
+    const TS *__restrict__ gbi = p.gb_in;
+    const TS *__restrict__ hbi = p.hb_in;
 
     // state that formed gbar: macros of the previous step
     const double Uc[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
@@ -617,132 +719,63 @@ __global__ __launch_bounds__(MAXNT) void k_cells(P<TS> p)
         const double u = Uc[ax] - p.axis[t - ax * p.nxi];
         tabC[t] = exp(-(u * u) * (0.5 * ec.invRT));
     }
-    for (int j = 0; j < ns; ++j) {
-        const Slot &S = p.slots[s0 + j];
-        for (int t = threadIdx.x; t < p.nax; t += blockDim.x) {
-            tabF[j][t] = p.fexp[(size_t)S.face * p.nax + t];
-            if (S.kind == KIND_WALL) tabW[j][t] = p.wexp[(size_t)S.bidx * p.nax + t];
-        }
-    }
     __syncthreads();
 
-    double gnew[NIT], hnew[NIT];
+    double gnew[NR], hnew[NR];
     double acc[NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) acc[m] = 0.0;
 
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        gnew[it] = 0.0;
-        hnew[it] = 0.0;
+#pragma unroll(NIT > 0 ? NIT : 2)
+    for (int it = 0; it < NL; ++it) {
         if (it < p.niter) {
             const int k = threadIdx.x + it * p.nt;
             const Vel v = load_vel(p, k);
-            const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
             const size_t ck = (size_t)c * p.nvp + k;
-            const double gb = ld(p.gb_in, ck);
-            const double hb = H ? ld(p.hb_in, ck) : 0.0;
+            const double gb = ld(gbi, ck);
+            const double hb = H ? ld(hbi, ck) : 0.0;
+            // relaxed face values of every face of the cell, loaded together
+            double fgv[MAXS], fhv[MAXS];
+#pragma unroll
+            for (int j = 0; j < MAXS; ++j) {
+                fgv[j] = 0.0;
+                fhv[j] = 0.0;
+                if (j < ns) {
+                    const size_t fk = (size_t)p.slots[s0 + j].face * p.nvp + k;
+                    fgv[j] = ld(fg, fk);
+                    if (H) fhv[j] = ld(fh, fk);
+                }
+            }
             // g~ of this step from gbar (inverse of S1)
             double geq, heq;
             eq_eval<D, H>(ec, tabC, p.nxi, v, p.kc, p.K, geq, heq);
             const double gt = (gb - rc * geq) * inv1mr;
             const double ht = H ? (hb - rc * heq) * inv1mr : 0.0;
-            double Gc[3] = {0, 0, 0}, Hc[3] = {0, 0, 0};
-            grad_cell<D, H>(p, c, k, gb, hb, Gc, Hc);
-            // S8 first part (discreteVelocity.C:937-938)
+            // S8 (discreteVelocity.C:937-975): -1/3 g~ + 4/3 gbar, then the faces in face order
             double ag = -1.0 / 3 * gt + 4.0 / 3 * gb;
             double ah = H ? (-1.0 / 3 * ht + 4.0 / 3 * hb) : 0.0;
-            for (int j = 0; j < ns; ++j) {
-                const Slot &S = p.slots[s0 + j];
-                const double s = dotnc(v.x, v.y, v.z, S.S);
-                const FaceEq &fe = p.feq[S.face];
-                double gf, hf = 0.0;
-                bool relax = true;
-                if (S.kind == KIND_WALL) {
-                    if (s > 0) {
-                        gf = gb + dotg<D>(Gc, S.dCs, hx);
-                        if (H) hf = hb + dotg<D>(Hc, S.dCs, hx);
-                    } else {
-                        // S4 incoming Maxwellian, left unrelaxed by S7 (maxwellWall operators are no-ops)
-                        const WallC &W = p.wall[S.bidx];
-                        double m = W.pre * tabW[j][v.ix] * tabW[j][p.nxi + v.iy];
-                        if (D == 3) m *= tabW[j][2 * p.nxi + v.iz];
-                        gf = fe.rho_w * m;
-                        if (H) hf = gf * (p.R * W.T) * p.kc;
-                        relax = false;
+#pragma unroll
+            for (int j = 0; j < MAXS; ++j) {
+                if (j < ns) {
+                    const Slot &S = p.slots[s0 + j];
+                    const double s = dotnc(v.x, v.y, v.z, S.S);
+                    const double tg = s * fgv[j] * dt / V;
+                    if (S.kind == KIND_NEI) ag += tg;
+                    else ag -= tg;
+                    if (H) {
+                        const double th = s * fhv[j] * dt / V;
+                        if (S.kind == KIND_NEI) ah += th;
+                        else ah -= th;
                     }
-                } else {
-                    const bool nei = S.kind == KIND_NEI;
-                    const bool cyc = S.kind == KIND_CYC;
-                    // upwind side: the owner for s >= VSMALL, the neighbour for s < -VSMALL
-                    const bool up_self = nei ? (s < -VSMALL) : (cyc ? (s > VSMALL) : (s >= VSMALL));
-                    const bool up_other = nei ? (s >= VSMALL) : (s < -VSMALL);
-                    const double dcg = dotg<D>(Gc, S.dCs, hx);
-                    const double dch = H ? dotg<D>(Hc, S.dCs, hx) : 0.0;
-                    double go = 0.0, ho = 0.0, dog = 0.0, doh = 0.0;
-                    if (__any(!up_self)) {
-                        const size_t ok = (size_t)S.other * p.nvp + k;
-                        go = ld(p.gb_in, ok);
-                        ho = H ? ld(p.hb_in, ok) : 0.0;
-                        double Go[3] = {0, 0, 0}, Ho[3] = {0, 0, 0};
-                        grad_cell<D, H>(p, S.other, k, go, ho, Go, Ho);
-                        if (cyc) {
-                            const double wb = S.aux;
-                            double Gb[3], Hb[3];
-                            for (int d = 0; d < D; ++d) {
-                                Gb[d] = wb * Gc[d] + (1.0 - wb) * Go[d];
-                                Hb[d] = wb * Hc[d] + (1.0 - wb) * Ho[d];
-                            }
-                            const double z3[3] = {0, 0, 0};
-                            go = wb * gb + (1.0 - wb) * go;
-                            dog = dotg<D>(Gb, z3, hx);
-                            if (H) {
-                                ho = wb * hb + (1.0 - wb) * ho;
-                                doh = dotg<D>(Hb, z3, hx);
-                            }
-                        } else {
-                            dog = dotg<D>(Go, S.dCo, hx);
-                            if (H) doh = dotg<D>(Ho, S.dCo, hx);
-                        }
-                    }
-                    if (up_self) {
-                        gf = gb + dcg;
-                        hf = hb + dch;
-                    } else if (up_other) {
-                        gf = go + dog;
-                        hf = ho + doh;
-                    } else if (nei || cyc) {
-                        // self first: neighbour side of an internal face, or the cyclic owner side
-                        gf = 0.5 * (((gb + dcg) + go) + dog);
-                        hf = 0.5 * (((hb + dch) + ho) + doh);
-                    } else {
-                        gf = 0.5 * (((go + dog) + gb) + dcg);
-                        hf = 0.5 * (((ho + doh) + hb) + dch);
-                    }
-                }
-                if (relax) {
-                    // S7 (discreteVelocity.C:867-881, 905-920)
-                    EqC e;
-                    e.U[0] = fe.U[0]; e.U[1] = fe.U[1]; e.U[2] = fe.U[2];
-                    e.invRT = fe.invRT; e.pre = fe.pre; e.RT = fe.RT;
-                    e.qc[0] = fe.qc[0]; e.qc[1] = fe.qc[1]; e.qc[2] = fe.qc[2];
-                    double gq, hq;
-                    eq_eval<D, H>(e, tabF[j], p.nxi, v, p.kc, p.K, gq, hq);
-                    gf = (1.0 - fe.r) * gf + fe.r * gq;
-                    if (H) hf = (1.0 - fe.r) * hf + fe.r * hq;
-                }
-                // S8 flux (discreteVelocity.C:948-975)
-                const double tg = s * gf * dt / V;
-                if (S.kind == KIND_NEI) ag += tg;
-                else ag -= tg;
-                if (H) {
-                    const double th = s * hf * dt / V;
-                    if (S.kind == KIND_NEI) ah += th;
-                    else ah -= th;
                 }
             }
-            gnew[it] = ag;
-            hnew[it] = ah;
+            if (NIT > 0) {
+                gnew[NIT > 0 ? it : 0] = ag;
+                hnew[NIT > 0 ? it : 0] = ah;
+            } else {
+                p.gb_out[ck] = (TS)ag;
+                if (H) p.hb_out[ck] = (TS)ah;
+            }
             acc_moments<D, H>(acc, ag, ah, v, Uc);
         }
     }
@@ -780,16 +813,18 @@ __global__ __launch_bounds__(MAXNT) void k_cells(P<TS> p)
     __syncthreads();
     // S1 of the next step (discreteVelocity.C:393-406)
     const double rn = 1.5 * dt / (2.0 * newm[8] + dt);
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
+#pragma unroll(NIT > 0 ? NIT : 2)
+    for (int it = 0; it < NL; ++it) {
         if (it < p.niter) {
             const int k = threadIdx.x + it * p.nt;
             const Vel v = load_vel(p, k);
             double geq, heq;
             eq_eval<D, H>(en, tabN, p.nxi, v, p.kc, p.K, geq, heq);
             const size_t ck = (size_t)c * p.nvp + k;
-            p.gb_out[ck] = (TS)((1.0 - rn) * gnew[it] + rn * geq);
-            if (H) p.hb_out[ck] = (TS)((1.0 - rn) * hnew[it] + rn * heq);
+            const double gn = NIT > 0 ? gnew[NIT > 0 ? it : 0] : ld(p.gb_out, ck);
+            const double hn = !H ? 0.0 : (NIT > 0 ? hnew[NIT > 0 ? it : 0] : ld(p.hb_out, ck));
+            p.gb_out[ck] = (TS)((1.0 - rn) * gn + rn * geq);
+            if (H) p.hb_out[ck] = (TS)((1.0 - rn) * hn + rn * heq);
         }
     }
 }
@@ -956,7 +991,7 @@ struct dugks_engine {
     int D = 2, nc = 0, nfi = 0, nf = 0, nb = 0, nw = 0, nv = 0, nvp = 0, nxi = 0, nax = 0, K = 0;
     int storage = DUGKS_STORAGE_FP64, n_cases = 1;
     bool has_h = true;
-    int nt = 64, niter = 1, nit_max = 1;
+    int nt = 64, niter = 1, nit_max = 1, fmom_nit = 0;
     double R = 0, omega = 0, Tref = 0, mu_ref = 0, Pr = 0, xi_max = 0;
     std::vector<int> perm;          // device velocity -> reference velocity, -1 for padding
     std::vector<double> dcoef_int;  // internal deltaCoeffs (Courant number)
@@ -965,18 +1000,20 @@ struct dugks_engine {
     std::vector<int> patch_type, patch_start, patch_size, patch_wall0;
     std::vector<int> wall_face_slot;  // per wall: its slot index
     std::vector<int> ncell_slots;
-    int n_items = 0;
+
     // device buffers
-    double *vx = nullptr, *vy = nullptr, *vz = nullptr, *vw = nullptr, *vxi2 = nullptr, *axis = nullptr;
-    int *vidx = nullptr, *cs = nullptr, *ccase = nullptr, *flag = nullptr, *wfslot = nullptr;
+    VelRec *vel = nullptr;
+    double *axis = nullptr;
+    int *cs = nullptr, *ccase = nullptr, *flag = nullptr, *wfslot = nullptr;
     Slot *slots = nullptr;
     double *cV = nullptr, *dtc = nullptr, *dtp = nullptr, *muc = nullptr;
     double *crho = nullptr, *cU = nullptr, *cT = nullptr, *cq = nullptr, *ctau = nullptr;
-    double *frho = nullptr, *fU = nullptr, *fT = nullptr, *fq = nullptr, *fexp = nullptr;
+    double *frho = nullptr, *fU = nullptr, *fT = nullptr, *fq = nullptr;
     FaceEq *feq = nullptr;
     WallC *wall = nullptr;
     double *wexp = nullptr, *qwall = nullptr, *swall = nullptr;
-    int4 *items = nullptr;
+    FaceInfo *finfo = nullptr;
+    void *fg = nullptr, *fh = nullptr;
     void *gb[2] = {nullptr, nullptr}, *hb[2] = {nullptr, nullptr};
     void *gB[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
     int cur = 0;         // buffer holding the current state
@@ -987,8 +1024,8 @@ struct dugks_engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int profiling = 0;
-    std::vector<hipEvent_t> pev;     // 3 events per profiled step
-    double prof_faces = 0.0, prof_cells = 0.0;
+    std::vector<hipEvent_t> pev;     // 4 events per profiled step
+    double prof_recon = 0.0, prof_fmom = 0.0, prof_cells = 0.0;
     int64_t prof_steps = 0;
     std::string err;
 };
@@ -1049,12 +1086,13 @@ P<TS> params(dugks_engine *h)
     p.D = h->D; p.nc = h->nc; p.nf = h->nf; p.nvp = h->nvp; p.nxi = h->nxi; p.nax = h->nax; p.K = h->K;
     p.nt = h->nt; p.niter = h->niter; p.nw = h->nw;
     p.R = h->R; p.Pr = h->Pr; p.omega = h->omega; p.Tref = h->Tref; p.kc = (double)(h->K + 3 - h->D);
-    p.vx = h->vx; p.vy = h->vy; p.vz = h->vz; p.vw = h->vw; p.vxi2 = h->vxi2; p.vidx = h->vidx;
+    p.vel = h->vel;
     p.axis = h->axis; p.cs = h->cs; p.slots = h->slots; p.cV = h->cV; p.ccase = h->ccase;
     p.dtc = h->dtc; p.muc = h->muc;
     p.crho = h->crho; p.cU = h->cU; p.cT = h->cT; p.cq = h->cq; p.ctau = h->ctau;
-    p.frho = h->frho; p.fU = h->fU; p.fT = h->fT; p.fq = h->fq; p.feq = h->feq; p.fexp = h->fexp;
-    p.wall = h->wall; p.wexp = h->wexp; p.qwall = h->qwall; p.swall = h->swall; p.items = h->items;
+    p.frho = h->frho; p.fU = h->fU; p.fT = h->fT; p.fq = h->fq; p.feq = h->feq;
+    p.wall = h->wall; p.wexp = h->wexp; p.qwall = h->qwall; p.swall = h->swall; p.finfo = h->finfo;
+    p.fg = (TS *)h->fg; p.fh = (TS *)h->fh;
     p.gb_in = (const TS *)h->gb[h->cur]; p.hb_in = (const TS *)h->hb[h->cur];
     p.gb_out = (TS *)h->gb[1 - h->cur]; p.hb_out = (TS *)h->hb[1 - h->cur];
     p.gBg_in = (const TS *)h->gB[h->gcur]; p.gBh_in = (const TS *)h->hB[h->gcur];
@@ -1079,9 +1117,27 @@ void launch_cells_n(dugks_engine *h, const P<TS> &p)
     case 2: launch_cells_t<D, H, TS, 2>(h, p); break;
     case 4: launch_cells_t<D, H, TS, 4>(h, p); break;
     case 8: launch_cells_t<D, H, TS, 8>(h, p); break;
-    case 16: launch_cells_t<D, H, TS, 16>(h, p); break;
-    case 32: launch_cells_t<D, H, TS, 32>(h, p); break;
-    default: launch_cells_t<D, H, TS, 64>(h, p); break;
+    default: launch_cells_t<D, H, TS, 0>(h, p); break;
+    }
+}
+
+template <int D, bool H, typename TS, int NIT>
+void launch_fmom_t(dugks_engine *h, const P<TS> &p)
+{
+    hipLaunchKernelGGL((k_fmom<D, H, TS, NIT>), dim3(h->nf), dim3(h->nt), 0, h->stream, p);
+}
+
+template <int D, bool H, typename TS>
+void launch_fmom_n(dugks_engine *h, const P<TS> &p)
+{
+    switch (h->fmom_nit) {
+    case 1: launch_fmom_t<D, H, TS, 1>(h, p); break;
+    case 2: launch_fmom_t<D, H, TS, 2>(h, p); break;
+    case 4: launch_fmom_t<D, H, TS, 4>(h, p); break;
+    case 8: launch_fmom_t<D, H, TS, 8>(h, p); break;
+    case 16: launch_fmom_t<D, H, TS, 16>(h, p); break;
+    case 32: launch_fmom_t<D, H, TS, 32>(h, p); break;
+    default: launch_fmom_t<D, H, TS, 0>(h, p); break;
     }
 }
 
@@ -1090,10 +1146,12 @@ void launch_step_t(dugks_engine *h, hipEvent_t *ev)
 {
     P<TS> p = params<TS>(h);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    hipLaunchKernelGGL((k_faces<D, H, TS>), dim3(h->n_items), dim3(h->nt), 0, h->stream, p);
+    hipLaunchKernelGGL((k_recon<D, H, TS>), dim3(h->nc), dim3(h->nt), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
-    launch_cells_n<D, H, TS>(h, p);
+    launch_fmom_n<D, H, TS>(h, p);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
+    launch_cells_n<D, H, TS>(h, p);
+    if (ev) (void)hipEventRecord(ev[3], h->stream);
 }
 
 template <typename F>
@@ -1179,11 +1237,12 @@ void choose_launch(int nv, int &nt, int &niter, int &nvp)
     nvp = nt * niter;
 }
 
+// k_cells variant: registers for up to 8 velocities per thread, else the output-row round trip (0)
 int nit_bucket(int niter)
 {
     int b = 1;
     while (b < niter) b <<= 1;
-    return b;
+    return b <= 8 ? b : 0;
 }
 
 double mag3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
@@ -1410,21 +1469,22 @@ int build(dugks_engine *h, const dugks_case *cs)
             sl[e] = s;
         }
     }
-    // ---- k_faces items: faces owned by each cell, up to GMAX per item ----
-    std::vector<int4> items;
-    for (int c = 0; c < nc; ++c) {
-        int buf[GMAX], nb_ = 0;
-        for (int e = cnt[c]; e < cnt[c + 1]; ++e) {
-            if (sl[e].kind == KIND_NEI) continue;
-            buf[nb_++] = e;
-            if (nb_ == GMAX) {
-                items.push_back(make_int4(c, buf[0], buf[1], buf[2]));
-                nb_ = 0;
-            }
+    // ---- per-face data of k_fmom ----
+    std::vector<FaceInfo> finfo(nf);
+    for (int f = 0; f < nf; ++f) {
+        FaceInfo fi;
+        std::memset(&fi, 0, sizeof fi);
+        for (int d = 0; d < 3; ++d) fi.S[d] = cs->Sf[3 * f + d];
+        fi.own = cs->owner[f];
+        fi.kind = KIND_OWN;
+        fi.bidx = -1;
+        if (f >= nfi) {
+            const int b = f - nfi;
+            fi.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : KIND_CYC;
+            fi.bidx = bkind[b] == DUGKS_PATCH_WALL ? bwall[b] : b;
         }
-        if (nb_ > 0) items.push_back(make_int4(c, buf[0], nb_ > 1 ? buf[1] : -1, nb_ > 2 ? buf[2] : -1));
+        finfo[f] = fi;
     }
-    h->n_items = (int)items.size();
 
     // ---- velocities: tensor product in reference order, then grouped by sign pattern ----
     const int nxi = h->nxi;
@@ -1467,15 +1527,20 @@ int build(dugks_engine *h, const dugks_case *cs)
     if (h->nt <= 0 || h->nt > MAXNT || h->niter > 64 || h->nt * h->niter < nv)
         return fail(h, DUGKS_ERR_UNSUPPORTED, "velocity count too large for one workgroup per cell");
     h->nit_max = nit_bucket(h->niter);
+    {
+        int b = 1;
+        while (b < h->niter) b <<= 1;
+        h->fmom_nit = b <= 32 ? b : 0;
+    }
     const int nvp = h->nvp;
-    std::vector<double> vx(nvp, 0.0), vy(nvp, 0.0), vz(nvp, 0.0), vw(nvp, 0.0), vxi2(nvp, 0.0);
-    std::vector<int> vidx(nvp, 0);
+    std::vector<VelRec> vel(nvp);
+    std::memset(vel.data(), 0, sizeof(VelRec) * nvp);
     h->perm.assign(nvp, -1);
     for (int k = 0; k < nv; ++k) {
         const int r = order[k];
-        vx[k] = rx[r]; vy[k] = ry[r]; vz[k] = rz[r]; vw[k] = rw[r];
-        vxi2[k] = rx[r] * rx[r] + ry[r] * ry[r] + rz[r] * rz[r];
-        vidx[k] = rid[r];
+        vel[k].x = rx[r]; vel[k].y = ry[r]; vel[k].z = rz[r]; vel[k].w = rw[r];
+        vel[k].xi2 = rx[r] * rx[r] + ry[r] * ry[r] + rz[r] * rz[r];
+        vel[k].idx = rid[r];
         h->perm[k] = r;
     }
     std::vector<double> axis(h->nax);
@@ -1531,34 +1596,34 @@ int build(dugks_engine *h, const dugks_case *cs)
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
     int r = 0;
-    if ((r = dalloc(h, &h->vx, nvp)) || (r = dalloc(h, &h->vy, nvp)) || (r = dalloc(h, &h->vz, nvp)) ||
-        (r = dalloc(h, &h->vw, nvp)) || (r = dalloc(h, &h->vxi2, nvp)) || (r = dalloc(h, &h->vidx, nvp)) ||
+    if ((r = dalloc(h, &h->vel, nvp)) ||
         (r = dalloc(h, &h->axis, h->nax)) || (r = dalloc(h, &h->cs, nc + 1)) || (r = dalloc(h, &h->slots, nslot)) ||
         (r = dalloc(h, &h->cV, nc)) || (r = dalloc(h, &h->ccase, nc)) || (r = dalloc(h, &h->dtc, h->n_cases)) ||
         (r = dalloc(h, &h->dtp, h->n_cases)) || (r = dalloc(h, &h->muc, h->n_cases)) ||
         (r = dalloc(h, &h->crho, nc)) || (r = dalloc(h, &h->cU, 3 * (size_t)nc)) || (r = dalloc(h, &h->cT, nc)) ||
         (r = dalloc(h, &h->cq, 3 * (size_t)nc)) || (r = dalloc(h, &h->ctau, nc)) || (r = dalloc(h, &h->frho, nf)) ||
         (r = dalloc(h, &h->fU, 3 * (size_t)nf)) || (r = dalloc(h, &h->fT, nf)) || (r = dalloc(h, &h->fq, 3 * (size_t)nf)) ||
-        (r = dalloc(h, &h->feq, nf)) || (r = dalloc(h, &h->fexp, (size_t)nf * h->nax)) ||
+        (r = dalloc(h, &h->feq, nf)) ||
         (r = dalloc(h, &h->wall, nw)) || (r = dalloc(h, &h->wexp, (size_t)nw * h->nax)) ||
         (r = dalloc(h, &h->qwall, 3 * (size_t)nw)) || (r = dalloc(h, &h->swall, 9 * (size_t)nw)) ||
-        (r = dalloc(h, &h->items, items.size())) || (r = dalloc(h, &h->flag, 1)) || (r = dalloc(h, &h->wfslot, nw)))
+        (r = dalloc(h, &h->finfo, nf)) || (r = dalloc(h, &h->flag, 1)) || (r = dalloc(h, &h->wfslot, nw)))
         return r;
     const size_t es = ts_size(h);
     const size_t row = (size_t)nc * nvp * es, brow = (size_t)(nw ? nw : 1) * nvp * es;
+    const size_t frow = (size_t)nf * nvp * es;
+    if ((r = dallocb(h, &h->fg, frow))) return r;
+    if (h->has_h && (r = dallocb(h, &h->fh, frow))) return r;
     for (int i = 0; i < 2; ++i) {
         if ((r = dallocb(h, &h->gb[i], row))) return r;
         if (h->has_h && (r = dallocb(h, &h->hb[i], row))) return r;
         if ((r = dallocb(h, &h->gB[i], brow))) return r;
         if (h->has_h && (r = dallocb(h, &h->hB[i], brow))) return r;
     }
-    if ((r = upload(h, h->vx, vx.data(), nvp)) || (r = upload(h, h->vy, vy.data(), nvp)) ||
-        (r = upload(h, h->vz, vz.data(), nvp)) || (r = upload(h, h->vw, vw.data(), nvp)) ||
-        (r = upload(h, h->vxi2, vxi2.data(), nvp)) || (r = upload(h, h->vidx, vidx.data(), nvp)) ||
+    if ((r = upload(h, h->vel, vel.data(), nvp)) ||
         (r = upload(h, h->axis, axis.data(), h->nax)) || (r = upload(h, h->cs, cnt.data(), nc + 1)) ||
         (r = upload(h, h->slots, sl.data(), nslot)) || (r = upload(h, h->cV, cs->V, nc)) ||
         (r = upload(h, h->ccase, ccase.data(), nc)) || (r = upload(h, h->muc, muc.data(), h->n_cases)) ||
-        (r = upload(h, h->items, items.data(), items.size())) || (r = upload(h, h->wfslot, wfslot.data(), nw)))
+        (r = upload(h, h->finfo, finfo.data(), nf)) || (r = upload(h, h->wfslot, wfslot.data(), nw)))
         return r;
     if (nw && (r = upload(h, h->wall, wc.data(), nw))) return r;
     HIPCHK(h, hipMemset(h->flag, 0, sizeof(int)));
@@ -1578,11 +1643,12 @@ void release(dugks_engine *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->vx); dfree(h->vy); dfree(h->vz); dfree(h->vw); dfree(h->vxi2); dfree(h->axis); dfree(h->vidx);
+    dfree(h->vel); dfree(h->axis);
     dfree(h->cs); dfree(h->ccase); dfree(h->flag); dfree(h->wfslot); dfree(h->slots); dfree(h->cV);
     dfree(h->dtc); dfree(h->dtp); dfree(h->muc); dfree(h->crho); dfree(h->cU); dfree(h->cT); dfree(h->cq);
-    dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->fexp); dfree(h->feq);
-    dfree(h->wall); dfree(h->wexp); dfree(h->qwall); dfree(h->swall); dfree(h->items);
+    dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->feq);
+    dfree(h->wall); dfree(h->wexp); dfree(h->qwall); dfree(h->swall); dfree(h->finfo);
+    dfree(h->fg); dfree(h->fh);
     for (int i = 0; i < 2; ++i) {
         dfree(h->gb[i]); dfree(h->hb[i]); dfree(h->gB[i]); dfree(h->hB[i]);
     }
@@ -1632,14 +1698,14 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
         HIPCHK(h, hipMemcpyAsync(h->dtp, h->dt_host.data(), sizeof(double) * h->n_cases, hipMemcpyHostToDevice, h->stream));
     }
     if (h->profiling) {
-        while ((int)h->pev.size() < 3 * nsteps) {
+        while ((int)h->pev.size() < 4 * nsteps) {
             hipEvent_t e;
             HIPCHK(h, hipEventCreate(&e));
             h->pev.push_back(e);
         }
     }
     for (int s = 0; s < nsteps; ++s) {
-        StepOp op{h, h->profiling ? &h->pev[3 * (size_t)s] : nullptr};
+        StepOp op{h, h->profiling ? &h->pev[4 * (size_t)s] : nullptr};
         dispatch(h, op);
         h->cur = 1 - h->cur;
         h->gcur = 1 - h->gcur;
@@ -1652,11 +1718,14 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
     h->last_ms = ms;
     if (h->profiling) {
         for (int s = 0; s < nsteps; ++s) {
-            float a = 0.f, b = 0.f;
-            HIPCHK(h, hipEventElapsedTime(&a, h->pev[3 * (size_t)s], h->pev[3 * (size_t)s + 1]));
-            HIPCHK(h, hipEventElapsedTime(&b, h->pev[3 * (size_t)s + 1], h->pev[3 * (size_t)s + 2]));
-            h->prof_faces += a;
-            h->prof_cells += b;
+            float a = 0.f, b = 0.f, c = 0.f;
+            const hipEvent_t *e = &h->pev[4 * (size_t)s];
+            HIPCHK(h, hipEventElapsedTime(&a, e[0], e[1]));
+            HIPCHK(h, hipEventElapsedTime(&b, e[1], e[2]));
+            HIPCHK(h, hipEventElapsedTime(&c, e[2], e[3]));
+            h->prof_recon += a;
+            h->prof_fmom += b;
+            h->prof_cells += c;
         }
         h->prof_steps += nsteps;
     }
@@ -1990,16 +2059,19 @@ int dugks_set_profiling(dugks_engine *h, int32_t on)
 {
     if (!h) return DUGKS_ERR_INVALID;
     h->profiling = on ? 1 : 0;
-    h->prof_faces = h->prof_cells = 0.0;
+    h->prof_recon = h->prof_fmom = h->prof_cells = 0.0;
     h->prof_steps = 0;
     return 0;
 }
 
-int dugks_get_profile(dugks_engine *h, double *faces_ms, double *cells_ms, int64_t *steps)
+int dugks_get_profile(dugks_engine *h, double *ms, int64_t *steps)
 {
     if (!h) return DUGKS_ERR_INVALID;
-    if (faces_ms) *faces_ms = h->prof_faces;
-    if (cells_ms) *cells_ms = h->prof_cells;
+    if (ms) {
+        ms[0] = h->prof_recon;
+        ms[1] = h->prof_fmom;
+        ms[2] = h->prof_cells;
+    }
     if (steps) *steps = h->prof_steps;
     return 0;
 }

@@ -61,7 +61,7 @@ def load_library(path: str = LIB_PATH):
     L.dugks_last_error.restype = C.c_char_p
     L.dugks_abi_version.restype = C.c_int
     L.dugks_set_profiling.argtypes = [vp, C.c_int32]
-    L.dugks_get_profile.argtypes = [vp, dp, dp, C.POINTER(C.c_int64)]
+    L.dugks_get_profile.argtypes = [vp, dp, C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -141,7 +141,7 @@ class Engine:
         return dict(rho=rho, U=U, T=T, q=q)
 
     def wall(self, patch: int):
-        n = self.case.patches[patch]["size"]
+        n = self.case.patches[patch]["size"] if 0 <= patch < len(self.case.patches) else 1
         rw, qw, sw = np.zeros(n), np.zeros((n, 3)), np.zeros((n, 9))
         self._chk(load_library().dugks_get_wall(self._h, int(patch), dptr(rw), dptr(qw), dptr(sw)))
         return dict(rho_w=rw, q_wall=qw, stress_wall=sw)
@@ -174,11 +174,14 @@ class Engine:
         """HIP-event timing of the two step kernels on the engine stream."""
         self._chk(load_library().dugks_set_profiling(self._h, 1 if on else 0))
 
+    KERNELS = ("k_recon", "k_fmom", "k_cells")
+
     def profile(self):
-        """(face-kernel ms, cell-kernel ms, steps) accumulated since set_profiling(True)."""
-        a, b, n = C.c_double(), C.c_double(), C.c_int64()
-        self._chk(load_library().dugks_get_profile(self._h, C.byref(a), C.byref(b), C.byref(n)))
-        return a.value, b.value, n.value
+        """({kernel: accumulated ms}, steps) since set_profiling(True)."""
+        ms = np.zeros(3)
+        n = C.c_int64()
+        self._chk(load_library().dugks_get_profile(self._h, dptr(ms), C.byref(n)))
+        return dict(zip(self.KERNELS, ms.tolist())), n.value
 
     @property
     def last_step_ms(self) -> float:

@@ -155,11 +155,11 @@ int dugks_last_step_ms(dugks_engine* h, double* ms);
 
 /* Per-kernel timing (no reference counterpart; for measurement).  With
  * profiling on, every step records HIP events on the engine stream around its
- * two kernels; dugks_get_profile returns the accumulated device time of the
- * face kernel (S1-S6) and the cell kernel (S7-S9 + next S1) and the number of
- * steps since profiling was switched on. */
+ * three kernels; dugks_get_profile returns in ms[3] the accumulated device time
+ * of the reconstruction kernel (S2), the face-moment kernel (S3-S6) and the cell
+ * kernel (S7-S9 + next S1), and the number of steps since profiling was switched on. */
 int dugks_set_profiling(dugks_engine* h, int32_t on);
-int dugks_get_profile(dugks_engine* h, double* faces_ms, double* cells_ms, int64_t* steps);
+int dugks_get_profile(dugks_engine* h, double* ms, int64_t* steps);
 
 void dugks_destroy(dugks_engine* h);
 const char* dugks_last_error(dugks_engine* h);
