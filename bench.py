@@ -136,7 +136,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage form")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of GIB GiB (rocprof counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration > 0:
+        import torch
+        n = int(a.copy_calibration * 2 ** 30) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the DUGKS engine on a single MI355X; "
               "multi-GPU velocity-space sharding is not built (see DESIGN.md)", file=sys.stderr)
