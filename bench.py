@@ -26,39 +26,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_model(case, storage_bytes: int):
-    """Bytes per launch of each step kernel (bytes model).
+def bytes_model(case, storage_bytes: int, n_blocks: int):
+    """Algorithmic bytes per launch of each step kernel: every array the kernel must
+    touch, read once and written once (the per-kernel table of DESIGN.md).
 
-    Per kernel, the compulsory traffic of its design: every input read once and
-    every output written once.  k_recon reads the resident distributions and the
-    lagged boundary gradient, writes the new boundary gradient and one value per
-    (face, velocity) it reconstructs; k_fmom reads those face values and writes
-    the relaxed value of every (face, velocity); k_cells reads the resident
-    distributions and the relaxed face values and writes the next resident
-    distributions.  The whole-step model is the minimum any
-    implementation must move: the distributions read twice and written once plus
-    the boundary gradient (two reads, one write).  h is not resident when K + 3 - D = 0."""
+    k_face reads the resident gbar rows (each cell's row once, however many faces use it)
+    and the lagged wall-gradient rows, writes the new wall-gradient rows and the block
+    partial sums; k_cell reads gbar and the wall rows and writes g~ and its partial sums;
+    k_cell_s1 reads g~ and writes the next gbar; the two _fin kernels touch O(faces) and
+    O(cells) data.  h is carried only when K + 3 - D != 0.  The whole-step figure is the
+    floor of any implementation: gbar read twice and written once (3 s per update)."""
     s = storage_bytes
     nd = 1 if (case.K + 3 - case.dim) == 0 else 2
     nc, nv = case.n_cells, case.nv
-    xi, _ = case.velocities()
-    nfi = case.n_internal_faces
-    nw = 0
-    face_vals = nfi * nv
-    for p in case.patches:
-        a, n = p["start"], p["size"]
-        if p["type"] == 1:
-            nw += n
-            face_vals += int(np.sum(xi @ case.Sf[a:a + n].T > 0))
-        else:
-            face_vals += n * nv
-    per_face_small = 8 * (16 + 8)
-    all_vals = case.n_faces * nv
-    recon = nd * s * (nc * nv + 2 * nw * nv + face_vals)
-    fmom = nd * s * (face_vals + all_vals) + case.n_faces * per_face_small
-    cells = nd * s * (2 * nc * nv + all_vals)
+    nf = case.n_faces
+    nw = sum(p["size"] for p in case.patches if p["type"] == 1)
+    nm_f = (case.dim + 2 + case.dim + case.dim * (case.dim + 1) // 2) + 1
+    nm_c = nm_f - 1
+    fpart = nf * n_blocks * nm_f * 8
+    cpart = nc * n_blocks * nm_c * 8
+    kb = {
+        "k_face": nd * s * (nc * nv + 2 * nw * nv) + fpart,
+        "k_face_fin": fpart + nf * 8 * (3 + 8 + 16),
+        "k_cell": nd * s * (2 * nc * nv + nw * nv) + cpart + nf * 8 * 16,
+        "k_cell_fin": cpart + nc * 8 * 9,
+        "k_cell_s1": nd * s * 2 * nc * nv,
+    }
     step_min = nd * s * (3 * nc * nv + 3 * nw * nv)
-    return {"k_recon": recon, "k_fmom": fmom, "k_cells": cells}, step_min
+    return kb, step_min
 
 
 def traffic_from_profile(kernel_prefix: str, storage: str):
@@ -84,13 +79,14 @@ def run_gpu(case, storage: str, steps: int, warmup: int):
     e.step(nsteps=steps)          # returns after the engine stream has drained
     t1 = time.perf_counter()
     kms, n = e.profile()
+    n_blocks = e.blocks()[2]
     m = e.macro()
     ok = bool(np.all(np.isfinite(m["rho"])) and np.all(m["rho"] > 0))
     e.close()
     wall = t1 - t0
     upd = case.n_cells * case.nv * steps
     sb = 8 if storage == "fp64" else 4
-    kb, step_min = bytes_model(case, sb)
+    kb, step_min = bytes_model(case, sb, n_blocks)
     kern = {k: (kms[k] / n, kb[k]) for k in kb}
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms, bytes_ = kern[dom]

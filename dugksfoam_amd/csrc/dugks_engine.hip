@@ -1,23 +1,25 @@
 // dugks_engine.hip -- MI355X (gfx950) DUGKS time-step engine behind the C-ABI of include/dugks.h.
 //
 // One time step of dugksFoam's fvDVM::evolution() (src/fvDVM/fvDVM/fvDVM.C:1086-1108) as two
-// kernels over resident device state:
+// kernels over resident device state, separated by the step's two velocity-space reductions:
 //
-//   k_recon  (one workgroup per cell, all velocities):
-//            S2 least-squares gradient of the resident gBarPvol and upwind face reconstruction;
-//            each face value is written once, by the upwind cell.  Also the lagged fixedGradient
-//            boundary gradient of the next step.
-//   k_fmom   (one workgroup per face, all velocities):
-//            S3/S4 Maxwell wall (outGoing, rho_w, incoming Maxwellian), S6 face moments (rho, rhoU,
-//            rhoE and the heat flux) reduced across the velocity lanes; face macros, equilibrium
-//            constants and exp tables.
-//   k_cells  (one workgroup per cell, all velocities):
-//            S7 face relaxation, S8 flux divergence and g~ update in the reference's per-cell face
-//            order, S9 cell moments and macros, then S1 of the next step (gBarPvol = (1-r) g~ + r geq),
-//            written to the other ping-pong buffer.
+//   k_face   (one workgroup per face, all velocities):
+//            S2 upwind reconstruction of the face value from the resident gBarPvol of the upwind
+//            cell and its least-squares gradient, S3/S4 Maxwell wall (outGoing, rho_w), S6 face
+//            moments reduced across the velocity lanes; writes the face macros, the S7 equilibrium
+//            constants and the lagged fixedGradient boundary gradient of the next step.
+//   k_cell   (one workgroup per cell, all velocities):
+//            rebuilds the upwind value of each of its faces the same way, S7 relaxation, S8 flux
+//            divergence and g~ update in the reference's per-cell face order, S9 cell moments and
+//            macros, then S1 of the next step (gBarPvol = (1-r) g~ + r geq), written to the other
+//            ping-pong buffer.
+//
+// No per-(face, velocity) array exists: face values are recomputed from the resident state where
+// they are needed, so a step streams the distributions through HBM three times (two reads, one
+// write) and the neighbour rows of the gradients come from L2.
 //
 // Data layout: distributions are [cell][velocity] with the velocity index contiguous and padded to
-// NVP = NT*NITER; a workgroup's lanes are velocities of one cell, so every mesh quantity is
+// NVP = NT*NITER; a workgroup's lanes are velocities of one cell or face, so every mesh quantity is
 // wave-uniform (scalar loads) and every distribution access is a coalesced row segment.  The
 // resident state is gBarPvol (g-bar-plus) and hBarPvol; g~ is recovered as (gbar - r geq)/(1 - r)
 // with the same r and geq that formed gbar.  Velocities are stored grouped by the sign pattern of
@@ -40,6 +42,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -55,10 +58,12 @@ constexpr int KIND_OWN = 0;   // internal face, this cell is the owner
 constexpr int KIND_NEI = 1;   // internal face, this cell is the neighbour
 constexpr int KIND_WALL = 2;  // Maxwell wall (fixedGradient gBarPvol, maxwellWall gSurf)
 constexpr int KIND_CYC = 3;   // cyclic (coupled) patch face
-constexpr int MAXS = 8;       // faces per cell handled by k_cells
+constexpr int MAXS = 8;       // faces per cell handled by k_cell
 constexpr int NAXMAX = 192;   // per-axis exp table entries (3 axes x 64 points)
 constexpr int MAXNT = 512;     // workgroup size cap (256 VGPRs per lane available)
 constexpr int MAXWAVES = MAXNT / 64;
+constexpr int NIBMAX = 64;     // velocity iterations per thread in a block
+constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
 
 // One (cell, face) incidence; a cell's slots are in ascending face order, the order in which the
 // reference's face loops touch that cell.
@@ -103,25 +108,42 @@ struct EqC {
     double invRT, pre, qc[3], RT;
 };
 
-// One discrete velocity: xi, weight, |xi|^2 and the packed axis indices ix | iy << 8 | iz << 16.
-struct alignas(16) VelRec {
-    double x, y, z, w, xi2;
-    int32_t idx, pad;
-};
-static_assert(sizeof(VelRec) == 48, "velrec size");
+// A discrete velocity is stored as its packed axis indices ix | iy << 8 | iz << 16 (bit 31 marks a
+// padding slot, weight 0); xi and the weight come from per-axis tables staged in LDS.
+constexpr uint32_t VPAD = 0x80000000u;
 
-// Per-face data of k_fmom.
-struct alignas(16) FaceInfo {
-    double S[3];
-    int32_t own, kind, bidx, pad;
+// Least-squares stencil of one cell, laid out for scalar loads: entry j is the cell's j-th slot
+// (ascending face order).  The distribution buffers hold the cell rows followed by one row per
+// wall face with the lagged fixedGradient gradient, and off[j] is the element offset of entry
+// j's row: the other cell (internal, cyclic) or the wall row.  Entries past the cell's face count
+// point at the cell's own row with L = 0 and contribute nothing.
+struct alignas(64) CellSt {
+    int64_t off[MAXS];
+    double L[MAXS][3];
+    double a[MAXS];       // wall entries: 1/deltaCoeffs
+    int32_t wmask;        // bit j set: entry j is a wall face
+    int32_t pad[15];
 };
-static_assert(sizeof(FaceInfo) == 48, "faceinfo size");
+static_assert(sizeof(CellSt) == 384, "cellst size");
+
+// Per-face data of k_face.
+struct alignas(16) FaceX {
+    double S[3];
+    double dP[3];   // Cf - C(owner)
+    double dN[3];   // Cf - C(neighbour) | wall: unit normal Sf/|Sf|
+    double w;       // cyclic: patch weight
+    int32_t P, N;   // owner; neighbour or coupled partner cell (-1 on walls)
+    int32_t kind, bidx;  // KIND_OWN (internal), KIND_WALL, KIND_CYC; wall index
+};
+static_assert(sizeof(FaceX) == 96, "facex size");
 
 template <typename TS>
 struct P {
     int D, nc, nf, nvp, nxi, nax, K, nt, niter, nw;
+    int nib, nb, vb;    // velocity blocks: nb blocks of vb = nt * nib velocities (niter = nb * nib)
     double R, Pr, omega, Tref, kc;  // kc = K + 3 - D
-    const VelRec *__restrict__ vel;
+    const uint32_t *__restrict__ vidx;  // packed axis indices of each (sign-sorted) velocity
+    const double *__restrict__ axw;     // per-axis quadrature weights, same layout as axis
     const double *__restrict__ axis;
     const int *__restrict__ cs;
     const Slot *__restrict__ slots;
@@ -135,8 +157,12 @@ struct P {
     WallC *wall;
     double *wexp;
     double *qwall, *swall;
-    const FaceInfo *__restrict__ finfo;
-    TS *fg, *fh;        // face values [face][velocity] (S2/S4 result, before S7)
+    const CellSt *__restrict__ cst;
+    const FaceX *__restrict__ fx;
+    const int *__restrict__ corder;  // launch order of the cells
+    const int *__restrict__ forder;  // launch order of the faces
+    double *fpart;      // [face][block][moment] partial sums of k_face
+    double *cpart;      // [cell][block][moment] partial sums of k_cell
     const TS *gb_in, *hb_in;
     TS *gb_out, *hb_out;
     const TS *gBg_in, *gBh_in;
@@ -162,6 +188,8 @@ DEVI double dotnc(double ax, double ay, double az, const double *b)
     return (ax * b[0] + ay * b[1]) + az * b[2];
 }
 
+constexpr int NXCD = 8;  // accelerator complex dies of an MI355X, each with its own L2
+
 template <typename TS>
 DEVI double ld(const TS *a, size_t i)
 {
@@ -173,19 +201,32 @@ struct Vel {
     int ix, iy, iz;
 };
 
+// Per-axis velocity table in LDS: vt[0, nax) = axis values, vt[nax, 2 nax) = axis weights.
 template <typename TS>
-DEVI Vel load_vel(const P<TS> &p, int k)
+DEVI void stage_vt(const P<TS> &p, double *vt)
 {
-    const VelRec r = p.vel[k];
+    for (int t = threadIdx.x; t < 2 * p.nax; t += blockDim.x) vt[t] = t < p.nax ? p.axis[t] : p.axw[t - p.nax];
+}
+
+// Velocity k: w = (w_z w_y) w_x and |xi|^2 = (x x + y y) + z z in the order the velocity set is
+// built (fvDVM.C:183-195), without contraction.
+template <int D, typename TS>
+DEVI Vel load_vel(const P<TS> &p, unsigned k, const double *vt)
+{
+#pragma clang fp contract(off)
+    const uint32_t id = p.vidx[k];
+    const int nxi = p.nxi, nax = p.nax;
     Vel v;
-    v.x = r.x;
-    v.y = r.y;
-    v.z = r.z;
-    v.w = r.w;
-    v.xi2 = r.xi2;
-    v.ix = r.idx & 0xff;
-    v.iy = (r.idx >> 8) & 0xff;
-    v.iz = (r.idx >> 16) & 0xff;
+    v.ix = id & 0xff;
+    v.iy = (id >> 8) & 0xff;
+    v.iz = (id >> 16) & 0xff;
+    v.x = vt[v.ix];
+    v.y = vt[nxi + v.iy];
+    v.z = D == 3 ? vt[2 * nxi + v.iz] : 0.0;
+    const double wx = vt[nax + v.ix], wy = vt[nax + nxi + v.iy];
+    v.w = D == 3 ? (vt[nax + 2 * nxi + v.iz] * wy) * wx : (wy * wx) * 1.0;
+    if (id & VPAD) v.w = 0.0;
+    v.xi2 = (v.x * v.x + v.y * v.y) + v.z * v.z;
     return v;
 }
 
@@ -221,66 +262,53 @@ DEVI void eq_eval(const EqC &e, const double *tab, int nxi, const Vel &v, double
     else heq = 0.0;
 }
 
+// Mesh constants are read through the constant address space: with a wave-uniform index the
+// loads become scalar loads whose results live in SGPRs, and they are never re-fetched per lane.
+// Only arrays that no kernel of the launch writes are read this way.
+#define CONST_AS __attribute__((address_space(4)))
+
+template <typename T>
+DEVI const CONST_AS T *cptr(const T *p)
+{
+    return (const CONST_AS T *)p;
+}
+
 // Least-squares gradient of gbar (and hbar) at one cell for velocity k
 // (leastSquaresGrad: owner += pVec (phi_N - phi_P), neighbour -= nVec (phi_N - phi_P),
 // non-coupled boundary += pVec_b (phi_b - phi_P) with phi_b = phi_P + gradient_b / deltaCoeffs_b).
-// The stencil values of up to MAXG faces are loaded before any is used, so their
-// memory latencies overlap.
-constexpr int MAXG = 6;
-
-template <int D, bool H, typename TS>
-DEVI void grad_cell(const P<TS> &p, int cell, int k, double pg, double ph, double *G, double *Hg)
+// All NS stencil rows are loaded before any is used, so their latencies overlap; the stencil
+// record comes in through scalar loads.  cell must be wave-uniform.
+template <int D, bool H, int NS, typename TS>
+DEVI void grad_at(const P<TS> &p, int cell, unsigned k, double pg, double ph, double *G, double *Hg)
 {
+    const CONST_AS CellSt &st = cptr(p.cst)[cell];
+    double vg[NS], vh[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        vg[j] = ld(p.gb_in + st.off[j], k);
+        vh[j] = H ? ld(p.hb_in + st.off[j], k) : 0.0;
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         G[d] = 0.0;
         Hg[d] = 0.0;
     }
-    const int s0 = p.cs[cell], ns = p.cs[cell + 1] - s0;
-    double vg[MAXG], vh[MAXG];
+    const int wm = st.wmask;
 #pragma unroll
-    for (int j = 0; j < MAXG; ++j) {
-        vg[j] = 0.0;
-        vh[j] = 0.0;
-        if (j < ns) {
-            const Slot &sl = p.slots[s0 + j];
-            const bool wall = sl.kind == KIND_WALL;
-            const size_t row = (size_t)(wall ? sl.bidx : sl.other) * p.nvp + k;
-            vg[j] = ld(wall ? p.gBg_in : p.gb_in, row);
-            if (H) vh[j] = ld(wall ? p.gBh_in : p.hb_in, row);
+    for (int j = 0; j < NS; ++j) {
+        double dg, dh = 0.0;
+        if (wm & (1 << j)) {
+            const double a = st.a[j];
+            dg = (pg + vg[j] * a) - pg;
+            if (H) dh = (ph + vh[j] * a) - ph;
+        } else {
+            dg = vg[j] - pg;
+            if (H) dh = vh[j] - ph;
         }
-    }
-#pragma unroll
-    for (int j = 0; j < MAXG; ++j) {
-        if (j < ns) {
-            const Slot &sl = p.slots[s0 + j];
-            double dg, dh = 0.0;
-            if (sl.kind == KIND_WALL) {
-                dg = (pg + vg[j] * sl.aux) - pg;
-                if (H) dh = (ph + vh[j] * sl.aux) - ph;
-            } else {
-                dg = vg[j] - pg;
-                if (H) dh = vh[j] - ph;
-            }
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                G[d] = fma(sl.L[d], dg, G[d]);
-                if (H) Hg[d] = fma(sl.L[d], dh, Hg[d]);
-            }
-        }
-    }
-    for (int s = s0 + MAXG; s < s0 + ns; ++s) {
-        const Slot &sl = p.slots[s];
-        const bool wall = sl.kind == KIND_WALL;
-        const size_t row = (size_t)(wall ? sl.bidx : sl.other) * p.nvp + k;
-        const double a = ld(wall ? p.gBg_in : p.gb_in, row);
-        const double b = H ? ld(wall ? p.gBh_in : p.hb_in, row) : 0.0;
-        const double dg = wall ? (pg + a * sl.aux) - pg : a - pg;
-        const double dh = wall ? (ph + b * sl.aux) - ph : b - ph;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            G[d] = fma(sl.L[d], dg, G[d]);
-            if (H) Hg[d] = fma(sl.L[d], dh, Hg[d]);
+            G[d] = fma(st.L[j][d], dg, G[d]);
+            if (H) Hg[d] = fma(st.L[j][d], dh, Hg[d]);
         }
     }
 }
@@ -406,425 +434,460 @@ DEVI void block_sum(double *v, double *lds, double *out)
 
 
 // ---------------------------------------------------------------------------------------------
-// k_recon: S1 state, S2 gradient and upwind face reconstruction (discreteVelocity.C:412-691)
-// One workgroup per cell, all velocities.  For every face and velocity the upwind cell writes the
-// face value: the owner for xi.S >= VSMALL, the neighbour for xi.S < -VSMALL; the owner writes the
-// average branch (|xi.S| < VSMALL) and the cyclic incoming value, which need the other cell's
-// gradient.  Wall faces get their outgoing values; incoming wall values come from rho_w later.
-// Also writes the next step's fixedGradient gradient n . grad at wall faces.
+// Upwind face value (discreteVelocity.C:472-530 internal faces, 605-627 Maxwell wall outgoing,
+// 628-672 coupled faces) from the gbar values, gradients and centre-to-face vectors of the face
+// owner P and of the other cell N.  hx = xi dt / 2.
+//   internal: xi.S >= VSMALL: P;  xi.S < -VSMALL: N;  otherwise the mean of both reconstructions
+//   cyclic:   xi.S > VSMALL: P;   otherwise the coupled boundary value w phi_P + (1 - w) phi_N with
+//             the coupled gradient, at distance -hx (the mean of both when |xi.S| <= VSMALL)
+//   wall:     P (only used for xi.S > 0)
 // ---------------------------------------------------------------------------------------------
-template <int D, bool H, typename TS>
-__global__ __launch_bounds__(MAXNT) void k_recon(P<TS> p)
+template <int D>
+DEVI double face_value(int kind, double s, double gP, const double *GP, const double *dP, double gN,
+                       const double *GN, const double *dN, double w, const double *hx)
 {
-    const int c = blockIdx.x;
-    const double dt = p.dtc[p.ccase[c]];
-    const int s0 = p.cs[c], ns = p.cs[c + 1] - s0;
-    for (int it = 0; it < p.niter; ++it) {
-        const int k = threadIdx.x + it * p.nt;
-        const Vel v = load_vel(p, k);
-        const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
-        const size_t ck = (size_t)c * p.nvp + k;
-        const double gc = ld(p.gb_in, ck);
-        const double hc = H ? ld(p.hb_in, ck) : 0.0;
-        double Gc[3] = {0, 0, 0}, Hc[3] = {0, 0, 0};
-        grad_cell<D, H>(p, c, k, gc, hc, Gc, Hc);
-        for (int j = 0; j < ns; ++j) {
-            const Slot &S = p.slots[s0 + j];
-            const double s = dotnc(v.x, v.y, v.z, S.S);
-            const size_t fk = (size_t)S.face * p.nvp + k;
-            if (S.kind == KIND_WALL) {
-                const size_t bi = (size_t)S.bidx * p.nvp + k;
-                double ng = Gc[0] * S.dCo[0] + Gc[1] * S.dCo[1];
-                if (D == 3) ng += Gc[2] * S.dCo[2];
-                p.gBg_out[bi] = (TS)ng;
-                if (H) {
-                    double nh = Hc[0] * S.dCo[0] + Hc[1] * S.dCo[1];
-                    if (D == 3) nh += Hc[2] * S.dCo[2];
-                    p.gBh_out[bi] = (TS)nh;
-                }
-                if (s > 0) {
-                    p.fg[fk] = (TS)(gc + dotg<D>(Gc, S.dCs, hx));
-                    if (H) p.fh[fk] = (TS)(hc + dotg<D>(Hc, S.dCs, hx));
-                }
-            } else if (S.kind == KIND_NEI) {
-                if (s < -VSMALL) {
-                    p.fg[fk] = (TS)(gc + dotg<D>(Gc, S.dCs, hx));
-                    if (H) p.fh[fk] = (TS)(hc + dotg<D>(Hc, S.dCs, hx));
-                }
-            } else {
-                const bool cyc = S.kind == KIND_CYC;
-                const bool up = cyc ? (s > VSMALL) : (s >= VSMALL);
-                const bool need = !up && (cyc || !(s < -VSMALL));
-                const double dcg = dotg<D>(Gc, S.dCs, hx);
-                const double dch = H ? dotg<D>(Hc, S.dCs, hx) : 0.0;
-                double vg = gc + dcg, vh = hc + dch;
-                if (__any(need)) {
-                    const size_t ok = (size_t)S.other * p.nvp + k;
-                    double go = ld(p.gb_in, ok);
-                    double ho = H ? ld(p.hb_in, ok) : 0.0;
-                    double Go[3] = {0, 0, 0}, Ho[3] = {0, 0, 0};
-                    grad_cell<D, H>(p, S.other, k, go, ho, Go, Ho);
-                    double dog, doh = 0.0;
-                    if (cyc) {
-                        // coupled boundary values w phi_P + (1-w) phi_N, same for the gradient,
-                        // at distance Cf - Cf - xi dt/2 (discreteVelocity.C:645-667)
-                        const double wb = S.aux;
-                        double Gb[3] = {0, 0, 0}, Hb[3] = {0, 0, 0};
-                        for (int d = 0; d < D; ++d) {
-                            Gb[d] = wb * Gc[d] + (1.0 - wb) * Go[d];
-                            Hb[d] = wb * Hc[d] + (1.0 - wb) * Ho[d];
-                        }
-                        const double z3[3] = {0, 0, 0};
-                        go = wb * gc + (1.0 - wb) * go;
-                        dog = dotg<D>(Gb, z3, hx);
-                        if (H) {
-                            ho = wb * hc + (1.0 - wb) * ho;
-                            doh = dotg<D>(Hb, z3, hx);
-                        }
-                    } else {
-                        dog = dotg<D>(Go, S.dCo, hx);
-                        if (H) doh = dotg<D>(Ho, S.dCo, hx);
-                    }
-                    if (need) {
-                        if (cyc && s < -VSMALL) {
-                            vg = go + dog;
-                            vh = ho + doh;
-                        } else if (cyc) {
-                            // 0.5*(phi_P + grad_P.d_P + phi_b + grad_b.d_b)
-                            vg = 0.5 * (((gc + dcg) + go) + dog);
-                            vh = 0.5 * (((hc + dch) + ho) + doh);
-                        } else {
-                            // 0.5*(phi_N + grad_N.d_N + phi_P + grad_P.d_P) (discreteVelocity.C:515-528)
-                            vg = 0.5 * (((go + dog) + gc) + dcg);
-                            vh = 0.5 * (((ho + doh) + hc) + dch);
-                        }
-                    }
-                }
-                if (up || need) {
-                    p.fg[fk] = (TS)vg;
-                    if (H) p.fh[fk] = (TS)vh;
-                }
-            }
-        }
+    if (kind == KIND_WALL) return gP + dotg<D>(GP, dP, hx);
+    if (kind == KIND_CYC) {
+        if (s > VSMALL) return gP + dotg<D>(GP, dP, hx);
+        double Gb[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < D; ++d) Gb[d] = w * GP[d] + (1.0 - w) * GN[d];
+        const double z3[3] = {0, 0, 0};
+        const double gb = w * gP + (1.0 - w) * gN;
+        const double dog = dotg<D>(Gb, z3, hx);
+        if (s < -VSMALL) return gb + dog;
+        return 0.5 * (((gP + dotg<D>(GP, dP, hx)) + gb) + dog);
     }
+    if (s >= VSMALL) return gP + dotg<D>(GP, dP, hx);
+    if (s < -VSMALL) return gN + dotg<D>(GN, dN, hx);
+    return 0.5 * (((gN + dotg<D>(GN, dN, hx)) + gP) + dotg<D>(GP, dP, hx));
+}
+
+// Lanes of a face that need the gradient of the owner P / of the other cell N.
+DEVI bool need_P(int kind, double s) { return kind != KIND_OWN || !(s < -VSMALL); }
+DEVI bool need_N(int kind, double s)
+{
+    if (kind == KIND_WALL) return false;
+    if (kind == KIND_CYC) return !(s > VSMALL);
+    return !(s >= VSMALL);
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_fmom: S3 wall density, S4 incoming wall Maxwellian, S6 face moments (fvDVM.C:347-582) and
-// S7 face relaxation (discreteVelocity.C:819-932).  One workgroup per face, all velocities:
-// the first pass reduces the moments of the stored face values, the second rewrites them
-// relaxed towards the face equilibrium (incoming wall values: rho_w times the wall Maxwellian,
-// unrelaxed).  With NIT > 0 a thread keeps its face values in registers between the passes.
+// Work decomposition of the step kernels: the velocity space is cut into nb blocks of vb = nt*nib
+// consecutive (sign-sorted) velocities, and a workgroup handles one (face or cell, block) pair.
+// The grid is 2-D: x runs over NXCD * per entity slots, y over the velocity blocks (a 1-D grid of
+// entities x blocks can exceed the 32-bit work-item count of one dispatch).  Workgroups are
+// dealt to the XCDs round-robin in linear id order, and NXCD divides the x extent, so XCD
+// x = blockIdx.x mod NXCD walks entity positions [x*per, (x+1)*per) of the launch order for
+// block 0, then block 1, ...  The launch order is a
+// space-filling-curve order of the cell (face) centres, so the workgroups resident on one XCD at a
+// time cover a compact patch of the mesh and one velocity block: the neighbour rows that the
+// least-squares gradients read are then shared through that XCD's L2.  Returns false for the
+// padding workgroups.
 // ---------------------------------------------------------------------------------------------
-template <int D, bool H, typename TS, int NIT>
-__global__ __launch_bounds__(MAXNT) void k_fmom(P<TS> p)
+DEVI bool xcd_block(int n, int nb, int &pos, int &b)
+{
+    const int per = (n + NXCD - 1) / NXCD;
+    b = (int)blockIdx.y;
+    pos = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+    return pos < n && b < nb;
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_face: S2 reconstruction and the S6 face-moment sums over one velocity block
+// (discreteVelocity.C:412-627, fvDVM.C:456-516).  The face value of each velocity is rebuilt from
+// the resident gbar of the upwind cell and its least-squares gradient (never stored); the moments
+// about the previous face velocity (wall velocity on walls) and the wall outGoing sum are reduced
+// across the lanes and written as the block's partial sums.  Wall faces also write the next
+// step's fixedGradient boundary gradient n . grad(gbar) (discreteVelocity.C:436-470).
+// ---------------------------------------------------------------------------------------------
+template <int D, bool H, int NS, typename TS>
+__global__ __launch_bounds__(MAXNT) void k_face(P<TS> p)
 {
     constexpr int NM = NMom<D>::v + 1;  // + wall outGoing
-    constexpr int NR = NIT > 0 ? NIT : 1;
-    constexpr int NL = NIT > 0 ? NIT : 64;
     __shared__ double red[MAXWAVES * NM];
     __shared__ double tot[NM];
-    __shared__ double tabF[NAXMAX];
-    __shared__ double tabW[NAXMAX];
-    __shared__ double fin[16];
-    const int f = blockIdx.x;
-    const FaceInfo fi = p.finfo[f];
-    const int cas = p.ccase[fi.own];
-    const double dt = p.dtc[cas];
-    const bool wall = fi.kind == KIND_WALL;
-    const TS *__restrict__ fg = p.fg;
-    const TS *__restrict__ fh = p.fh;
+    __shared__ double vt[2 * NAXMAX];
+    int pos, b;
+    if (!xcd_block(p.nf, p.nb, pos, b)) return;
+    stage_vt(p, vt);
+    __syncthreads();
+    const int f = cptr(p.forder)[pos];
+    const CONST_AS FaceX &fx = cptr(p.fx)[f];
+    const int kind = fx.kind;
+    const bool wall = kind == KIND_WALL;
+    const int cP = fx.P, cN = fx.N;
+    const int cas = cptr(p.ccase)[cP];
+    const double dt = cptr(p.dtc)[cas];
+    const double S[3] = {fx.S[0], fx.S[1], fx.S[2]};
+    const double dP[3] = {fx.dP[0], fx.dP[1], fx.dP[2]};
+    const double dN[3] = {fx.dN[0], fx.dN[1], fx.dN[2]};
+    const double wcy = fx.w;
     double Uh[3];
     if (wall) {
-        const WallC &W = p.wall[fi.bidx];
+        const CONST_AS WallC &W = cptr(p.wall)[fx.bidx];
         Uh[0] = W.U[0]; Uh[1] = W.U[1]; Uh[2] = W.U[2];
-        for (int t = threadIdx.x; t < p.nax; t += blockDim.x) tabW[t] = p.wexp[(size_t)fi.bidx * p.nax + t];
     } else {
-        Uh[0] = p.fU[3 * f]; Uh[1] = p.fU[3 * f + 1]; Uh[2] = p.fU[3 * f + 2];
+        Uh[0] = cptr(p.fU)[3 * f]; Uh[1] = cptr(p.fU)[3 * f + 1]; Uh[2] = cptr(p.fU)[3 * f + 2];
     }
+    const TS *__restrict__ rowP = p.gb_in + (size_t)cP * p.nvp;
+    const TS *__restrict__ rowN = p.gb_in + (size_t)(cN >= 0 ? cN : cP) * p.nvp;
+    const TS *__restrict__ hrowP = H ? p.hb_in + (size_t)cP * p.nvp : nullptr;
+    const TS *__restrict__ hrowN = H ? p.hb_in + (size_t)(cN >= 0 ? cN : cP) * p.nvp : nullptr;
     double acc[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) acc[m] = 0.0;
-    double vg[NR], vh[NR];
-#pragma unroll(NIT > 0 ? NIT : 2)
-    for (int it = 0; it < NL; ++it) {
-        if (it < p.niter) {
-            const int k = threadIdx.x + it * p.nt;
-            const Vel v = load_vel(p, k);
-            const double s = dotnc(v.x, v.y, v.z, fi.S);
-            const bool in = wall && !(s > 0);   // incoming wall values: rho_w times the wall Maxwellian
-            const size_t fk = (size_t)f * p.nvp + k;
-            const double gf = in ? 0.0 : ld(fg, fk);
-            const double hf = (H && !in) ? ld(fh, fk) : 0.0;
-            if (NIT > 0) {
-                vg[NIT > 0 ? it : 0] = gf;
-                vh[NIT > 0 ? it : 0] = hf;
+    const unsigned k0 = (unsigned)b * p.vb + threadIdx.x;
+    for (int it = 0; it < p.nib; ++it) {
+        const unsigned k = k0 + it * p.nt;
+        const Vel v = load_vel<D>(p, k, vt);
+        const double s = dotnc(v.x, v.y, v.z, S);
+        const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
+        double gP = 0.0, hP = 0.0, gN = 0.0, hN = 0.0;
+        double GP[3] = {0, 0, 0}, HP[3] = {0, 0, 0}, GN[3] = {0, 0, 0}, HN[3] = {0, 0, 0};
+        if (__any(need_P(kind, s))) {
+            gP = ld(rowP, k);
+            if (H) hP = ld(hrowP, k);
+            grad_at<D, H, NS>(p, cP, k, gP, hP, GP, HP);
+        }
+        if (__any(need_N(kind, s))) {
+            gN = ld(rowN, k);
+            if (H) hN = ld(hrowN, k);
+            grad_at<D, H, NS>(p, cN, k, gN, hN, GN, HN);
+        }
+        if (wall) {
+            const size_t bi = (size_t)fx.bidx * p.nvp + k;
+            double ng = GP[0] * dN[0] + GP[1] * dN[1];
+            if (D == 3) ng += GP[2] * dN[2];
+            p.gBg_out[bi] = (TS)ng;
+            if (H) {
+                double nh = HP[0] * dN[0] + HP[1] * dN[1];
+                if (D == 3) nh += HP[2] * dN[2];
+                p.gBh_out[bi] = (TS)nh;
             }
-            if (!in) {
-                acc_moments<D, H>(acc, gf, hf, v, Uh);
-                if (wall) acc[NM - 1] = fma(v.w * s, gf, acc[NM - 1]);
-            }
+        }
+        if (!wall || s > 0) {
+            const double gf = face_value<D>(kind, s, gP, GP, dP, gN, GN, dN, wcy, hx);
+            const double hf = H ? face_value<D>(kind, s, hP, HP, dP, hN, HN, dN, wcy, hx) : 0.0;
+            acc_moments<D, H>(acc, gf, hf, v, Uh);
+            if (wall) acc[NM - 1] = fma(v.w * s, gf, acc[NM - 1]);
         }
     }
     block_sum<NM>(acc, red, tot);
-    if (threadIdx.x == 0) {
-        double a[NM];
-        for (int m = 0; m < NM; ++m) a[m] = tot[m];
-        double rho_w = 0.0;
-        if (wall) {
-            const WallC &W = p.wall[fi.bidx];
-            // S3: rho_w = outGoing / |inComingByRho| (calculatedMaxwellFvPatchField.C:139-164)
-            rho_w = a[NM - 1] / fabs(W.incoming);
-            // S4: incoming DFs are rho_w times the wall Maxwellian; add their moments
-            for (int m = 0; m < NM - 1; ++m) a[m] = fma(rho_w, W.A[m], a[m]);
-        }
-        double rho, U[3], T, tau, q[3];
-        const double mu = p.muc[cas];
-        finish_moments<D>(a, Uh, p.R, p.K, mu, p.omega, p.Tref, rho, U, T, tau, q);
-        const double t2 = 2.0 * tau;
-        const double fq = t2 / (t2 + 0.5 * dt * p.Pr);
-        for (int d = 0; d < 3; ++d) q[d] = fq * q[d];
-        p.frho[f] = rho;
-        p.fT[f] = T;
-        for (int d = 0; d < 3; ++d) {
-            p.fU[3 * f + d] = U[d];
-            p.fq[3 * f + d] = q[d];
-        }
-        EqC e;
-        eq_consts(rho, U, T, q, p.R, p.Pr, D, e);
-        const double hh = 0.5 * dt;
-        FaceEq fe;
-        for (int d = 0; d < 3; ++d) {
-            fe.U[d] = e.U[d];
-            fe.qc[d] = e.qc[d];
-        }
-        fe.invRT = e.invRT;
-        fe.pre = e.pre;
-        fe.RT = e.RT;
-        fe.r = hh / (2 * tau + hh);
-        fe.rho_w = rho_w;
-        for (int d = 0; d < 5; ++d) fe.pad[d] = 0.0;
-        p.feq[f] = fe;
-        if (wall) {
-            // qWall, stressWall (fvDVM.C:539-581): moments about the wall velocity
-            const WallC &W = p.wall[fi.bidx];
-            const double fw = t2 / (t2 + 0.5 * dt * p.Pr);
-            const double fs = t2 / (t2 + 0.5 * dt);
-            double Pm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-            int m = 2 * D + 2;
-            for (int i = 0; i < D; ++i)
-                for (int jj = i; jj < D; ++jj) {
-                    Pm[i][jj] = a[m];
-                    Pm[jj][i] = a[m];
-                    ++m;
-                }
-            double rU[3] = {0, 0, 0};
-            for (int d = 0; d < D; ++d) rU[d] = a[1 + d];
-            for (int d = 0; d < 3; ++d) p.qwall[3 * fi.bidx + d] = d < D ? fw * a[D + 2 + d] : 0.0;
-            for (int i = 0; i < 3; ++i)
-                for (int jj = 0; jj < 3; ++jj) {
-                    double sv = 2.0 * Pm[i][jj] + W.U[i] * rU[jj] + W.U[jj] * rU[i] - W.U[i] * W.U[jj] * rho;
-                    p.swall[9 * fi.bidx + 3 * i + jj] = fs * sv;
-                }
-        }
-        fin[0] = e.U[0]; fin[1] = e.U[1]; fin[2] = e.U[2];
-        fin[3] = e.invRT; fin[4] = e.pre; fin[5] = e.qc[0]; fin[6] = e.qc[1]; fin[7] = e.qc[2];
-        fin[8] = e.RT; fin[9] = fe.r; fin[10] = rho_w;
-    }
-    __syncthreads();
-    // per-axis exp table of the face equilibrium
-    for (int t = threadIdx.x; t < p.nax; t += blockDim.x) {
-        const int ax = t / p.nxi;
-        const double u = fin[ax] - p.axis[t - ax * p.nxi];
-        tabF[t] = exp(-(u * u) * (0.5 * fin[3]));
-    }
-    __syncthreads();
-    EqC e;
-    e.U[0] = fin[0]; e.U[1] = fin[1]; e.U[2] = fin[2];
-    e.invRT = fin[3]; e.pre = fin[4]; e.qc[0] = fin[5]; e.qc[1] = fin[6]; e.qc[2] = fin[7]; e.RT = fin[8];
-    const double r = fin[9], rho_w = fin[10];
-    double Tw = 1.0, prew = 0.0;
-    if (wall) {
-        Tw = p.wall[fi.bidx].T;
-        prew = p.wall[fi.bidx].pre;
-    }
-    TS *__restrict__ fgo = p.fg;
-    TS *__restrict__ fho = p.fh;
-#pragma unroll(NIT > 0 ? NIT : 2)
-    for (int it = 0; it < NL; ++it) {
-        if (it < p.niter) {
-            const int k = threadIdx.x + it * p.nt;
-            const Vel v = load_vel(p, k);
-            const double s = dotnc(v.x, v.y, v.z, fi.S);
-            const size_t fk = (size_t)f * p.nvp + k;
-            double gf, hf = 0.0;
-            if (wall && !(s > 0)) {
-                // S4 incoming Maxwellian, left unrelaxed by S7 (maxwellWall operators are no-ops)
-                double m = prew * tabW[v.ix] * tabW[p.nxi + v.iy];
-                if (D == 3) m *= tabW[2 * p.nxi + v.iz];
-                gf = rho_w * m;
-                if (H) hf = gf * (p.R * Tw) * p.kc;
-            } else {
-                gf = NIT > 0 ? vg[NIT > 0 ? it : 0] : ld(fgo, fk);
-                if (H) hf = NIT > 0 ? vh[NIT > 0 ? it : 0] : ld(fho, fk);
-                double gq, hq;
-                eq_eval<D, H>(e, tabF, p.nxi, v, p.kc, p.K, gq, hq);
-                gf = (1.0 - r) * gf + r * gq;
-                if (H) hf = (1.0 - r) * hf + r * hq;
-            }
-            fgo[fk] = (TS)gf;
-            if (H) fho[fk] = (TS)hf;
-        }
-    }
+    if ((int)threadIdx.x < NM) p.fpart[((size_t)f * p.nb + b) * NM + threadIdx.x] = tot[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_cells: S8 g~ update with the relaxed face values, S9 cell moments, then S1 of the next step
-// One workgroup per cell, all velocities.  With NIT > 0 each thread keeps its NIT new g~ values
-// in registers until the cell moments are known; with NIT = 0 it writes them to its own slots of
-// the output row and rewrites them there as gbar.
+// k_face_fin: one thread per face.  Sums the block partials in block order, then S3 wall density
+// rho_w = outGoing / |inComingByRho| (calculatedMaxwellFvPatchField.C:139-164) with the moments of
+// the S4 incoming Maxwellian added, the S6 face macros (fvDVM.C:456-522), the S7 equilibrium
+// constants and relaxation factor, and the wall diagnostics qWall, stressWall (fvDVM.C:539-581).
 // ---------------------------------------------------------------------------------------------
-template <int D, bool H, typename TS, int NIT>
-__global__ __launch_bounds__(MAXNT) void k_cells(P<TS> p)
+template <int D, typename TS>
+__global__ __launch_bounds__(256) void k_face_fin(P<TS> p)
 {
-    constexpr int NR = NIT > 0 ? NIT : 1;
-    constexpr int NL = NIT > 0 ? NIT : 64;
+    constexpr int NM = NMom<D>::v + 1;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= p.nf) return;
+    const FaceX &fx = p.fx[f];
+    const bool wall = fx.kind == KIND_WALL;
+    const int cas = p.ccase[fx.P];
+    const double dt = p.dtc[cas];
+    double a[NM];
+    for (int m = 0; m < NM; ++m) a[m] = 0.0;
+    const double *fp = p.fpart + (size_t)f * p.nb * NM;
+    for (int b = 0; b < p.nb; ++b)
+        for (int m = 0; m < NM; ++m) a[m] += fp[b * NM + m];
+    double Uh[3];
+    double rho_w = 0.0;
+    if (wall) {
+        const WallC &W = p.wall[fx.bidx];
+        Uh[0] = W.U[0]; Uh[1] = W.U[1]; Uh[2] = W.U[2];
+        rho_w = a[NM - 1] / fabs(W.incoming);
+        for (int m = 0; m < NM - 1; ++m) a[m] = fma(rho_w, W.A[m], a[m]);
+    } else {
+        Uh[0] = p.fU[3 * f]; Uh[1] = p.fU[3 * f + 1]; Uh[2] = p.fU[3 * f + 2];
+    }
+    double rho, U[3], T, tau, q[3];
+    finish_moments<D>(a, Uh, p.R, p.K, p.muc[cas], p.omega, p.Tref, rho, U, T, tau, q);
+    const double t2 = 2.0 * tau;
+    const double fq = t2 / (t2 + 0.5 * dt * p.Pr);
+    for (int d = 0; d < 3; ++d) q[d] = fq * q[d];
+    p.frho[f] = rho;
+    p.fT[f] = T;
+    for (int d = 0; d < 3; ++d) {
+        p.fU[3 * f + d] = U[d];
+        p.fq[3 * f + d] = q[d];
+    }
+    EqC e;
+    eq_consts(rho, U, T, q, p.R, p.Pr, D, e);
+    const double hh = 0.5 * dt;
+    FaceEq fe;
+    for (int d = 0; d < 3; ++d) {
+        fe.U[d] = e.U[d];
+        fe.qc[d] = e.qc[d];
+    }
+    fe.invRT = e.invRT;
+    fe.pre = e.pre;
+    fe.RT = e.RT;
+    fe.r = hh / (2 * tau + hh);
+    fe.rho_w = rho_w;
+    for (int d = 0; d < 5; ++d) fe.pad[d] = 0.0;
+    if (wall) {
+        const WallC &W = p.wall[fx.bidx];
+        fe.pad[0] = W.T;
+        fe.pad[1] = W.pre;
+        const double fw = t2 / (t2 + 0.5 * dt * p.Pr);
+        const double fs = t2 / (t2 + 0.5 * dt);
+        double Pm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        int m = 2 * D + 2;
+        for (int i = 0; i < D; ++i)
+            for (int jj = i; jj < D; ++jj) {
+                Pm[i][jj] = a[m];
+                Pm[jj][i] = a[m];
+                ++m;
+            }
+        double rU[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) rU[d] = a[1 + d];
+        for (int d = 0; d < 3; ++d) p.qwall[3 * fx.bidx + d] = d < D ? fw * a[D + 2 + d] : 0.0;
+        for (int i = 0; i < 3; ++i)
+            for (int jj = 0; jj < 3; ++jj) {
+                double sv = 2.0 * Pm[i][jj] + W.U[i] * rU[jj] + W.U[jj] * rU[i] - W.U[i] * W.U[jj] * rho;
+                p.swall[9 * fx.bidx + 3 * i + jj] = fs * sv;
+            }
+    }
+    p.feq[f] = fe;
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_cell: S7 face relaxation, S8 g~ update and the S9 cell-moment sums over one velocity block
+// (discreteVelocity.C:819-978, fvDVM.C:604-622, 712-721).  For each face of the cell (ascending
+// face order, the order of the reference's face loops) the upwind face value is rebuilt exactly as
+// k_face built it, relaxed towards the face equilibrium (incoming wall values: rho_w times the
+// wall Maxwellian, left unrelaxed, discreteVelocity.C:693-731) and its flux applied.  g~ of this
+// step is recovered from the resident gbar as (gbar - r geq) / (1 - r); the new g~ goes to the
+// output buffer, where k_cell_s1 turns it into the next gbar.
+// Dynamic LDS: [velocity table (2 nax) | tabC | tabF x MAXS | tabW x MAXS], nax doubles each.
+// ---------------------------------------------------------------------------------------------
+template <int D, bool H, int NS, typename TS>
+__global__ __launch_bounds__(MAXNT) void k_cell(P<TS> p)
+{
     constexpr int NC = NMom<D>::v;
-    __shared__ double tabC[NAXMAX];
-    __shared__ double tabN[NAXMAX];
+    extern __shared__ double tabs[];
     __shared__ double red[MAXWAVES * NC];
     __shared__ double tot[NC];
-    __shared__ double newm[16];
 
-    const int c = blockIdx.x;
-    const int cas = p.ccase[c];
-    const double dt = p.dtc[cas];
-    const double V = p.cV[c];
-    const int s0 = p.cs[c];
-    const int ns = p.cs[c + 1] - s0;
-    const TS *__restrict__ fg = p.fg;
-    const TS *__restrict__ fh = p.fh;
-    const TS *__restrict__ gbi = p.gb_in;
-    const TS *__restrict__ hbi = p.hb_in;
+    int pos, b;
+    if (!xcd_block(p.nc, p.nb, pos, b)) return;
+    const int c = cptr(p.corder)[pos];
+    const int nax = p.nax;
+    double *vt = tabs;
+    double *tabC = tabs + 2 * nax;
+    double *tabF = tabs + 3 * nax;
+    double *tabW = tabs + (3 + MAXS) * nax;
+    stage_vt(p, vt);
+    const int cas = cptr(p.ccase)[c];
+    const double dt = cptr(p.dtc)[cas];
+    const double dtV = dt / cptr(p.cV)[c];
+    const int s0 = cptr(p.cs)[c];
+    const int ns = cptr(p.cs)[c + 1] - s0;
+    const CONST_AS Slot *SL = cptr(p.slots) + s0;
+    const TS *__restrict__ rowC = p.gb_in + (size_t)c * p.nvp;
+    const TS *__restrict__ hrowC = H ? p.hb_in + (size_t)c * p.nvp : nullptr;
 
     // state that formed gbar: macros of the previous step
-    const double Uc[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
-    const double qcv[3] = {p.cq[3 * c], p.cq[3 * c + 1], p.cq[3 * c + 2]};
+    const double Uc[3] = {cptr(p.cU)[3 * c], cptr(p.cU)[3 * c + 1], cptr(p.cU)[3 * c + 2]};
+    const double qcv[3] = {cptr(p.cq)[3 * c], cptr(p.cq)[3 * c + 1], cptr(p.cq)[3 * c + 2]};
     EqC ec;
-    eq_consts(p.crho[c], Uc, p.cT[c], qcv, p.R, p.Pr, D, ec);
-    const double rc = 1.5 * dt / (2.0 * p.ctau[c] + dt);
+    eq_consts(cptr(p.crho)[c], Uc, cptr(p.cT)[c], qcv, p.R, p.Pr, D, ec);
+    const double rc = 1.5 * dt / (2.0 * cptr(p.ctau)[c] + dt);
     const double inv1mr = 1.0 / (1.0 - rc);
-    for (int t = threadIdx.x; t < p.nax; t += blockDim.x) {
+    for (int t = threadIdx.x; t < nax; t += blockDim.x) {
         const int ax = t / p.nxi;
         const double u = Uc[ax] - p.axis[t - ax * p.nxi];
         tabC[t] = exp(-(u * u) * (0.5 * ec.invRT));
     }
+    // face equilibrium tables of every face and the wall Maxwellian tables of the wall faces
+    for (int t = threadIdx.x; t < ns * nax; t += blockDim.x) {
+        const int j = t / nax, a = t - j * nax;
+        const int ax = a / p.nxi;
+        const Slot &sj = p.slots[s0 + j];
+        const FaceEq &fe = p.feq[sj.face];
+        const double u = fe.U[ax] - p.axis[a - ax * p.nxi];
+        tabF[j * nax + a] = exp(-(u * u) * (0.5 * fe.invRT));
+        if (sj.kind == KIND_WALL) tabW[j * nax + a] = p.wexp[(size_t)sj.bidx * nax + a];
+    }
     __syncthreads();
 
-    double gnew[NR], hnew[NR];
     double acc[NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) acc[m] = 0.0;
-
-#pragma unroll(NIT > 0 ? NIT : 2)
-    for (int it = 0; it < NL; ++it) {
-        if (it < p.niter) {
-            const int k = threadIdx.x + it * p.nt;
-            const Vel v = load_vel(p, k);
-            const size_t ck = (size_t)c * p.nvp + k;
-            const double gb = ld(gbi, ck);
-            const double hb = H ? ld(hbi, ck) : 0.0;
-            // relaxed face values of every face of the cell, loaded together
-            double fgv[MAXS], fhv[MAXS];
-#pragma unroll
-            for (int j = 0; j < MAXS; ++j) {
-                fgv[j] = 0.0;
-                fhv[j] = 0.0;
-                if (j < ns) {
-                    const size_t fk = (size_t)p.slots[s0 + j].face * p.nvp + k;
-                    fgv[j] = ld(fg, fk);
-                    if (H) fhv[j] = ld(fh, fk);
-                }
+    const unsigned k0 = (unsigned)b * p.vb + threadIdx.x;
+    for (int it = 0; it < p.nib; ++it) {
+        const unsigned k = k0 + it * p.nt;
+        const Vel v = load_vel<D>(p, k, vt);
+        const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
+        const double gC = ld(rowC, k);
+        const double hC = H ? ld(hrowC, k) : 0.0;
+        double GC[3] = {0, 0, 0}, HC[3] = {0, 0, 0};
+        grad_at<D, H, NS>(p, c, k, gC, hC, GC, HC);
+        // g~ of this step from gbar (inverse of S1)
+        double geq, heq;
+        eq_eval<D, H>(ec, tabC, p.nxi, v, p.kc, p.K, geq, heq);
+        const double gt = (gC - rc * geq) * inv1mr;
+        const double ht = H ? (hC - rc * heq) * inv1mr : 0.0;
+        // S8 (discreteVelocity.C:937-975): -1/3 g~ + 4/3 gbar, then the faces in face order
+        double ag = -1.0 / 3 * gt + 4.0 / 3 * gC;
+        double ah = H ? (-1.0 / 3 * ht + 4.0 / 3 * hC) : 0.0;
+        for (int j = 0; j < ns; ++j) {
+            const CONST_AS Slot &S = SL[j];
+            const int kind = S.kind;
+            const double Sv[3] = {S.S[0], S.S[1], S.S[2]};
+            const double s = dotnc(v.x, v.y, v.z, Sv);
+            // roles: this cell is the face owner P except on KIND_NEI faces
+            const bool nei = kind == KIND_NEI;
+            bool needO;
+            if (kind == KIND_WALL) needO = false;
+            else if (nei) needO = need_P(KIND_OWN, s);
+            else needO = need_N(kind, s);
+            double gO = 0.0, hO = 0.0;
+            double GO[3] = {0, 0, 0}, HO[3] = {0, 0, 0};
+            if (__any(needO)) {
+                const int o = S.other;
+                gO = ld(p.gb_in + (size_t)o * p.nvp, k);
+                if (H) hO = ld(p.hb_in + (size_t)o * p.nvp, k);
+                grad_at<D, H, NS>(p, o, k, gO, hO, GO, HO);
             }
-            // g~ of this step from gbar (inverse of S1)
-            double geq, heq;
-            eq_eval<D, H>(ec, tabC, p.nxi, v, p.kc, p.K, geq, heq);
-            const double gt = (gb - rc * geq) * inv1mr;
-            const double ht = H ? (hb - rc * heq) * inv1mr : 0.0;
-            // S8 (discreteVelocity.C:937-975): -1/3 g~ + 4/3 gbar, then the faces in face order
-            double ag = -1.0 / 3 * gt + 4.0 / 3 * gb;
-            double ah = H ? (-1.0 / 3 * ht + 4.0 / 3 * hb) : 0.0;
-#pragma unroll
-            for (int j = 0; j < MAXS; ++j) {
-                if (j < ns) {
-                    const Slot &S = p.slots[s0 + j];
-                    const double s = dotnc(v.x, v.y, v.z, S.S);
-                    const double tg = s * fgv[j] * dt / V;
-                    if (S.kind == KIND_NEI) ag += tg;
-                    else ag -= tg;
-                    if (H) {
-                        const double th = s * fhv[j] * dt / V;
-                        if (S.kind == KIND_NEI) ah += th;
-                        else ah -= th;
-                    }
-                }
-            }
-            if (NIT > 0) {
-                gnew[NIT > 0 ? it : 0] = ag;
-                hnew[NIT > 0 ? it : 0] = ah;
+            const double dCs[3] = {S.dCs[0], S.dCs[1], S.dCs[2]};
+            const double dCo[3] = {S.dCo[0], S.dCo[1], S.dCo[2]};
+            const CONST_AS FaceEq &fe = cptr(p.feq)[S.face];
+            double gf, hf = 0.0;
+            if (kind == KIND_WALL && !(s > 0)) {
+                // S4 incoming Maxwellian, left unrelaxed by S7 (maxwellWall operators are no-ops)
+                const double *tw = tabW + j * nax;
+                double m = fe.pad[1] * tw[v.ix] * tw[p.nxi + v.iy];
+                if (D == 3) m *= tw[2 * p.nxi + v.iz];
+                gf = fe.rho_w * m;
+                if (H) hf = gf * (p.R * fe.pad[0]) * p.kc;
             } else {
-                p.gb_out[ck] = (TS)ag;
-                if (H) p.hb_out[ck] = (TS)ah;
+                if (nei) {
+                    gf = face_value<D>(KIND_OWN, s, gO, GO, dCo, gC, GC, dCs, 0.0, hx);
+                    if (H) hf = face_value<D>(KIND_OWN, s, hO, HO, dCo, hC, HC, dCs, 0.0, hx);
+                } else {
+                    gf = face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx);
+                    if (H) hf = face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx);
+                }
+                // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium
+                EqC e;
+                e.U[0] = fe.U[0]; e.U[1] = fe.U[1]; e.U[2] = fe.U[2];
+                e.invRT = fe.invRT; e.pre = fe.pre; e.RT = fe.RT;
+                e.qc[0] = fe.qc[0]; e.qc[1] = fe.qc[1]; e.qc[2] = fe.qc[2];
+                double gq, hq;
+                eq_eval<D, H>(e, tabF + j * nax, p.nxi, v, p.kc, p.K, gq, hq);
+                const double r = fe.r;
+                gf = (1.0 - r) * gf + r * gq;
+                if (H) hf = (1.0 - r) * hf + r * hq;
             }
-            acc_moments<D, H>(acc, ag, ah, v, Uc);
+            const double tg = s * gf * dtV;
+            if (nei) ag += tg;
+            else ag -= tg;
+            if (H) {
+                const double th = s * hf * dtV;
+                if (nei) ah += th;
+                else ah -= th;
+            }
         }
+        p.gb_out[(size_t)c * p.nvp + k] = (TS)ag;
+        if (H) p.hb_out[(size_t)c * p.nvp + k] = (TS)ah;
+        acc_moments<D, H>(acc, ag, ah, v, Uc);
     }
-
     block_sum<NC>(acc, red, tot);
-    if (threadIdx.x == 0) {
-        double rho, U[3], T, tau, q[3];
-        const double mu = p.muc[cas];
-        finish_moments<D>(tot, Uc, p.R, p.K, mu, p.omega, p.Tref, rho, U, T, tau, q);
-        const double fqc = 2.0 * tau / (2.0 * tau + dt * p.Pr);
-        for (int d = 0; d < 3; ++d) q[d] = fqc * q[d];
-        p.crho[c] = rho;
-        p.cT[c] = T;
-        p.ctau[c] = tau;
-        for (int d = 0; d < 3; ++d) {
-            p.cU[3 * c + d] = U[d];
-            p.cq[3 * c + d] = q[d];
-        }
-        if (!(rho > 0.0) || !(T > 0.0) || !isfinite(rho) || !isfinite(T) || !isfinite(tau)) p.flag[0] = 1;
-        newm[0] = rho; newm[1] = U[0]; newm[2] = U[1]; newm[3] = U[2]; newm[4] = T;
-        newm[5] = q[0]; newm[6] = q[1]; newm[7] = q[2]; newm[8] = tau;
+    if ((int)threadIdx.x < NC) p.cpart[((size_t)c * p.nb + b) * NC + threadIdx.x] = tot[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_cell_fin: one thread per cell.  Sums the block partials in block order and forms the S9 cell
+// macros (fvDVM.C:604-727): rho, U, T, tau and q with the 2 tau / (2 tau + dt Pr) factor.
+// ---------------------------------------------------------------------------------------------
+template <int D, typename TS>
+__global__ __launch_bounds__(256) void k_cell_fin(P<TS> p)
+{
+    constexpr int NC = NMom<D>::v;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nc) return;
+    const int cas = p.ccase[c];
+    const double dt = p.dtc[cas];
+    double a[NC];
+    for (int m = 0; m < NC; ++m) a[m] = 0.0;
+    const double *cp = p.cpart + (size_t)c * p.nb * NC;
+    for (int b = 0; b < p.nb; ++b)
+        for (int m = 0; m < NC; ++m) a[m] += cp[b * NC + m];
+    const double Uc[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
+    double rho, U[3], T, tau, q[3];
+    finish_moments<D>(a, Uc, p.R, p.K, p.muc[cas], p.omega, p.Tref, rho, U, T, tau, q);
+    const double fqc = 2.0 * tau / (2.0 * tau + dt * p.Pr);
+    for (int d = 0; d < 3; ++d) q[d] = fqc * q[d];
+    p.crho[c] = rho;
+    p.cT[c] = T;
+    p.ctau[c] = tau;
+    for (int d = 0; d < 3; ++d) {
+        p.cU[3 * c + d] = U[d];
+        p.cq[3 * c + d] = q[d];
     }
-    __syncthreads();
+    if (!(rho > 0.0) || !(T > 0.0) || !isfinite(rho) || !isfinite(T) || !isfinite(tau)) p.flag[0] = 1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_cell_s1: S1 of the next step (discreteVelocity.C:393-406): gbar = (1 - r) g~ + r geq with the
+// new cell macros, in place in the output buffer.  One workgroup per cell over all velocities
+// (a pure stream of the row; the exp table is built once per cell).
+// ---------------------------------------------------------------------------------------------
+template <int D, bool H, typename TS>
+__global__ __launch_bounds__(MAXNT) void k_cell_s1(P<TS> p)
+{
+    __shared__ double tabN[NAXMAX];
+    __shared__ double vt[2 * NAXMAX];
+    const int c = blockIdx.x;
+    stage_vt(p, vt);
+    const int cas = cptr(p.ccase)[c];
+    const double dt = cptr(p.dtc)[cas];
+    const double Un[3] = {cptr(p.cU)[3 * c], cptr(p.cU)[3 * c + 1], cptr(p.cU)[3 * c + 2]};
+    const double qn[3] = {cptr(p.cq)[3 * c], cptr(p.cq)[3 * c + 1], cptr(p.cq)[3 * c + 2]};
     EqC en;
-    {
-        const double Un[3] = {newm[1], newm[2], newm[3]};
-        const double qn[3] = {newm[5], newm[6], newm[7]};
-        eq_consts(newm[0], Un, newm[4], qn, p.R, p.Pr, D, en);
-        for (int t = threadIdx.x; t < p.nax; t += blockDim.x) {
-            const int ax = t / p.nxi;
-            const double u = Un[ax] - p.axis[t - ax * p.nxi];
-            tabN[t] = exp(-(u * u) * (0.5 * en.invRT));
-        }
+    eq_consts(cptr(p.crho)[c], Un, cptr(p.cT)[c], qn, p.R, p.Pr, D, en);
+    for (int t = threadIdx.x; t < p.nax; t += blockDim.x) {
+        const int ax = t / p.nxi;
+        const double u = Un[ax] - p.axis[t - ax * p.nxi];
+        tabN[t] = exp(-(u * u) * (0.5 * en.invRT));
     }
     __syncthreads();
-    // S1 of the next step (discreteVelocity.C:393-406)
-    const double rn = 1.5 * dt / (2.0 * newm[8] + dt);
-#pragma unroll(NIT > 0 ? NIT : 2)
-    for (int it = 0; it < NL; ++it) {
-        if (it < p.niter) {
-            const int k = threadIdx.x + it * p.nt;
-            const Vel v = load_vel(p, k);
-            double geq, heq;
-            eq_eval<D, H>(en, tabN, p.nxi, v, p.kc, p.K, geq, heq);
-            const size_t ck = (size_t)c * p.nvp + k;
-            const double gn = NIT > 0 ? gnew[NIT > 0 ? it : 0] : ld(p.gb_out, ck);
-            const double hn = !H ? 0.0 : (NIT > 0 ? hnew[NIT > 0 ? it : 0] : ld(p.hb_out, ck));
-            p.gb_out[ck] = (TS)((1.0 - rn) * gn + rn * geq);
-            if (H) p.hb_out[ck] = (TS)((1.0 - rn) * hn + rn * heq);
+    const double rn = 1.5 * dt / (2.0 * cptr(p.ctau)[c] + dt);
+    TS *__restrict__ go = p.gb_out + (size_t)c * p.nvp;
+    TS *__restrict__ ho = H ? p.hb_out + (size_t)c * p.nvp : nullptr;
+    // S1_UNR velocities per thread in flight: all loads of a group are issued before its stores
+    constexpr int U = S1_UNR;
+    for (unsigned k0 = threadIdx.x; k0 < (unsigned)p.nvp; k0 += U * blockDim.x) {
+        double gv[U], hv[U];
+        Vel vv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned k = k0 + u * blockDim.x;
+            gv[u] = hv[u] = 0.0;
+            if (k < (unsigned)p.nvp) {
+                vv[u] = load_vel<D>(p, k, vt);
+                gv[u] = ld(go, k);
+                if (H) hv[u] = ld(ho, k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned k = k0 + u * blockDim.x;
+            if (k < (unsigned)p.nvp) {
+                double geq, heq;
+                eq_eval<D, H>(en, tabN, p.nxi, vv[u], p.kc, p.K, geq, heq);
+                go[k] = (TS)((1.0 - rn) * gv[u] + rn * geq);
+                if (H) ho[k] = (TS)((1.0 - rn) * hv[u] + rn * heq);
+            }
         }
     }
 }
@@ -838,6 +901,8 @@ template <int D, bool H, typename TS>
 __global__ __launch_bounds__(MAXNT) void k_init_raw(P<TS> p)
 {
     __shared__ double tab[NAXMAX];
+    __shared__ double vt[2 * NAXMAX];
+    stage_vt(p, vt);
     const int c = blockIdx.x;
     const double U[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
     const double q0[3] = {0, 0, 0};
@@ -851,7 +916,7 @@ __global__ __launch_bounds__(MAXNT) void k_init_raw(P<TS> p)
     __syncthreads();
     for (int it = 0; it < p.niter; ++it) {
         const int k = threadIdx.x + it * p.nt;
-        const Vel v = load_vel(p, k);
+        const Vel v = load_vel<D>(p, k, vt);
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
@@ -866,6 +931,8 @@ template <int D, bool H, typename TS>
 __global__ __launch_bounds__(MAXNT) void k_convert(P<TS> p, const double *dt_old, const double *dt_new, int mode)
 {
     __shared__ double tab[NAXMAX];
+    __shared__ double vt[2 * NAXMAX];
+    stage_vt(p, vt);
     const int c = blockIdx.x;
     const int cas = p.ccase[c];
     const double U[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
@@ -885,7 +952,7 @@ __global__ __launch_bounds__(MAXNT) void k_convert(P<TS> p, const double *dt_old
     const double inv1mr = 1.0 / (1.0 - ro);
     for (int it = 0; it < p.niter; ++it) {
         const int k = threadIdx.x + it * p.nt;
-        const Vel v = load_vel(p, k);
+        const Vel v = load_vel<D>(p, k, vt);
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
@@ -910,6 +977,8 @@ __global__ __launch_bounds__(256) void k_wall_init(P<TS> p, const int *wall_face
 {
     constexpr int NM = NMom<D>::v;
     __shared__ double tab[NAXMAX];
+    __shared__ double vt[2 * NAXMAX];
+    stage_vt(p, vt);
     __shared__ double red[MAXWAVES * (NM + 1)];
     __shared__ double tot[NM + 1];
     const int b = blockIdx.x;
@@ -929,7 +998,7 @@ __global__ __launch_bounds__(256) void k_wall_init(P<TS> p, const int *wall_face
     for (int m = 0; m <= NM; ++m) acc[m] = 0.0;
     const double Uw[3] = {W.U[0], W.U[1], W.U[2]};
     for (int k = threadIdx.x; k < p.nvp; k += blockDim.x) {
-        const Vel v = load_vel(p, k);
+        const Vel v = load_vel<D>(p, k, vt);
         const double s = dotnc(v.x, v.y, v.z, Sf);
         double m = W.pre * tab[v.ix] * tab[p.nxi + v.iy];
         if (D == 3) m *= tab[2 * p.nxi + v.iz];
@@ -949,6 +1018,8 @@ __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, co
                                                   double *g_out, double *h_out)
 {
     __shared__ double tab[NAXMAX];
+    __shared__ double vt[2 * NAXMAX];
+    stage_vt(p, vt);
     const int c = cells[blockIdx.x];
     const int cas = p.ccase[c];
     const double U[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
@@ -965,7 +1036,7 @@ __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, co
     const double inv1mr = 1.0 / (1.0 - ro);
     for (int it = 0; it < p.niter; ++it) {
         const int k = threadIdx.x + it * p.nt;
-        const Vel v = load_vel(p, k);
+        const Vel v = load_vel<D>(p, k, vt);
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
@@ -991,7 +1062,8 @@ struct dugks_engine {
     int D = 2, nc = 0, nfi = 0, nf = 0, nb = 0, nw = 0, nv = 0, nvp = 0, nxi = 0, nax = 0, K = 0;
     int storage = DUGKS_STORAGE_FP64, n_cases = 1;
     bool has_h = true;
-    int nt = 64, niter = 1, nit_max = 1, fmom_nit = 0;
+    int nt = 64, niter = 1, nib = 1, nvb = 1, vb = 64;  // velocity blocks: nvb x (vb = nt * nib)
+    int ns_max = 0;      // largest face count of a cell
     double R = 0, omega = 0, Tref = 0, mu_ref = 0, Pr = 0, xi_max = 0;
     std::vector<int> perm;          // device velocity -> reference velocity, -1 for padding
     std::vector<double> dcoef_int;  // internal deltaCoeffs (Courant number)
@@ -1002,7 +1074,8 @@ struct dugks_engine {
     std::vector<int> ncell_slots;
 
     // device buffers
-    VelRec *vel = nullptr;
+    uint32_t *vidx = nullptr;
+    double *axw = nullptr;
     double *axis = nullptr;
     int *cs = nullptr, *ccase = nullptr, *flag = nullptr, *wfslot = nullptr;
     Slot *slots = nullptr;
@@ -1012,20 +1085,21 @@ struct dugks_engine {
     FaceEq *feq = nullptr;
     WallC *wall = nullptr;
     double *wexp = nullptr, *qwall = nullptr, *swall = nullptr;
-    FaceInfo *finfo = nullptr;
-    void *fg = nullptr, *fh = nullptr;
+    CellSt *cst = nullptr;
+    FaceX *fx = nullptr;
+    int *corder = nullptr, *forder = nullptr;
+    double *fpart = nullptr, *cpart = nullptr;
     void *gb[2] = {nullptr, nullptr}, *hb[2] = {nullptr, nullptr};
     void *gB[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
     int cur = 0;         // buffer holding the current state
-    int gcur = 0;        // boundary-gradient buffer to read
     int raw = 1;         // state holds g~ (1) or gbar prepared with dt_prep (0)
     int initialised = 0;
     std::vector<double> dt_prep;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int profiling = 0;
-    std::vector<hipEvent_t> pev;     // 4 events per profiled step
-    double prof_recon = 0.0, prof_fmom = 0.0, prof_cells = 0.0;
+    std::vector<hipEvent_t> pev;     // NKERN + 1 events per profiled step
+    double prof[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int64_t prof_steps = 0;
     std::string err;
 };
@@ -1085,73 +1159,57 @@ P<TS> params(dugks_engine *h)
     P<TS> p;
     p.D = h->D; p.nc = h->nc; p.nf = h->nf; p.nvp = h->nvp; p.nxi = h->nxi; p.nax = h->nax; p.K = h->K;
     p.nt = h->nt; p.niter = h->niter; p.nw = h->nw;
+    p.nib = h->nib; p.nb = h->nvb; p.vb = h->vb;
     p.R = h->R; p.Pr = h->Pr; p.omega = h->omega; p.Tref = h->Tref; p.kc = (double)(h->K + 3 - h->D);
-    p.vel = h->vel;
+    p.vidx = h->vidx; p.axw = h->axw;
     p.axis = h->axis; p.cs = h->cs; p.slots = h->slots; p.cV = h->cV; p.ccase = h->ccase;
     p.dtc = h->dtc; p.muc = h->muc;
     p.crho = h->crho; p.cU = h->cU; p.cT = h->cT; p.cq = h->cq; p.ctau = h->ctau;
     p.frho = h->frho; p.fU = h->fU; p.fT = h->fT; p.fq = h->fq; p.feq = h->feq;
-    p.wall = h->wall; p.wexp = h->wexp; p.qwall = h->qwall; p.swall = h->swall; p.finfo = h->finfo;
-    p.fg = (TS *)h->fg; p.fh = (TS *)h->fh;
+    p.wall = h->wall; p.wexp = h->wexp; p.qwall = h->qwall; p.swall = h->swall;
+    p.cst = h->cst; p.fx = h->fx;
+    p.corder = h->corder; p.forder = h->forder; p.fpart = h->fpart; p.cpart = h->cpart;
     p.gb_in = (const TS *)h->gb[h->cur]; p.hb_in = (const TS *)h->hb[h->cur];
     p.gb_out = (TS *)h->gb[1 - h->cur]; p.hb_out = (TS *)h->hb[1 - h->cur];
-    p.gBg_in = (const TS *)h->gB[h->gcur]; p.gBh_in = (const TS *)h->hB[h->gcur];
-    p.gBg_out = (TS *)h->gB[1 - h->gcur]; p.gBh_out = (TS *)h->hB[1 - h->gcur];
+    p.gBg_in = (const TS *)h->gB[h->cur]; p.gBh_in = (const TS *)h->hB[h->cur];
+    p.gBg_out = (TS *)h->gB[1 - h->cur]; p.gBh_out = (TS *)h->hB[1 - h->cur];
     p.flag = h->flag;
     return p;
 }
 
 // ---- launch dispatch over (D, H, TS, NIT) ----
 
-template <int D, bool H, typename TS, int NIT>
-void launch_cells_t(dugks_engine *h, const P<TS> &p)
-{
-    hipLaunchKernelGGL((k_cells<D, H, TS, NIT>), dim3(h->nc), dim3(h->nt), 0, h->stream, p);
-}
 
-template <int D, bool H, typename TS>
-void launch_cells_n(dugks_engine *h, const P<TS> &p)
-{
-    switch (h->nit_max) {
-    case 1: launch_cells_t<D, H, TS, 1>(h, p); break;
-    case 2: launch_cells_t<D, H, TS, 2>(h, p); break;
-    case 4: launch_cells_t<D, H, TS, 4>(h, p); break;
-    case 8: launch_cells_t<D, H, TS, 8>(h, p); break;
-    default: launch_cells_t<D, H, TS, 0>(h, p); break;
-    }
-}
+size_t cell_lds(const dugks_engine *h) { return sizeof(double) * (3 + 2 * MAXS) * (size_t)h->nax; }
 
-template <int D, bool H, typename TS, int NIT>
-void launch_fmom_t(dugks_engine *h, const P<TS> &p)
-{
-    hipLaunchKernelGGL((k_fmom<D, H, TS, NIT>), dim3(h->nf), dim3(h->nt), 0, h->stream, p);
-}
+dim3 block_grid(int n, int nb) { return dim3((unsigned)(((n + NXCD - 1) / NXCD) * NXCD), (unsigned)nb); }
 
-template <int D, bool H, typename TS>
-void launch_fmom_n(dugks_engine *h, const P<TS> &p)
-{
-    switch (h->fmom_nit) {
-    case 1: launch_fmom_t<D, H, TS, 1>(h, p); break;
-    case 2: launch_fmom_t<D, H, TS, 2>(h, p); break;
-    case 4: launch_fmom_t<D, H, TS, 4>(h, p); break;
-    case 8: launch_fmom_t<D, H, TS, 8>(h, p); break;
-    case 16: launch_fmom_t<D, H, TS, 16>(h, p); break;
-    case 32: launch_fmom_t<D, H, TS, 32>(h, p); break;
-    default: launch_fmom_t<D, H, TS, 0>(h, p); break;
-    }
-}
+constexpr int NKERN = 5;  // k_face, k_face_fin, k_cell, k_cell_fin, k_cell_s1
+constexpr int S1_NT = 256;  // workgroup size of k_cell_s1
 
-template <int D, bool H, typename TS>
-void launch_step_t(dugks_engine *h, hipEvent_t *ev)
+template <int D, bool H, int NS, typename TS>
+void launch_step_ns(dugks_engine *h, hipEvent_t *ev)
 {
     P<TS> p = params<TS>(h);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    hipLaunchKernelGGL((k_recon<D, H, TS>), dim3(h->nc), dim3(h->nt), 0, h->stream, p);
+    hipLaunchKernelGGL((k_face<D, H, NS, TS>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
-    launch_fmom_n<D, H, TS>(h, p);
+    hipLaunchKernelGGL((k_face_fin<D, TS>), dim3((h->nf + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
-    launch_cells_n<D, H, TS>(h, p);
+    hipLaunchKernelGGL((k_cell<D, H, NS, TS>), block_grid(h->nc, h->nvb), dim3(h->nt), cell_lds(h), h->stream, p);
     if (ev) (void)hipEventRecord(ev[3], h->stream);
+    hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
+    if (ev) (void)hipEventRecord(ev[4], h->stream);
+    hipLaunchKernelGGL((k_cell_s1<D, H, TS>), dim3(h->nc), dim3(S1_NT), 0, h->stream, p);
+    if (ev) (void)hipEventRecord(ev[5], h->stream);
+}
+
+// stencil sizes: 4 (triangles, quads) or 8 in 2-D, 6 (hexahedra, tetrahedra) or 8 in 3-D
+template <int D, bool H, typename TS>
+void launch_step_t(dugks_engine *h, hipEvent_t *ev)
+{
+    if (h->ns_max <= (D == 2 ? 4 : 6)) launch_step_ns<D, H, D == 2 ? 4 : 6, TS>(h, ev);
+    else launch_step_ns<D, H, MAXS, TS>(h, ev);
 }
 
 template <typename F>
@@ -1218,31 +1276,75 @@ struct ExtractOp {
     }
 };
 
-void choose_launch(int nv, int &nt, int &niter, int &nvp)
+// Velocity blocks: nb blocks of vb = nt * nib velocities, nt a multiple of 64 up to MAXNT and
+// nib <= NIBMAX.  Least padding first, then a block size close to 4096 velocities (long enough to
+// amortise a workgroup's set-up and reduction, measured on the 48^3 x 24^3 cavity), then the
+// workgroup size closest to 256.
+// DUGKS_NT / DUGKS_NIB in the environment pin the workgroup size / iterations (tuning).
+void choose_blocks(int nv, int &nt, int &nib, int &nb, int &nvp)
 {
-    int best_pad = 1 << 30, best_it = 0, best_nt = 0;
-    for (int it = 1; it <= 64; ++it) {
-        int t = (nv + it - 1) / it;
-        t = (t + 63) / 64 * 64;
-        if (t > MAXNT) continue;
-        int pad = t * it - nv;
-        if (pad < best_pad) {
-            best_pad = pad;
-            best_it = it;
-            best_nt = t;
+    long best_pad = 1L << 40;
+    double best_sz = 1e30, best_wg = 1e30;
+    const char *ent = std::getenv("DUGKS_NT"), *eit = std::getenv("DUGKS_NIB");
+    const int pin_t = ent ? std::atoi(ent) : 0, pin_it = eit ? std::atoi(eit) : 0;
+    for (int t = 64; t <= MAXNT; t += 64)
+        for (int it = 1; it <= NIBMAX; ++it) {
+            if ((pin_t > 0 && t != pin_t) || (pin_it > 0 && it != pin_it)) continue;
+            const int vb = t * it;
+            const int b = (nv + vb - 1) / vb;
+            const long pad = (long)b * vb - nv;
+            const double sz = std::fabs(std::log((double)vb / 4096.0));
+            const double wg = std::fabs(t - 256.0);
+            const bool better = pad < best_pad || (pad == best_pad && (sz < best_sz - 1e-12 ||
+                                                  (std::fabs(sz - best_sz) <= 1e-12 && wg < best_wg)));
+            if (better) {
+                best_pad = pad;
+                best_sz = sz;
+                best_wg = wg;
+                nt = t;
+                nib = it;
+                nb = b;
+            }
         }
-    }
-    nt = best_nt;
-    niter = best_it;
-    nvp = nt * niter;
+    nvp = nb * nt * nib;
 }
 
-// k_cells variant: registers for up to 8 velocities per thread, else the output-row round trip (0)
-int nit_bucket(int niter)
+// Launch order: Morton (Z-order) key of the centres, case-major for a batch of cases.
+std::vector<int> morton_order(int n, const double *X, const int *group)
 {
-    int b = 1;
-    while (b < niter) b <<= 1;
-    return b <= 8 ? b : 0;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], X[3 * i + d]);
+            hi[d] = std::max(hi[d], X[3 * i + d]);
+        }
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    std::vector<uint64_t> key(n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t q[3];
+        for (int d = 0; d < 3; ++d) {
+            const double ext = hi[d] - lo[d];
+            const double u = ext > 0 ? (X[3 * i + d] - lo[d]) / ext : 0.0;
+            q[d] = (uint64_t)std::min(2097151.0, std::max(0.0, std::floor(u * 2097151.0)));
+        }
+        key[i] = spread(q[0]) | spread(q[1]) << 1 | spread(q[2]) << 2;
+    }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        const int ga = group ? group[a] : 0, gb = group ? group[b] : 0;
+        if (ga != gb) return ga < gb;
+        return key[a] < key[b];
+    });
+    return ord;
 }
 
 double mag3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
@@ -1469,21 +1571,37 @@ int build(dugks_engine *h, const dugks_case *cs)
             sl[e] = s;
         }
     }
-    // ---- per-face data of k_fmom ----
-    std::vector<FaceInfo> finfo(nf);
+    // ---- per-face data of k_face ----
+    std::vector<FaceX> fxs(nf);
     for (int f = 0; f < nf; ++f) {
-        FaceInfo fi;
-        std::memset(&fi, 0, sizeof fi);
-        for (int d = 0; d < 3; ++d) fi.S[d] = cs->Sf[3 * f + d];
-        fi.own = cs->owner[f];
-        fi.kind = KIND_OWN;
-        fi.bidx = -1;
-        if (f >= nfi) {
-            const int b = f - nfi;
-            fi.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : KIND_CYC;
-            fi.bidx = bkind[b] == DUGKS_PATCH_WALL ? bwall[b] : b;
+        FaceX x;
+        std::memset(&x, 0, sizeof x);
+        const int o = cs->owner[f];
+        for (int d = 0; d < 3; ++d) {
+            x.S[d] = cs->Sf[3 * f + d];
+            x.dP[d] = cs->Cf[3 * f + d] - Cc[3 * o + d];
         }
-        finfo[f] = fi;
+        x.P = o;
+        x.N = -1;
+        x.kind = KIND_OWN;
+        x.bidx = -1;
+        if (f < nfi) {
+            x.N = cs->neighbour[f];
+            for (int d = 0; d < 3; ++d) x.dN[d] = cs->Cf[3 * f + d] - Cc[3 * x.N + d];
+        } else {
+            const int b = f - nfi;
+            if (bkind[b] == DUGKS_PATCH_WALL) {
+                x.kind = KIND_WALL;
+                x.bidx = bwall[b];
+                for (int d = 0; d < 3; ++d) x.dN[d] = cs->Sf[3 * f + d] / magSf[f];
+            } else {
+                x.kind = KIND_CYC;
+                x.bidx = b;
+                x.N = cs->owner[nfi + bpartner[b]];
+                x.w = w[f];
+            }
+        }
+        fxs[f] = x;
     }
 
     // ---- velocities: tensor product in reference order, then grouped by sign pattern ----
@@ -1491,16 +1609,15 @@ int build(dugks_engine *h, const dugks_case *cs)
     if (nxi < 2 || nxi > 64) return fail(h, DUGKS_ERR_UNSUPPORTED, "n_xi must be in [2, 64]");
     if (h->nax > NAXMAX) return fail(h, DUGKS_ERR_UNSUPPORTED, "velocity table too large");
     const int nv = h->nv;
-    std::vector<double> rx(nv), ry(nv), rz(nv), rw(nv);
+    std::vector<double> rx(nv), ry(nv), rz(nv);
     std::vector<int> rid(nv);
     {
         int i = 0;
-        const double *X = cs->xi, *W = cs->xi_weights;
+        const double *X = cs->xi;
         if (D == 3) {
             for (int iz = 0; iz < nxi; ++iz)
                 for (int iy = 0; iy < nxi; ++iy)
                     for (int ix = 0; ix < nxi; ++ix) {
-                        rw[i] = W[iz] * W[iy] * W[ix];
                         rx[i] = X[ix]; ry[i] = X[iy]; rz[i] = X[iz];
                         rid[i] = ix | (iy << 8) | (iz << 16);
                         ++i;
@@ -1508,7 +1625,6 @@ int build(dugks_engine *h, const dugks_case *cs)
         } else {
             for (int iy = 0; iy < nxi; ++iy)
                 for (int ix = 0; ix < nxi; ++ix) {
-                    rw[i] = W[iy] * W[ix] * 1;
                     rx[i] = X[ix]; ry[i] = X[iy]; rz[i] = 0.0;
                     rid[i] = ix | (iy << 8);
                     ++i;
@@ -1523,29 +1639,51 @@ int build(dugks_engine *h, const dugks_case *cs)
         int kb = cls(rz[b]) * 9 + cls(ry[b]) * 3 + cls(rx[b]);
         return ka < kb;
     });
-    choose_launch(nv, h->nt, h->niter, h->nvp);
-    if (h->nt <= 0 || h->nt > MAXNT || h->niter > 64 || h->nt * h->niter < nv)
-        return fail(h, DUGKS_ERR_UNSUPPORTED, "velocity count too large for one workgroup per cell");
-    h->nit_max = nit_bucket(h->niter);
-    {
-        int b = 1;
-        while (b < h->niter) b <<= 1;
-        h->fmom_nit = b <= 32 ? b : 0;
+    choose_blocks(nv, h->nt, h->nib, h->nvb, h->nvp);
+    h->vb = h->nt * h->nib;
+    h->niter = h->nvb * h->nib;
+    if (h->nt <= 0 || h->nt > MAXNT || h->nvp < nv)
+        return fail(h, DUGKS_ERR_UNSUPPORTED, "velocity blocking failed");
+    // one dispatch holds at most 2^32 - 1 work-items per grid dimension
+    if ((double)((std::max(nc, nf) + NXCD - 1) / NXCD * NXCD) * h->nt >= 4294967295.0)
+        return fail(h, DUGKS_ERR_UNSUPPORTED, "mesh too large for one dispatch per kernel");
+    // ---- least-squares stencils (entry j = slot j; rows: cells, then walls) ----
+    std::vector<CellSt> cst(nc);
+    h->ns_max = 0;
+    for (int c = 0; c < nc; ++c) {
+        CellSt t;
+        std::memset(&t, 0, sizeof t);
+        const int ns = cnt[c + 1] - cnt[c];
+        h->ns_max = std::max(h->ns_max, ns);
+        for (int j = 0; j < MAXS; ++j) {
+            t.off[j] = (int64_t)c * h->nvp;
+            if (j >= ns) continue;
+            const Slot &q = sl[cnt[c] + j];
+            for (int d = 0; d < 3; ++d) t.L[j][d] = q.L[d];
+            if (q.kind == KIND_WALL) {
+                t.off[j] = (int64_t)(nc + q.bidx) * h->nvp;
+                t.a[j] = q.aux;
+                t.wmask |= 1 << j;
+            } else {
+                t.off[j] = (int64_t)q.other * h->nvp;
+            }
+        }
+        cst[c] = t;
     }
     const int nvp = h->nvp;
-    std::vector<VelRec> vel(nvp);
-    std::memset(vel.data(), 0, sizeof(VelRec) * nvp);
+    std::vector<uint32_t> vidx(nvp, VPAD);
     h->perm.assign(nvp, -1);
     for (int k = 0; k < nv; ++k) {
         const int r = order[k];
-        vel[k].x = rx[r]; vel[k].y = ry[r]; vel[k].z = rz[r]; vel[k].w = rw[r];
-        vel[k].xi2 = rx[r] * rx[r] + ry[r] * ry[r] + rz[r] * rz[r];
-        vel[k].idx = rid[r];
+        vidx[k] = (uint32_t)rid[r];
         h->perm[k] = r;
     }
-    std::vector<double> axis(h->nax);
+    std::vector<double> axis(h->nax), axw(h->nax);
     for (int a = 0; a < D; ++a)
-        for (int i = 0; i < nxi; ++i) axis[a * nxi + i] = cs->xi[i];
+        for (int i = 0; i < nxi; ++i) {
+            axis[a * nxi + i] = cs->xi[i];
+            axw[a * nxi + i] = cs->xi_weights[i];
+        }
 
     // ---- per-case data ----
     std::vector<int> ccase(nc, 0);
@@ -1563,6 +1701,11 @@ int build(dugks_engine *h, const dugks_case *cs)
             if (ccase[cs->owner[f]] != ccase[cs->neighbour[f]])
                 return fail(h, DUGKS_ERR_INVALID, "a face joins two cases");
     }
+    // ---- launch orders ----
+    const std::vector<int> corder = morton_order(nc, Cc, ccase.data());
+    std::vector<int> fcase(nf);
+    for (int f = 0; f < nf; ++f) fcase[f] = ccase[cs->owner[f]];
+    const std::vector<int> forder = morton_order(nf, cs->Cf, fcase.data());
     h->face_w.assign(w, w + nf);
     h->dcoef_int.resize(nfi);
     for (int f = 0; f < nfi; ++f) {
@@ -1596,7 +1739,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
     int r = 0;
-    if ((r = dalloc(h, &h->vel, nvp)) ||
+    if ((r = dalloc(h, &h->vidx, nvp)) || (r = dalloc(h, &h->axw, h->nax)) ||
         (r = dalloc(h, &h->axis, h->nax)) || (r = dalloc(h, &h->cs, nc + 1)) || (r = dalloc(h, &h->slots, nslot)) ||
         (r = dalloc(h, &h->cV, nc)) || (r = dalloc(h, &h->ccase, nc)) || (r = dalloc(h, &h->dtc, h->n_cases)) ||
         (r = dalloc(h, &h->dtp, h->n_cases)) || (r = dalloc(h, &h->muc, h->n_cases)) ||
@@ -1606,24 +1749,27 @@ int build(dugks_engine *h, const dugks_case *cs)
         (r = dalloc(h, &h->feq, nf)) ||
         (r = dalloc(h, &h->wall, nw)) || (r = dalloc(h, &h->wexp, (size_t)nw * h->nax)) ||
         (r = dalloc(h, &h->qwall, 3 * (size_t)nw)) || (r = dalloc(h, &h->swall, 9 * (size_t)nw)) ||
-        (r = dalloc(h, &h->finfo, nf)) || (r = dalloc(h, &h->flag, 1)) || (r = dalloc(h, &h->wfslot, nw)))
+        (r = dalloc(h, &h->fx, nf)) || (r = dalloc(h, &h->cst, nc)) || (r = dalloc(h, &h->corder, nc)) ||
+        (r = dalloc(h, &h->forder, nf)) || (r = dalloc(h, &h->fpart, (size_t)nf * h->nvb * (NMom<3>::v + 1))) ||
+        (r = dalloc(h, &h->cpart, (size_t)nc * h->nvb * NMom<3>::v)) || (r = dalloc(h, &h->flag, 1)) || (r = dalloc(h, &h->wfslot, nw)))
         return r;
     const size_t es = ts_size(h);
+    // one allocation per ping-pong side: nc cell rows, then nw wall rows (boundary gradient)
     const size_t row = (size_t)nc * nvp * es, brow = (size_t)(nw ? nw : 1) * nvp * es;
-    const size_t frow = (size_t)nf * nvp * es;
-    if ((r = dallocb(h, &h->fg, frow))) return r;
-    if (h->has_h && (r = dallocb(h, &h->fh, frow))) return r;
     for (int i = 0; i < 2; ++i) {
-        if ((r = dallocb(h, &h->gb[i], row))) return r;
-        if (h->has_h && (r = dallocb(h, &h->hb[i], row))) return r;
-        if ((r = dallocb(h, &h->gB[i], brow))) return r;
-        if (h->has_h && (r = dallocb(h, &h->hB[i], brow))) return r;
+        if ((r = dallocb(h, &h->gb[i], row + brow))) return r;
+        h->gB[i] = (char *)h->gb[i] + row;
+        if (h->has_h) {
+            if ((r = dallocb(h, &h->hb[i], row + brow))) return r;
+            h->hB[i] = (char *)h->hb[i] + row;
+        }
     }
-    if ((r = upload(h, h->vel, vel.data(), nvp)) ||
+    if ((r = upload(h, h->vidx, vidx.data(), nvp)) || (r = upload(h, h->axw, axw.data(), h->nax)) ||
         (r = upload(h, h->axis, axis.data(), h->nax)) || (r = upload(h, h->cs, cnt.data(), nc + 1)) ||
         (r = upload(h, h->slots, sl.data(), nslot)) || (r = upload(h, h->cV, cs->V, nc)) ||
         (r = upload(h, h->ccase, ccase.data(), nc)) || (r = upload(h, h->muc, muc.data(), h->n_cases)) ||
-        (r = upload(h, h->finfo, finfo.data(), nf)) || (r = upload(h, h->wfslot, wfslot.data(), nw)))
+        (r = upload(h, h->fx, fxs.data(), nf)) || (r = upload(h, h->cst, cst.data(), nc)) ||
+        (r = upload(h, h->corder, corder.data(), nc)) || (r = upload(h, h->forder, forder.data(), nf)) || (r = upload(h, h->wfslot, wfslot.data(), nw)))
         return r;
     if (nw && (r = upload(h, h->wall, wc.data(), nw))) return r;
     HIPCHK(h, hipMemset(h->flag, 0, sizeof(int)));
@@ -1643,14 +1789,15 @@ void release(dugks_engine *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->vel); dfree(h->axis);
+    dfree(h->vidx); dfree(h->axw); dfree(h->axis);
     dfree(h->cs); dfree(h->ccase); dfree(h->flag); dfree(h->wfslot); dfree(h->slots); dfree(h->cV);
     dfree(h->dtc); dfree(h->dtp); dfree(h->muc); dfree(h->crho); dfree(h->cU); dfree(h->cT); dfree(h->cq);
     dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->feq);
-    dfree(h->wall); dfree(h->wexp); dfree(h->qwall); dfree(h->swall); dfree(h->finfo);
-    dfree(h->fg); dfree(h->fh);
+    dfree(h->wall); dfree(h->wexp); dfree(h->qwall); dfree(h->swall); dfree(h->fx); dfree(h->cst);
+    dfree(h->corder); dfree(h->forder); dfree(h->fpart); dfree(h->cpart);
     for (int i = 0; i < 2; ++i) {
-        dfree(h->gb[i]); dfree(h->hb[i]); dfree(h->gB[i]); dfree(h->hB[i]);
+        dfree(h->gb[i]); dfree(h->hb[i]);
+        h->gB[i] = h->hB[i] = nullptr;
     }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1691,6 +1838,11 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
     if (!same && nsteps > 0) {
         ConvertOp op{h, h->raw ? 0 : 1};
         dispatch(h, op);
+        // the boundary-gradient rows travel with the cell rows
+        const size_t brow = (size_t)(h->nw ? h->nw : 1) * h->nvp * ts_size(h);
+        HIPCHK(h, hipMemcpyAsync(h->gB[1 - h->cur], h->gB[h->cur], brow, hipMemcpyDeviceToDevice, h->stream));
+        if (h->has_h)
+            HIPCHK(h, hipMemcpyAsync(h->hB[1 - h->cur], h->hB[h->cur], brow, hipMemcpyDeviceToDevice, h->stream));
         h->cur = 1 - h->cur;
         HIPCHK(h, hipGetLastError());
         h->raw = 0;
@@ -1698,17 +1850,16 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
         HIPCHK(h, hipMemcpyAsync(h->dtp, h->dt_host.data(), sizeof(double) * h->n_cases, hipMemcpyHostToDevice, h->stream));
     }
     if (h->profiling) {
-        while ((int)h->pev.size() < 4 * nsteps) {
+        while (h->pev.size() < (size_t)(NKERN + 1) * nsteps) {
             hipEvent_t e;
             HIPCHK(h, hipEventCreate(&e));
             h->pev.push_back(e);
         }
     }
     for (int s = 0; s < nsteps; ++s) {
-        StepOp op{h, h->profiling ? &h->pev[4 * (size_t)s] : nullptr};
+        StepOp op{h, h->profiling ? &h->pev[(NKERN + 1) * (size_t)s] : nullptr};
         dispatch(h, op);
         h->cur = 1 - h->cur;
-        h->gcur = 1 - h->gcur;
     }
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
@@ -1718,14 +1869,12 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
     h->last_ms = ms;
     if (h->profiling) {
         for (int s = 0; s < nsteps; ++s) {
-            float a = 0.f, b = 0.f, c = 0.f;
-            const hipEvent_t *e = &h->pev[4 * (size_t)s];
-            HIPCHK(h, hipEventElapsedTime(&a, e[0], e[1]));
-            HIPCHK(h, hipEventElapsedTime(&b, e[1], e[2]));
-            HIPCHK(h, hipEventElapsedTime(&c, e[2], e[3]));
-            h->prof_recon += a;
-            h->prof_fmom += b;
-            h->prof_cells += c;
+            const hipEvent_t *e = &h->pev[(NKERN + 1) * (size_t)s];
+            for (int i = 0; i < NKERN; ++i) {
+                float a = 0.f;
+                HIPCHK(h, hipEventElapsedTime(&a, e[i], e[i + 1]));
+                h->prof[i] += a;
+            }
         }
         h->prof_steps += nsteps;
     }
@@ -1825,7 +1974,6 @@ int dugks_init_equilibrium(dugks_engine *h, const double *rho, const double *U, 
     (void)nfi;
     if ((r = upload(h, h->fU, fU.data(), 3 * (size_t)nf))) return r;
     h->cur = 0;
-    h->gcur = 0;
     const size_t es = ts_size(h);
     const size_t brow = (size_t)(h->nw ? h->nw : 1) * h->nvp * es;
     for (int i = 0; i < 2; ++i) {
@@ -2000,8 +2148,8 @@ int dugks_save_state(dugks_engine *h, void *buf, int64_t bytes)
     const size_t row = (size_t)h->nc * h->nvp * es, brow = (size_t)(h->nw ? h->nw : 1) * h->nvp * es;
     HIPCHK(h, hipMemcpy(p, h->gb[h->cur], row, hipMemcpyDeviceToHost)); p += row;
     if (h->has_h) { HIPCHK(h, hipMemcpy(p, h->hb[h->cur], row, hipMemcpyDeviceToHost)); p += row; }
-    HIPCHK(h, hipMemcpy(p, h->gB[h->gcur], brow, hipMemcpyDeviceToHost)); p += brow;
-    if (h->has_h) { HIPCHK(h, hipMemcpy(p, h->hB[h->gcur], brow, hipMemcpyDeviceToHost)); p += brow; }
+    HIPCHK(h, hipMemcpy(p, h->gB[h->cur], brow, hipMemcpyDeviceToHost)); p += brow;
+    if (h->has_h) { HIPCHK(h, hipMemcpy(p, h->hB[h->cur], brow, hipMemcpyDeviceToHost)); p += brow; }
     const size_t nc = h->nc;
     HIPCHK(h, hipMemcpy(p, h->crho, 8 * nc, hipMemcpyDeviceToHost)); p += 8 * nc;
     HIPCHK(h, hipMemcpy(p, h->cU, 24 * nc, hipMemcpyDeviceToHost)); p += 24 * nc;
@@ -2033,8 +2181,8 @@ int dugks_load_state(dugks_engine *h, const void *buf, int64_t bytes)
     const size_t row = (size_t)h->nc * h->nvp * es, brow = (size_t)(h->nw ? h->nw : 1) * h->nvp * es;
     HIPCHK(h, hipMemcpy(h->gb[h->cur], p, row, hipMemcpyHostToDevice)); p += row;
     if (h->has_h) { HIPCHK(h, hipMemcpy(h->hb[h->cur], p, row, hipMemcpyHostToDevice)); p += row; }
-    HIPCHK(h, hipMemcpy(h->gB[h->gcur], p, brow, hipMemcpyHostToDevice)); p += brow;
-    if (h->has_h) { HIPCHK(h, hipMemcpy(h->hB[h->gcur], p, brow, hipMemcpyHostToDevice)); p += brow; }
+    HIPCHK(h, hipMemcpy(h->gB[h->cur], p, brow, hipMemcpyHostToDevice)); p += brow;
+    if (h->has_h) { HIPCHK(h, hipMemcpy(h->hB[h->cur], p, brow, hipMemcpyHostToDevice)); p += brow; }
     const size_t nc = h->nc;
     HIPCHK(h, hipMemcpy(h->crho, p, 8 * nc, hipMemcpyHostToDevice)); p += 8 * nc;
     HIPCHK(h, hipMemcpy(h->cU, p, 24 * nc, hipMemcpyHostToDevice)); p += 24 * nc;
@@ -2055,11 +2203,21 @@ int dugks_last_step_ms(dugks_engine *h, double *ms)
     return 0;
 }
 
+int dugks_get_launch(dugks_engine *h, int32_t *nt, int32_t *nib, int32_t *n_blocks, int32_t *nv_padded)
+{
+    if (!h) return DUGKS_ERR_INVALID;
+    if (nt) *nt = h->nt;
+    if (nib) *nib = h->nib;
+    if (n_blocks) *n_blocks = h->nvb;
+    if (nv_padded) *nv_padded = h->nvp;
+    return 0;
+}
+
 int dugks_set_profiling(dugks_engine *h, int32_t on)
 {
     if (!h) return DUGKS_ERR_INVALID;
     h->profiling = on ? 1 : 0;
-    h->prof_recon = h->prof_fmom = h->prof_cells = 0.0;
+    for (double &x : h->prof) x = 0.0;
     h->prof_steps = 0;
     return 0;
 }
@@ -2068,9 +2226,7 @@ int dugks_get_profile(dugks_engine *h, double *ms, int64_t *steps)
 {
     if (!h) return DUGKS_ERR_INVALID;
     if (ms) {
-        ms[0] = h->prof_recon;
-        ms[1] = h->prof_fmom;
-        ms[2] = h->prof_cells;
+        for (int i = 0; i < NKERN; ++i) ms[i] = h->prof[i];
     }
     if (steps) *steps = h->prof_steps;
     return 0;

@@ -21,6 +21,7 @@ SYMBOLS = (
     "dugks_get_face_macro", "dugks_get_wall", "dugks_get_courant", "dugks_get_df", "dugks_get_sizes",
     "dugks_state_size", "dugks_save_state", "dugks_load_state", "dugks_last_step_ms", "dugks_destroy",
     "dugks_last_error", "dugks_abi_version", "dugks_set_profiling", "dugks_get_profile",
+    "dugks_get_launch",
 )
 
 _lib = None
@@ -62,6 +63,7 @@ def load_library(path: str = LIB_PATH):
     L.dugks_abi_version.restype = C.c_int
     L.dugks_set_profiling.argtypes = [vp, C.c_int32]
     L.dugks_get_profile.argtypes = [vp, dp, C.POINTER(C.c_int64)]
+    L.dugks_get_launch.argtypes = [vp, ip, ip, ip, ip]
     _lib = L
     return L
 
@@ -174,14 +176,20 @@ class Engine:
         """HIP-event timing of the two step kernels on the engine stream."""
         self._chk(load_library().dugks_set_profiling(self._h, 1 if on else 0))
 
-    KERNELS = ("k_recon", "k_fmom", "k_cells")
+    KERNELS = ("k_face", "k_face_fin", "k_cell", "k_cell_fin", "k_cell_s1")
 
     def profile(self):
         """({kernel: accumulated ms}, steps) since set_profiling(True)."""
-        ms = np.zeros(3)
+        ms = np.zeros(len(self.KERNELS))
         n = C.c_int64()
         self._chk(load_library().dugks_get_profile(self._h, dptr(ms), C.byref(n)))
         return dict(zip(self.KERNELS, ms.tolist())), n.value
+
+    def blocks(self):
+        """(workgroup size, velocity iterations per thread, velocity blocks, padded Nv)."""
+        v = [C.c_int32() for _ in range(4)]
+        self._chk(load_library().dugks_get_launch(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     @property
     def last_step_ms(self) -> float:

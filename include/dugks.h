@@ -143,6 +143,11 @@ int dugks_get_df(dugks_engine* h, int32_t n, const int32_t* cells, double* g, do
 /* Number of velocities, cells and faces. */
 int dugks_get_sizes(dugks_engine* h, int32_t* nv, int32_t* n_cells, int32_t* n_faces);
 
+/* Work decomposition chosen at create (no reference counterpart; for measurement and
+ * tuning): workgroup size, velocity iterations per thread in a block, number of
+ * velocity blocks and the padded velocity count nt * nib * n_blocks.  NULL skips. */
+int dugks_get_launch(dugks_engine* h, int32_t* nt, int32_t* nib, int32_t* n_blocks, int32_t* nv_padded);
+
 /* Complete resident state as an opaque byte image: resuming from it continues
  * bit-identically.  dugks_state_size gives the byte count. */
 int dugks_state_size(dugks_engine* h, int64_t* bytes);
@@ -155,9 +160,11 @@ int dugks_last_step_ms(dugks_engine* h, double* ms);
 
 /* Per-kernel timing (no reference counterpart; for measurement).  With
  * profiling on, every step records HIP events on the engine stream around its
- * three kernels; dugks_get_profile returns in ms[3] the accumulated device time
- * of the reconstruction kernel (S2), the face-moment kernel (S3-S6) and the cell
- * kernel (S7-S9 + next S1), and the number of steps since profiling was switched on. */
+ * five kernels; dugks_get_profile returns in ms[5] the accumulated device time
+ * of k_face (S2 reconstruction + S6 face-moment sums), k_face_fin (S3 wall
+ * density, face macros), k_cell (S7 relaxation, S8 update, S9 moment sums),
+ * k_cell_fin (S9 cell macros) and k_cell_s1 (S1 of the next step), and the
+ * number of steps since profiling was switched on. */
 int dugks_set_profiling(dugks_engine* h, int32_t on);
 int dugks_get_profile(dugks_engine* h, double* ms, int64_t* steps);
 
