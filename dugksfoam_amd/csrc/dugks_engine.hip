@@ -64,6 +64,15 @@ constexpr int MAXNT = 512;     // workgroup size cap (256 VGPRs per lane availab
 constexpr int MAXWAVES = MAXNT / 64;
 constexpr int NIBMAX = 64;     // velocity iterations per thread in a block
 constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
+// Minimum resident waves per SIMD asked of the register allocator (measured on the 48^3 x 24^3
+// cavity: 6 for k_face costs a few spill bytes and gains 4 %; 5 for k_cell spills in its face
+// loop and loses 35 %).
+#ifndef FACE_WAVES
+#define FACE_WAVES 6
+#endif
+#ifndef CELL_WAVES
+#define CELL_WAVES 4
+#endif
 
 // One (cell, face) incidence; a cell's slots are in ascending face order, the order in which the
 // reference's face loops touch that cell.
@@ -159,6 +168,7 @@ struct P {
     double *qwall, *swall;
     const CellSt *__restrict__ cst;
     const FaceX *__restrict__ fx;
+    const double *__restrict__ moff;  // FP32 storage: offsets [g: nvp | h: nvp] (see row_off)
     const int *__restrict__ corder;  // launch order of the cells
     const int *__restrict__ forder;  // launch order of the faces
     double *fpart;      // [face][block][moment] partial sums of k_face
@@ -194,6 +204,18 @@ template <typename TS>
 DEVI double ld(const TS *a, size_t i)
 {
     return (double)a[i];
+}
+
+// FP32 storage keeps cell rows as the deviation from a fixed rest Maxwellian M_k (density and
+// temperature: volume means of the initial state): value = stored + M_k.  FP32 rounding then
+// acts on the small deviation only.  Least-squares gradients and wall rows are unaffected (a
+// constant per velocity), so they work on the stored deviations directly.  FP64 storage keeps
+// the values themselves (offset 0).
+template <typename TS>
+DEVI double row_off(const P<TS> &p, unsigned k, int which)
+{
+    if (sizeof(TS) == 8) return 0.0;
+    return p.moff[(size_t)which * p.nvp + k];
 }
 
 struct Vel {
@@ -311,6 +333,16 @@ DEVI void grad_at(const P<TS> &p, int cell, unsigned k, double pg, double ph, do
             if (H) Hg[d] = fma(st.L[j][d], dh, Hg[d]);
         }
     }
+}
+
+// a / b for a per-cell constant b with y = 1/b (rounded): q = a y, then one Newton correction with
+// the exact residual a - b q (an FMA) gives the correctly rounded quotient (Markstein, IBM J. Res.
+// Dev. 34 (1990) 111-119), so the S8 flux term s g dt / V rounds as the reference's division does.
+DEVI double div_rn(double a, double b, double y)
+{
+    const double q = a * y;
+    const double r = fma(-q, b, a);
+    return fma(r, y, q);
 }
 
 template <int D>
@@ -502,7 +534,7 @@ DEVI bool xcd_block(int n, int nb, int &pos, int &b)
 // step's fixedGradient boundary gradient n . grad(gbar) (discreteVelocity.C:436-470).
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, int NS, typename TS>
-__global__ __launch_bounds__(MAXNT) void k_face(P<TS> p)
+__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FACE_WAVES, 8))) void k_face(P<TS> p)
 {
     constexpr int NM = NMom<D>::v + 1;  // + wall outGoing
     __shared__ double red[MAXWAVES * NM];
@@ -567,8 +599,8 @@ __global__ __launch_bounds__(MAXNT) void k_face(P<TS> p)
             }
         }
         if (!wall || s > 0) {
-            const double gf = face_value<D>(kind, s, gP, GP, dP, gN, GN, dN, wcy, hx);
-            const double hf = H ? face_value<D>(kind, s, hP, HP, dP, hN, HN, dN, wcy, hx) : 0.0;
+            const double gf = face_value<D>(kind, s, gP, GP, dP, gN, GN, dN, wcy, hx) + row_off(p, k, 0);
+            const double hf = H ? face_value<D>(kind, s, hP, HP, dP, hN, HN, dN, wcy, hx) + row_off(p, k, 1) : 0.0;
             acc_moments<D, H>(acc, gf, hf, v, Uh);
             if (wall) acc[NM - 1] = fma(v.w * s, gf, acc[NM - 1]);
         }
@@ -670,7 +702,7 @@ __global__ __launch_bounds__(256) void k_face_fin(P<TS> p)
 // Dynamic LDS: [velocity table (2 nax) | tabC | tabF x MAXS | tabW x MAXS], nax doubles each.
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, int NS, typename TS>
-__global__ __launch_bounds__(MAXNT) void k_cell(P<TS> p)
+__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVES, 8))) void k_cell(P<TS> p)
 {
     constexpr int NC = NMom<D>::v;
     extern __shared__ double tabs[];
@@ -688,7 +720,8 @@ __global__ __launch_bounds__(MAXNT) void k_cell(P<TS> p)
     stage_vt(p, vt);
     const int cas = cptr(p.ccase)[c];
     const double dt = cptr(p.dtc)[cas];
-    const double dtV = dt / cptr(p.cV)[c];
+    const double Vc = cptr(p.cV)[c];
+    const double invV = 1.0 / Vc;
     const int s0 = cptr(p.cs)[c];
     const int ns = cptr(p.cs)[c + 1] - s0;
     const CONST_AS Slot *SL = cptr(p.slots) + s0;
@@ -727,18 +760,20 @@ __global__ __launch_bounds__(MAXNT) void k_cell(P<TS> p)
         const unsigned k = k0 + it * p.nt;
         const Vel v = load_vel<D>(p, k, vt);
         const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
-        const double gC = ld(rowC, k);
+        const double gC = ld(rowC, k);                 // stored value (FP32: deviation)
         const double hC = H ? ld(hrowC, k) : 0.0;
+        const double og = row_off(p, k, 0), oh = H ? row_off(p, k, 1) : 0.0;
+        const double gCv = gC + og, hCv = hC + oh;      // gbar itself
         double GC[3] = {0, 0, 0}, HC[3] = {0, 0, 0};
         grad_at<D, H, NS>(p, c, k, gC, hC, GC, HC);
         // g~ of this step from gbar (inverse of S1)
         double geq, heq;
         eq_eval<D, H>(ec, tabC, p.nxi, v, p.kc, p.K, geq, heq);
-        const double gt = (gC - rc * geq) * inv1mr;
-        const double ht = H ? (hC - rc * heq) * inv1mr : 0.0;
+        const double gt = (gCv - rc * geq) * inv1mr;
+        const double ht = H ? (hCv - rc * heq) * inv1mr : 0.0;
         // S8 (discreteVelocity.C:937-975): -1/3 g~ + 4/3 gbar, then the faces in face order
-        double ag = -1.0 / 3 * gt + 4.0 / 3 * gC;
-        double ah = H ? (-1.0 / 3 * ht + 4.0 / 3 * hC) : 0.0;
+        double ag = -1.0 / 3 * gt + 4.0 / 3 * gCv;
+        double ah = H ? (-1.0 / 3 * ht + 4.0 / 3 * hCv) : 0.0;
         for (int j = 0; j < ns; ++j) {
             const CONST_AS Slot &S = SL[j];
             const int kind = S.kind;
@@ -771,11 +806,11 @@ __global__ __launch_bounds__(MAXNT) void k_cell(P<TS> p)
                 if (H) hf = gf * (p.R * fe.pad[0]) * p.kc;
             } else {
                 if (nei) {
-                    gf = face_value<D>(KIND_OWN, s, gO, GO, dCo, gC, GC, dCs, 0.0, hx);
-                    if (H) hf = face_value<D>(KIND_OWN, s, hO, HO, dCo, hC, HC, dCs, 0.0, hx);
+                    gf = face_value<D>(KIND_OWN, s, gO, GO, dCo, gC, GC, dCs, 0.0, hx) + og;
+                    if (H) hf = face_value<D>(KIND_OWN, s, hO, HO, dCo, hC, HC, dCs, 0.0, hx) + oh;
                 } else {
-                    gf = face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx);
-                    if (H) hf = face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx);
+                    gf = face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx) + og;
+                    if (H) hf = face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx) + oh;
                 }
                 // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium
                 EqC e;
@@ -788,17 +823,17 @@ __global__ __launch_bounds__(MAXNT) void k_cell(P<TS> p)
                 gf = (1.0 - r) * gf + r * gq;
                 if (H) hf = (1.0 - r) * hf + r * hq;
             }
-            const double tg = s * gf * dtV;
+            const double tg = div_rn(s * gf * dt, Vc, invV);
             if (nei) ag += tg;
             else ag -= tg;
             if (H) {
-                const double th = s * hf * dtV;
+                const double th = div_rn(s * hf * dt, Vc, invV);
                 if (nei) ah += th;
                 else ah -= th;
             }
         }
-        p.gb_out[(size_t)c * p.nvp + k] = (TS)ag;
-        if (H) p.hb_out[(size_t)c * p.nvp + k] = (TS)ah;
+        p.gb_out[(size_t)c * p.nvp + k] = (TS)(ag - og);
+        if (H) p.hb_out[(size_t)c * p.nvp + k] = (TS)(ah - oh);
         acc_moments<D, H>(acc, ag, ah, v, Uc);
     }
     block_sum<NC>(acc, red, tot);
@@ -885,8 +920,9 @@ __global__ __launch_bounds__(MAXNT) void k_cell_s1(P<TS> p)
             if (k < (unsigned)p.nvp) {
                 double geq, heq;
                 eq_eval<D, H>(en, tabN, p.nxi, vv[u], p.kc, p.K, geq, heq);
-                go[k] = (TS)((1.0 - rn) * gv[u] + rn * geq);
-                if (H) ho[k] = (TS)((1.0 - rn) * hv[u] + rn * heq);
+                const double og = row_off(p, k, 0), oh = H ? row_off(p, k, 1) : 0.0;
+                go[k] = (TS)((1.0 - rn) * (gv[u] + og) + rn * geq - og);
+                if (H) ho[k] = (TS)((1.0 - rn) * (hv[u] + oh) + rn * heq - oh);
             }
         }
     }
@@ -920,8 +956,8 @@ __global__ __launch_bounds__(MAXNT) void k_init_raw(P<TS> p)
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
-        p.gb_out[ck] = (TS)geq;
-        if (H) p.hb_out[ck] = (TS)heq;
+        p.gb_out[ck] = (TS)(geq - row_off(p, k, 0));
+        if (H) p.hb_out[ck] = (TS)(heq - row_off(p, k, 1));
     }
 }
 
@@ -956,7 +992,8 @@ __global__ __launch_bounds__(MAXNT) void k_convert(P<TS> p, const double *dt_old
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
-        double g = ld(p.gb_in, ck), h = H ? ld(p.hb_in, ck) : 0.0;
+        const double og = row_off(p, k, 0), oh = H ? row_off(p, k, 1) : 0.0;
+        double g = ld(p.gb_in, ck) + og, h = H ? ld(p.hb_in, ck) + oh : 0.0;
         if (mode != 0) {
             g = (g - ro * geq) * inv1mr;
             h = (h - ro * heq) * inv1mr;
@@ -965,8 +1002,8 @@ __global__ __launch_bounds__(MAXNT) void k_convert(P<TS> p, const double *dt_old
             g = (1.0 - rn) * g + rn * geq;
             h = (1.0 - rn) * h + rn * heq;
         }
-        p.gb_out[ck] = (TS)g;
-        if (H) p.hb_out[ck] = (TS)h;
+        p.gb_out[ck] = (TS)(g - og);
+        if (H) p.hb_out[ck] = (TS)(h - oh);
     }
 }
 
@@ -1040,7 +1077,7 @@ __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, co
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
-        double g = ld(p.gb_in, ck), h = H ? ld(p.hb_in, ck) : 0.0;
+        double g = ld(p.gb_in, ck) + row_off(p, k, 0), h = H ? ld(p.hb_in, ck) + row_off(p, k, 1) : 0.0;
         if (!raw) {
             g = (g - ro * geq) * inv1mr;
             h = (h - ro * heq) * inv1mr;
@@ -1066,6 +1103,9 @@ struct dugks_engine {
     int ns_max = 0;      // largest face count of a cell
     double R = 0, omega = 0, Tref = 0, mu_ref = 0, Pr = 0, xi_max = 0;
     std::vector<int> perm;          // device velocity -> reference velocity, -1 for padding
+    std::vector<double> vxyz;       // device-order velocities [nvp][3] (padding: 0)
+    double off_rho = 0.0, off_T = 0.0;  // FP32 storage: rest Maxwellian of the row offsets
+    double *moff = nullptr;
     std::vector<double> dcoef_int;  // internal deltaCoeffs (Courant number)
     std::vector<double> face_w;     // surfaceInterpolation weights
     std::vector<double> dt_host;    // staging of the per-case dt
@@ -1096,6 +1136,10 @@ struct dugks_engine {
     int initialised = 0;
     std::vector<double> dt_prep;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // captured two-step graphs, one per starting ping-pong side (kernel arguments hold the
+    // buffer pointers of that side; dt is read from device memory, so no re-capture on change)
+    int use_graphs = 1;
+    hipGraphExec_t g2[2] = {nullptr, nullptr};
     double last_ms = 0.0;
     int profiling = 0;
     std::vector<hipEvent_t> pev;     // NKERN + 1 events per profiled step
@@ -1161,7 +1205,7 @@ P<TS> params(dugks_engine *h)
     p.nt = h->nt; p.niter = h->niter; p.nw = h->nw;
     p.nib = h->nib; p.nb = h->nvb; p.vb = h->vb;
     p.R = h->R; p.Pr = h->Pr; p.omega = h->omega; p.Tref = h->Tref; p.kc = (double)(h->K + 3 - h->D);
-    p.vidx = h->vidx; p.axw = h->axw;
+    p.vidx = h->vidx; p.axw = h->axw; p.moff = h->moff;
     p.axis = h->axis; p.cs = h->cs; p.slots = h->slots; p.cV = h->cV; p.ccase = h->ccase;
     p.dtc = h->dtc; p.muc = h->muc;
     p.crho = h->crho; p.cU = h->cU; p.cT = h->cT; p.cq = h->cq; p.ctau = h->ctau;
@@ -1673,10 +1717,12 @@ int build(dugks_engine *h, const dugks_case *cs)
     const int nvp = h->nvp;
     std::vector<uint32_t> vidx(nvp, VPAD);
     h->perm.assign(nvp, -1);
+    h->vxyz.assign(3 * (size_t)nvp, 0.0);
     for (int k = 0; k < nv; ++k) {
         const int r = order[k];
         vidx[k] = (uint32_t)rid[r];
         h->perm[k] = r;
+        h->vxyz[3 * k] = rx[r]; h->vxyz[3 * k + 1] = ry[r]; h->vxyz[3 * k + 2] = rz[r];
     }
     std::vector<double> axis(h->nax), axw(h->nax);
     for (int a = 0; a < D; ++a)
@@ -1739,7 +1785,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
     int r = 0;
-    if ((r = dalloc(h, &h->vidx, nvp)) || (r = dalloc(h, &h->axw, h->nax)) ||
+    if ((r = dalloc(h, &h->vidx, nvp)) || (r = dalloc(h, &h->axw, h->nax)) || (r = dalloc(h, &h->moff, 2 * (size_t)nvp)) ||
         (r = dalloc(h, &h->axis, h->nax)) || (r = dalloc(h, &h->cs, nc + 1)) || (r = dalloc(h, &h->slots, nslot)) ||
         (r = dalloc(h, &h->cV, nc)) || (r = dalloc(h, &h->ccase, nc)) || (r = dalloc(h, &h->dtc, h->n_cases)) ||
         (r = dalloc(h, &h->dtp, h->n_cases)) || (r = dalloc(h, &h->muc, h->n_cases)) ||
@@ -1789,7 +1835,7 @@ void release(dugks_engine *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->vidx); dfree(h->axw); dfree(h->axis);
+    dfree(h->vidx); dfree(h->axw); dfree(h->moff); dfree(h->axis);
     dfree(h->cs); dfree(h->ccase); dfree(h->flag); dfree(h->wfslot); dfree(h->slots); dfree(h->cV);
     dfree(h->dtc); dfree(h->dtp); dfree(h->muc); dfree(h->crho); dfree(h->cU); dfree(h->cT); dfree(h->cq);
     dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->feq);
@@ -1799,6 +1845,10 @@ void release(dugks_engine *h)
         dfree(h->gb[i]); dfree(h->hb[i]);
         h->gB[i] = h->hB[i] = nullptr;
     }
+    for (hipGraphExec_t &g : h->g2) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     for (hipEvent_t e : h->pev) (void)hipEventDestroy(e);
@@ -1807,6 +1857,25 @@ void release(dugks_engine *h)
     h->ev0 = h->ev1 = nullptr;
     h->stream = nullptr;
     h->initialised = 0;
+}
+
+// Row offsets of the FP32 storage: the rest Maxwellian g = rho (2 pi R T)^(-D/2) exp(-|xi|^2 / 2RT)
+// and h = g R T (K + 3 - D) at (off_rho, off_T); zero for FP64 storage and padding velocities.
+int upload_offsets(dugks_engine *h)
+{
+    std::vector<double> m(2 * (size_t)h->nvp, 0.0);
+    if (h->storage == DUGKS_STORAGE_FP32) {
+        const double RT = h->R * h->off_T, tp = 2.0 * PI * RT;
+        const double pre = h->off_rho / (h->D == 3 ? tp * std::sqrt(tp) : tp);
+        for (int k = 0; k < h->nvp; ++k) {
+            if (h->perm[k] < 0) continue;
+            const double *x = &h->vxyz[3 * (size_t)k];
+            const double g = pre * std::exp(-(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) / (2.0 * RT));
+            m[k] = g;
+            m[h->nvp + k] = h->has_h ? g * RT * (h->K + 3 - h->D) : 0.0;
+        }
+    }
+    return upload(h, h->moff, m.data(), m.size());
 }
 
 int check_flag(dugks_engine *h)
@@ -1856,7 +1925,35 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
             h->pev.push_back(e);
         }
     }
-    for (int s = 0; s < nsteps; ++s) {
+    int s0 = 0;
+    if (!h->profiling && h->use_graphs && nsteps >= 2) {
+        // replay a captured pair of steps; an odd last step runs eagerly below
+        hipGraphExec_t &ex = h->g2[h->cur];
+        if (!ex) {
+            const int cur0 = h->cur;
+            hipGraph_t g = nullptr;
+            HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            for (int i = 0; i < 2; ++i) {
+                StepOp op{h, nullptr};
+                dispatch(h, op);
+                h->cur = 1 - h->cur;
+            }
+            const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+            h->cur = cur0;
+            if (ce != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(h, DUGKS_ERR_DEVICE, std::string("step graph capture: ") + hipGetErrorString(ce));
+            }
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                ex = nullptr;
+                return fail(h, DUGKS_ERR_DEVICE, std::string("step graph instantiate: ") + hipGetErrorString(ie));
+            }
+        }
+        for (; s0 + 2 <= nsteps; s0 += 2) HIPCHK(h, hipGraphLaunch(ex, h->stream));
+    }
+    for (int s = s0; s < nsteps; ++s) {
         StepOp op{h, h->profiling ? &h->pev[(NKERN + 1) * (size_t)s] : nullptr};
         dispatch(h, op);
         h->cur = 1 - h->cur;
@@ -1973,6 +2070,19 @@ int dugks_init_equilibrium(dugks_engine *h, const double *rho, const double *U, 
     }
     (void)nfi;
     if ((r = upload(h, h->fU, fU.data(), 3 * (size_t)nf))) return r;
+    {
+        std::vector<double> V(nc);
+        HIPCHK(h, hipMemcpy(V.data(), h->cV, sizeof(double) * nc, hipMemcpyDeviceToHost));
+        double sv = 0.0, sr = 0.0, sT = 0.0;
+        for (int c = 0; c < nc; ++c) {
+            sv += V[c];
+            sr += V[c] * rho[c];
+            sT += V[c] * T[c];
+        }
+        h->off_rho = sr / sv;
+        h->off_T = sT / sv;
+        if ((r = upload_offsets(h))) return r;
+    }
     h->cur = 0;
     const size_t es = ts_size(h);
     const size_t brow = (size_t)(h->nw ? h->nw : 1) * h->nvp * es;
@@ -2142,6 +2252,8 @@ int dugks_save_state(dugks_engine *h, void *buf, int64_t bytes)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     char *p = (char *)buf;
     int64_t hdr[16] = {STATE_MAGIC, h->nc, h->nvp, h->nw, h->nf, h->storage, h->has_h, h->raw, h->n_cases, h->D, 0, 0, 0, 0, 0, 0};
+    std::memcpy(&hdr[10], &h->off_rho, sizeof(double));   // FP32 row offsets travel with the rows
+    std::memcpy(&hdr[11], &h->off_T, sizeof(double));
     std::memcpy(p, hdr, sizeof hdr);
     p += sizeof hdr;
     const size_t es = ts_size(h);
@@ -2192,6 +2304,9 @@ int dugks_load_state(dugks_engine *h, const void *buf, int64_t bytes)
     HIPCHK(h, hipMemcpy(h->fU, p, 24 * (size_t)h->nf, hipMemcpyHostToDevice)); p += 24 * (size_t)h->nf;
     std::memcpy(h->dt_prep.data(), p, 8 * h->n_cases);
     h->raw = (int)hdr[7];
+    std::memcpy(&h->off_rho, &hdr[10], sizeof(double));
+    std::memcpy(&h->off_T, &hdr[11], sizeof(double));
+    if (int rr = upload_offsets(h)) return rr;
     HIPCHK(h, hipMemcpy(h->dtp, h->dt_prep.data(), 8 * h->n_cases, hipMemcpyHostToDevice));
     return 0;
 }
@@ -2200,6 +2315,13 @@ int dugks_last_step_ms(dugks_engine *h, double *ms)
 {
     if (!h || !ms) return DUGKS_ERR_INVALID;
     *ms = h->last_ms;
+    return 0;
+}
+
+int dugks_set_graphs(dugks_engine *h, int32_t on)
+{
+    if (!h) return DUGKS_ERR_INVALID;
+    h->use_graphs = on ? 1 : 0;
     return 0;
 }
 

@@ -21,7 +21,7 @@ SYMBOLS = (
     "dugks_get_face_macro", "dugks_get_wall", "dugks_get_courant", "dugks_get_df", "dugks_get_sizes",
     "dugks_state_size", "dugks_save_state", "dugks_load_state", "dugks_last_step_ms", "dugks_destroy",
     "dugks_last_error", "dugks_abi_version", "dugks_set_profiling", "dugks_get_profile",
-    "dugks_get_launch",
+    "dugks_get_launch", "dugks_set_graphs",
 )
 
 _lib = None
@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
     L.dugks_set_profiling.argtypes = [vp, C.c_int32]
     L.dugks_get_profile.argtypes = [vp, dp, C.POINTER(C.c_int64)]
     L.dugks_get_launch.argtypes = [vp, ip, ip, ip, ip]
+    L.dugks_set_graphs.argtypes = [vp, C.c_int32]
     _lib = L
     return L
 
@@ -184,6 +185,10 @@ class Engine:
         n = C.c_int64()
         self._chk(load_library().dugks_get_profile(self._h, dptr(ms), C.byref(n)))
         return dict(zip(self.KERNELS, ms.tolist())), n.value
+
+    def set_graphs(self, on: bool = True):
+        """Replay captured two-step HIP graphs for multi-step calls (default on)."""
+        self._chk(load_library().dugks_set_graphs(self._h, 1 if on else 0))
 
     def blocks(self):
         """(workgroup size, velocity iterations per thread, velocity blocks, padded Nv)."""

@@ -143,6 +143,11 @@ int dugks_get_df(dugks_engine* h, int32_t n, const int32_t* cells, double* g, do
 /* Number of velocities, cells and faces. */
 int dugks_get_sizes(dugks_engine* h, int32_t* nv, int32_t* n_cells, int32_t* n_faces);
 
+/* Step-loop graphs (no reference counterpart): with graphs on (the default) dugks_step
+ * with nsteps >= 2 replays a captured HIP graph of two steps instead of launching each
+ * kernel; results are identical either way.  Profiling runs eagerly. */
+int dugks_set_graphs(dugks_engine* h, int32_t on);
+
 /* Work decomposition chosen at create (no reference counterpart; for measurement and
  * tuning): workgroup size, velocity iterations per thread in a block, number of
  * velocity blocks and the padded velocity count nt * nib * n_blocks.  NULL skips. */

@@ -48,6 +48,15 @@
 
 #include "../include/dugks.h"
 
+/* Type of the velocity-space sums: double, as in the reference.  The extended-precision build
+ * (-DORACLE_ACC_T="long double" -mfpmath=387) sums in 80-bit, so its distance from this build
+ * measures the round-off of the reference's sequential FP64 sums as well. */
+#ifndef ORACLE_ACC_T
+#define ORACLE_ACC_T double
+#endif
+typedef ORACLE_ACC_T acc_t;
+
+
 #define VSMALL 1.0e-300
 #define PI_ 3.14159265358979323846
 
@@ -89,10 +98,20 @@ typedef struct oracle {
     double *bgrad_g, *bgrad_h; /* cyclic boundary gradient values [nb*3] */
     double *newgb_g, *newgb_h; /* [nb] new fixedGradient values           */
     double *cache_g, *cache_h; /* [nv][nf] cached pre-S4 face values      */
-    double *outgoing;          /* [nb] */
-    double *acc_rho, *acc_rhoU, *acc_rhoE; /* [nf] / [nf*3] */
-    double *cacc_rho, *cacc_rhoU, *cacc_rhoE; /* [nc] / [nc*3] */
+    acc_t *outgoing;           /* [nb] */
+    acc_t *acc_rho, *acc_rhoU, *acc_rhoE; /* [nf] / [nf*3] */
+    acc_t *cacc_rho, *cacc_rhoU, *cacc_rhoE; /* [nc] / [nc*3] */
+    acc_t *qacc_f, *qacc_c;    /* [nf*3] / [nc*3] heat-flux sums */
 } oracle;
+
+/* Velocity order of the step's passes.  The reference sums in k order; building with
+ * -DORACLE_KREV reverses every pass (same arithmetic, other summation order), which measures
+ * how far a field moves under an equally valid FP64 evaluation order (the parity floor). */
+#ifdef ORACLE_KREV
+#define K_LOOP(k, nv) for (int k = (nv) - 1; k >= 0; --k)
+#else
+#define K_LOOP(k, nv) for (int k = 0; k < (nv); ++k)
+#endif
 
 static void *xmalloc(size_t n) { void *p = calloc(n ? n : 1, 1); return p; }
 
@@ -332,11 +351,12 @@ oracle *oracle_create(const dugks_case *cs)
     o->bgrad_g = (double *)xmalloc(sizeof(double)*3*(nb + 1)); o->bgrad_h = (double *)xmalloc(sizeof(double)*3*(nb + 1));
     o->gs = (double *)xmalloc(sizeof(double)*nf); o->hs = (double *)xmalloc(sizeof(double)*nf);
     o->newgb_g = (double *)xmalloc(sizeof(double)*(nb + 1)); o->newgb_h = (double *)xmalloc(sizeof(double)*(nb + 1));
-    o->outgoing = (double *)xmalloc(sizeof(double)*(nb + 1));
-    o->acc_rho = (double *)xmalloc(sizeof(double)*nf); o->acc_rhoU = (double *)xmalloc(sizeof(double)*3*nf);
-    o->acc_rhoE = (double *)xmalloc(sizeof(double)*nf);
-    o->cacc_rho = (double *)xmalloc(sizeof(double)*nc); o->cacc_rhoU = (double *)xmalloc(sizeof(double)*3*nc);
-    o->cacc_rhoE = (double *)xmalloc(sizeof(double)*nc);
+    o->outgoing = (acc_t *)xmalloc(sizeof(acc_t)*(nb + 1));
+    o->acc_rho = (acc_t *)xmalloc(sizeof(acc_t)*nf); o->acc_rhoU = (acc_t *)xmalloc(sizeof(acc_t)*3*nf);
+    o->acc_rhoE = (acc_t *)xmalloc(sizeof(acc_t)*nf);
+    o->cacc_rho = (acc_t *)xmalloc(sizeof(acc_t)*nc); o->cacc_rhoU = (acc_t *)xmalloc(sizeof(acc_t)*3*nc);
+    o->cacc_rhoE = (acc_t *)xmalloc(sizeof(acc_t)*nc);
+    o->qacc_f = (acc_t *)xmalloc(sizeof(acc_t)*3*nf); o->qacc_c = (acc_t *)xmalloc(sizeof(acc_t)*3*nc);
     return o;
 }
 
@@ -350,7 +370,7 @@ void oracle_destroy(oracle *o)
                     o->U_f, o->T_f, o->q_f, o->tau_f, o->qwall, o->stresswall, o->gbar, o->hbar, o->phib_g,
                     o->phib_h, o->grad_g, o->grad_h, o->gs, o->hs, o->bgrad_g, o->bgrad_h, o->newgb_g,
                     o->newgb_h, o->cache_g, o->cache_h, o->outgoing, o->acc_rho, o->acc_rhoU, o->acc_rhoE,
-                    o->cacc_rho, o->cacc_rhoU, o->cacc_rhoE};
+                    o->cacc_rho, o->cacc_rhoU, o->cacc_rhoE, o->qacc_f, o->qacc_c};
     for (size_t i = 0; i < sizeof ptrs/sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(o);
 }
@@ -594,7 +614,7 @@ static int do_step(oracle *o)
         /* ---- pass A: S1, S2 and the wall outgoing flux (discreteVelocity.C:605-627) ---- */
         #pragma omp for schedule(static)
         for (int b = 0; b < nb; ++b) o->outgoing[b] = 0;
-        for (int k = 0; k < nv; ++k) {
+        K_LOOP(k, nv) {
             compute_k(o, k, 1);
             const double *xi = o->xi + 3*k;
             #pragma omp for schedule(static)
@@ -620,7 +640,7 @@ static int do_step(oracle *o)
             o->acc_rho[f] = 0; o->acc_rhoE[f] = 0;
             o->acc_rhoU[3*f] = o->acc_rhoU[3*f + 1] = o->acc_rhoU[3*f + 2] = 0;
         }
-        for (int k = 0; k < nv; ++k) {
+        K_LOOP(k, nv) {
             double *gs, *hs;
             faces_k(o, k, &gs, &hs);
             const double *xi = o->xi + 3*k;
@@ -642,7 +662,7 @@ static int do_step(oracle *o)
             o->T_f[f] = (o->acc_rhoE[f] - 0.5*r*magsqr3(o->U_f + 3*f))/((K + 3)/2.0*o->R*r);
             int own = o->owner[f];
             o->tau_f[f] = tau_of(o, mu_of_cell(o, own), o->T_f[f], r);
-            o->q_f[3*f] = o->q_f[3*f + 1] = o->q_f[3*f + 2] = 0;
+            o->qacc_f[3*f] = o->qacc_f[3*f + 1] = o->qacc_f[3*f + 2] = 0;
         }
         #pragma omp for schedule(static)
         for (int b = 0; b < nb; ++b) {
@@ -650,7 +670,7 @@ static int do_step(oracle *o)
             for (int d = 0; d < 9; ++d) o->stresswall[9*b + d] = 0;
         }
         /* ---- pass C: face heat flux (fvDVM.C:503-516) and wall diagnostics (555-570) ---- */
-        for (int k = 0; k < nv; ++k) {
+        K_LOOP(k, nv) {
             double *gs, *hs;
             faces_k(o, k, &gs, &hs);
             const double *xi = o->xi + 3*k;
@@ -659,7 +679,7 @@ static int do_step(oracle *o)
             for (int f = 0; f < nf; ++f) {
                 double c[3] = {xi[0] - o->U_f[3*f], xi[1] - o->U_f[3*f + 1], xi[2] - o->U_f[3*f + 2]};
                 double a = magsqr3(c)*gs[f] + hs[f];
-                for (int d = 0; d < 3; ++d) o->q_f[3*f + d] += 0.5*wk*c[d]*a;
+                for (int d = 0; d < 3; ++d) o->qacc_f[3*f + d] += 0.5*wk*c[d]*a;
                 if (f >= nfi && o->bkind[f - nfi] == DUGKS_PATCH_WALL) {
                     int b = f - nfi;
                     double cw[3] = {xi[0] - o->Ub[3*b], xi[1] - o->Ub[3*b + 1], xi[2] - o->Ub[3*b + 2]};
@@ -678,7 +698,7 @@ static int do_step(oracle *o)
             double dt = dt_of_cell(o, own);
             double t2 = 2.0*o->tau_f[f];
             double fac = t2/(t2 + 0.5*dt*o->Pr);
-            for (int d = 0; d < 3; ++d) o->q_f[3*f + d] = fac*o->q_f[3*f + d];
+            for (int d = 0; d < 3; ++d) o->q_f[3*f + d] = fac*(double)o->qacc_f[3*f + d];
             if (f >= nfi && o->bkind[f - nfi] == DUGKS_PATCH_WALL) {
                 int b = f - nfi;
                 double tp = o->tau_f[f];
@@ -692,7 +712,7 @@ static int do_step(oracle *o)
             o->cacc_rho[c] = 0; o->cacc_rhoE[c] = 0;
             o->cacc_rhoU[3*c] = o->cacc_rhoU[3*c + 1] = o->cacc_rhoU[3*c + 2] = 0;
         }
-        for (int k = 0; k < nv; ++k) {
+        K_LOOP(k, nv) {
             double *gs, *hs;
             if (o->cache_faces) compute_k(o, k, 0);   /* gbar, new fixedGradient */
             faces_k(o, k, &gs, &hs);
@@ -755,11 +775,11 @@ static int do_step(oracle *o)
             for (int d = 0; d < 3; ++d) o->U[3*c + d] = o->cacc_rhoU[3*c + d]/r;
             o->T[c] = (rE - 0.5*r*magsqr3(o->U + 3*c))/((K + 3)/2.0*o->R*r);
             o->tau[c] = tau_of(o, mu_of_cell(o, c), o->T[c], r);
-            o->q[3*c] = o->q[3*c + 1] = o->q[3*c + 2] = 0;
+            o->qacc_c[3*c] = o->qacc_c[3*c + 1] = o->qacc_c[3*c + 2] = 0;
             if (!(r > 0) || !isfinite(o->T[c])) bad = 1;
         }
         /* ---- pass E: cell heat flux (fvDVM.C:712-721) ---- */
-        for (int k = 0; k < nv; ++k) {
+        K_LOOP(k, nv) {
             const double *xi = o->xi + 3*k;
             const double wk = o->wv[k];
             const double *gt = o->gt + (size_t)k*nc, *ht = o->ht + (size_t)k*nc;
@@ -767,14 +787,14 @@ static int do_step(oracle *o)
             for (int c = 0; c < nc; ++c) {
                 double cc[3] = {xi[0] - o->U[3*c], xi[1] - o->U[3*c + 1], xi[2] - o->U[3*c + 2]};
                 double a = magsqr3(cc)*gt[c] + ht[c];
-                for (int d = 0; d < 3; ++d) o->q[3*c + d] += 0.5*wk*cc[d]*a;
+                for (int d = 0; d < 3; ++d) o->qacc_c[3*c + d] += 0.5*wk*cc[d]*a;
             }
         }
         #pragma omp for schedule(static)
         for (int c = 0; c < nc; ++c) {
             double dt = dt_of_cell(o, c);
             double fac = 2.0*o->tau[c]/(2.0*o->tau[c] + dt*o->Pr);
-            for (int d = 0; d < 3; ++d) o->q[3*c + d] = fac*o->q[3*c + d];
+            for (int d = 0; d < 3; ++d) o->q[3*c + d] = fac*(double)o->qacc_c[3*c + d];
         }
     }
     (void)D;

@@ -6,6 +6,17 @@
 
 Each fixture stores rho, U, T, q at a few step counts and the total-mass
 history, so a GPU run can be compared without re-running the CPU solver.
+
+    python tests/golden/make_golden.py channel --floors
+
+adds to an existing fixture, for its snapshots, the round-off floor of
+each field: the relative L2 distance between the FP64 oracle and two equally
+valid evaluations of the same arithmetic -- the oracle built with
+extended-precision intermediates and velocity-space sums (floor_ext_*) and
+with every velocity pass summed in reverse order (floor_rev_*).  Where a floor exceeds the parity bar,
+the bar cannot be resolved in FP64 arithmetic (fields that are sums cancelling
+almost to zero, e.g. the velocity one step after rest, or the heat flux of
+cells along two walls).
 """
 import os
 import sys
@@ -20,10 +31,39 @@ from dugksfoam_amd import cases  # noqa: E402
 from oracle.oracle import Oracle  # noqa: E402
 
 PLANS = {
-    "cavity2d": dict(case=lambda: cases.config("cavity2d"), snaps=(1, 10, 100, 500, 1000, 2000), cache=True),
-    "channel": dict(case=lambda: cases.config("channel"), snaps=(1, 10, 100, 500, 1000, 2000), cache=True),
-    "cavity3d": dict(case=lambda: cases.config("cavity3d"), snaps=(1, 2), cache=False),
+    "cavity2d": dict(case=lambda: cases.config("cavity2d"), snaps=(1, 10, 100, 500, 1000, 2000), cache=True,
+                     floor_until=2000),
+    "channel": dict(case=lambda: cases.config("channel"), snaps=(1, 10, 100, 500, 1000, 2000), cache=True,
+                    floor_until=2000),
+    "cavity3d": dict(case=lambda: cases.config("cavity3d"), snaps=(1, 2), cache=False, floor_until=2),
 }
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+
+
+def floors(name: str, threads: int, variant: str):
+    plan = PLANS[name]
+    path = os.path.join(HERE, f"{name}.npz")
+    out = dict(np.load(path))
+    c = plan["case"]()
+    o = Oracle(c, threads=threads, cache_faces=plan["cache"], variant=variant)
+    o.init_equilibrium()
+    n = 0
+    for s in plan["snaps"]:
+        if s > plan["floor_until"]:
+            break
+        while n < s:
+            o.step(nsteps=1)
+            n += 1
+        m = o.macro()
+        for k, v in m.items():
+            out[f"floor_{variant}_{k}_{s}"] = np.array(rel_l2(v, out[f"{k}_{s}"]))
+        print(name, variant, "step", s, {k: float(out[f"floor_{variant}_{k}_{s}"]) for k in m}, flush=True)
+    np.savez_compressed(path, **out)
+    print("updated", path)
 
 
 def run(name: str, threads: int):
@@ -54,4 +94,10 @@ def run(name: str, threads: int):
 
 
 if __name__ == "__main__":
-    run(sys.argv[1], int(sys.argv[2]) if len(sys.argv) > 2 else os.cpu_count())
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    nthreads = int(args[1]) if len(args) > 1 else os.cpu_count()
+    if "--floors" in sys.argv:
+        for variant in ("ext", "rev"):
+            floors(args[0], nthreads, variant)
+    else:
+        run(args[0], nthreads)

@@ -216,6 +216,29 @@ def _golden(name):
     return np.load(p)
 
 
+FLOOR_FACTOR = 2.0
+
+
+def _golden_tol(g, k, s):
+    """1e-12, unless the fixture records that the oracle's own round-off for this field and
+    step is above 1e-12 / FLOOR_FACTOR: then FLOOR_FACTOR times that floor.  The floor is the
+    larger distance of the FP64 oracle from two other evaluations of the same arithmetic: one
+    with 80-bit intermediates and 80-bit velocity sums (near-exact sums), one with every
+    velocity sum in reverse order.  Where it is large the field is a sum that cancels almost
+    to zero (the velocity one step after rest, a slow transpiration flow, the heat flux of
+    cells along two walls) and the reference's sequential FP64 sums are themselves that far
+    from the exact sums; the engine's tree-ordered sums sit near the exact ones, so its
+    distance from the oracle is about one floor."""
+    fl = [float(g[key]) for key in (f"floor_ext_{k}_{s}", f"floor_rev_{k}_{s}") if key in g]
+    return max([FP64_TOL] + [FLOOR_FACTOR * x for x in fl])
+
+
+def _check_snap(g, s, m, failures):
+    errs = {k: rel_l2(m[k], g[f"{k}_{s}"]) for k in ("rho", "U", "T", "q")}
+    _say(f"  step {s}: " + " ".join(f"{k}={v:.2e} (tol {_golden_tol(g, k, s):.1e})" for k, v in errs.items()))
+    failures += [(int(s), k, v) for k, v in errs.items() if not v <= _golden_tol(g, k, s)]
+
+
 @pytest.mark.parametrize("name", ["cavity2d", "channel"])
 def test_golden_full_size(name):
     """Full configuration, N = 2000 steps, against the committed oracle fixture."""
@@ -226,31 +249,36 @@ def test_golden_full_size(name):
     e.init_equilibrium()
     n = 0
     mass = [float(np.sum(c.V * c.rho0))]
+    failures = []
     for s in g["snaps"]:
         while n < s:
             e.step(nsteps=1)
             n += 1
             mass.append(float(np.sum(c.V * e.macro()["rho"])))
-        m = e.macro()
-        errs = {k: rel_l2(m[k], g[f"{k}_{s}"]) for k in ("rho", "U", "T", "q")}
-        _say(f"  step {s}: " + " ".join(f"{k}={v:.2e}" for k, v in errs.items()))
-        for k, v in errs.items():
-            assert v <= FP64_TOL, (s, k, v)
+        _check_snap(g, s, e.macro(), failures)
+    assert not failures, failures
+    # the final state at N steps meets the bar without any allowance
+    last = int(g["snaps"][-1])
+    m = e.macro()
+    for k in ("rho", "U", "T", "q"):
+        assert rel_l2(m[k], g[f"{k}_{last}"]) <= FP64_TOL, k
     np.testing.assert_allclose(np.array(mass), g["mass"], rtol=1e-12)
 
 
 def test_golden_cavity3d():
     g = _golden("cavity3d")
     c = cases.config("cavity3d")
+    _say(f"[golden] cavity3d cells={c.n_cells} nv={c.nv}")
     e = Engine(c)
     e.init_equilibrium()
     n = 0
+    failures = []
     for s in g["snaps"]:
         e.step(nsteps=int(s) - n)
         n = int(s)
-        m = e.macro()
-        for k in ("rho", "U", "T", "q"):
-            assert rel_l2(m[k], g[f"{k}_{s}"]) <= FP64_TOL, (s, k)
+        _check_snap(g, s, e.macro(), failures)
+    e.close()
+    assert not failures, failures
 
 
 def test_cavity3d_full_size_invariants():
@@ -275,3 +303,21 @@ def test_cavity3d_full_size_invariants():
     un = np.einsum("ij,ij->i", f["U"][nfi:], nrm)
     assert np.abs(un).max() <= 1e-12 * max(1.0, np.abs(f["U"]).max())
     e.close()
+
+
+def test_step_graphs_match_eager():
+    """Captured two-step graphs (the default for multi-step calls) give bit-identical results
+    to eager launches, over several replays with host work and synchronisation between them."""
+    c = cases.cavity(10, 13)
+    a, b = Engine(c), Engine(c)
+    a.init_equilibrium()
+    b.init_equilibrium()
+    b.set_graphs(False)
+    for n in (4, 3, 6, 1, 2):
+        a.step(nsteps=n)
+        b.step(nsteps=n)
+        ma, mb = a.macro(), b.macro()
+        for k in ma:
+            assert np.array_equal(ma[k], mb[k]), (n, k)
+    a.close()
+    b.close()
