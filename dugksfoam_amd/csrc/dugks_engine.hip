@@ -65,10 +65,13 @@ constexpr int MAXWAVES = MAXNT / 64;
 constexpr int NIBMAX = 64;     // velocity iterations per thread in a block
 constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
 // Minimum resident waves per SIMD asked of the register allocator (measured on the 48^3 x 24^3
-// cavity: 6 for k_face costs a few spill bytes and gains 4 %; 5 for k_cell spills in its face
-// loop and loses 35 %).
+// cavity: 6 for the FP64 k_face costs a few spill bytes and gains 4 %, the FP32 one spills more
+// and is held at 5; 5 for k_cell spills in its face loop and loses 35 %).
 #ifndef FACE_WAVES
 #define FACE_WAVES 6
+#endif
+#ifndef FACE_WAVES_FP32
+#define FACE_WAVES_FP32 5
 #endif
 #ifndef CELL_WAVES
 #define CELL_WAVES 4
@@ -206,28 +209,36 @@ DEVI double ld(const TS *a, size_t i)
     return (double)a[i];
 }
 
-// FP32 storage keeps cell rows as the deviation from a fixed rest Maxwellian M_k (density and
-// temperature: volume means of the initial state): value = stored + M_k.  FP32 rounding then
-// acts on the small deviation only.  Least-squares gradients and wall rows are unaffected (a
-// constant per velocity), so they work on the stored deviations directly.  FP64 storage keeps
-// the values themselves (offset 0).
-template <typename TS>
-DEVI double row_off(const P<TS> &p, unsigned k, int which)
-{
-    if (sizeof(TS) == 8) return 0.0;
-    return p.moff[(size_t)which * p.nvp + k];
-}
-
 struct Vel {
     double x, y, z, w, xi2;
     int ix, iy, iz;
 };
 
-// Per-axis velocity table in LDS: vt[0, nax) = axis values, vt[nax, 2 nax) = axis weights.
+// FP32 storage keeps cell rows as the deviation from a fixed rest Maxwellian M (density and
+// temperature: volume means of the initial state): value = stored + M(xi).  FP32 rounding then
+// acts on the small deviation only.  Least-squares gradients and wall rows are unaffected (M is
+// a constant per velocity), so they work on the stored deviations directly.  M is the product
+// of per-axis factors staged in LDS (vt[2 nax, 3 nax)) times a prefactor for g / h
+// (vt[3 nax + which]).  FP64 storage keeps the values themselves (offset 0).
+template <int D, typename TS>
+DEVI double row_off(const P<TS> &p, const Vel &v, int which, const double *vt)
+{
+    if (sizeof(TS) == 8) return 0.0;
+    const double *e = vt + 2 * p.nax;
+    double m = e[v.ix] * e[p.nxi + v.iy];
+    if (D == 3) m *= e[2 * p.nxi + v.iz];
+    return m * vt[3 * p.nax + which];
+}
+
+// Per-axis velocity table in LDS: vt[0, nax) = axis values, vt[nax, 2 nax) = axis weights,
+// vt[2 nax, 3 nax + 2) = the FP32 row-offset factors (row_off).
+constexpr int VT_SIZE = 3 * NAXMAX + 2;
+
 template <typename TS>
 DEVI void stage_vt(const P<TS> &p, double *vt)
 {
-    for (int t = threadIdx.x; t < 2 * p.nax; t += blockDim.x) vt[t] = t < p.nax ? p.axis[t] : p.axw[t - p.nax];
+    for (int t = threadIdx.x; t < 3 * p.nax + 2; t += blockDim.x)
+        vt[t] = t < p.nax ? p.axis[t] : (t < 2 * p.nax ? p.axw[t - p.nax] : p.moff[t - 2 * p.nax]);
 }
 
 // Velocity k: w = (w_z w_y) w_x and |xi|^2 = (x x + y y) + z z in the order the velocity set is
@@ -534,12 +545,12 @@ DEVI bool xcd_block(int n, int nb, int &pos, int &b)
 // step's fixedGradient boundary gradient n . grad(gbar) (discreteVelocity.C:436-470).
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, int NS, typename TS>
-__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FACE_WAVES, 8))) void k_face(P<TS> p)
+__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 8 ? FACE_WAVES : FACE_WAVES_FP32, 8))) void k_face(P<TS> p)
 {
     constexpr int NM = NMom<D>::v + 1;  // + wall outGoing
     __shared__ double red[MAXWAVES * NM];
     __shared__ double tot[NM];
-    __shared__ double vt[2 * NAXMAX];
+    __shared__ double vt[VT_SIZE];
     int pos, b;
     if (!xcd_block(p.nf, p.nb, pos, b)) return;
     stage_vt(p, vt);
@@ -599,8 +610,8 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FACE_WAVE
             }
         }
         if (!wall || s > 0) {
-            const double gf = face_value<D>(kind, s, gP, GP, dP, gN, GN, dN, wcy, hx) + row_off(p, k, 0);
-            const double hf = H ? face_value<D>(kind, s, hP, HP, dP, hN, HN, dN, wcy, hx) + row_off(p, k, 1) : 0.0;
+            const double gf = face_value<D>(kind, s, gP, GP, dP, gN, GN, dN, wcy, hx) + row_off<D>(p, v, 0, vt);
+            const double hf = H ? face_value<D>(kind, s, hP, HP, dP, hN, HN, dN, wcy, hx) + row_off<D>(p, v, 1, vt) : 0.0;
             acc_moments<D, H>(acc, gf, hf, v, Uh);
             if (wall) acc[NM - 1] = fma(v.w * s, gf, acc[NM - 1]);
         }
@@ -699,7 +710,7 @@ __global__ __launch_bounds__(256) void k_face_fin(P<TS> p)
 // wall Maxwellian, left unrelaxed, discreteVelocity.C:693-731) and its flux applied.  g~ of this
 // step is recovered from the resident gbar as (gbar - r geq) / (1 - r); the new g~ goes to the
 // output buffer, where k_cell_s1 turns it into the next gbar.
-// Dynamic LDS: [velocity table (2 nax) | tabC | tabF x MAXS | tabW x MAXS], nax doubles each.
+// Dynamic LDS: [velocity table (3 nax + 2) | tabC | tabF x MAXS | tabW x MAXS], nax doubles each.
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, int NS, typename TS>
 __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVES, 8))) void k_cell(P<TS> p)
@@ -714,9 +725,9 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
     const int c = cptr(p.corder)[pos];
     const int nax = p.nax;
     double *vt = tabs;
-    double *tabC = tabs + 2 * nax;
-    double *tabF = tabs + 3 * nax;
-    double *tabW = tabs + (3 + MAXS) * nax;
+    double *tabC = tabs + 3 * nax + 2;
+    double *tabF = tabC + nax;
+    double *tabW = tabF + MAXS * nax;
     stage_vt(p, vt);
     const int cas = cptr(p.ccase)[c];
     const double dt = cptr(p.dtc)[cas];
@@ -762,7 +773,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
         const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
         const double gC = ld(rowC, k);                 // stored value (FP32: deviation)
         const double hC = H ? ld(hrowC, k) : 0.0;
-        const double og = row_off(p, k, 0), oh = H ? row_off(p, k, 1) : 0.0;
+        const double og = row_off<D>(p, v, 0, vt), oh = H ? row_off<D>(p, v, 1, vt) : 0.0;
         const double gCv = gC + og, hCv = hC + oh;      // gbar itself
         double GC[3] = {0, 0, 0}, HC[3] = {0, 0, 0};
         grad_at<D, H, NS>(p, c, k, gC, hC, GC, HC);
@@ -881,7 +892,7 @@ template <int D, bool H, typename TS>
 __global__ __launch_bounds__(MAXNT) void k_cell_s1(P<TS> p)
 {
     __shared__ double tabN[NAXMAX];
-    __shared__ double vt[2 * NAXMAX];
+    __shared__ double vt[VT_SIZE];
     const int c = blockIdx.x;
     stage_vt(p, vt);
     const int cas = cptr(p.ccase)[c];
@@ -920,7 +931,7 @@ __global__ __launch_bounds__(MAXNT) void k_cell_s1(P<TS> p)
             if (k < (unsigned)p.nvp) {
                 double geq, heq;
                 eq_eval<D, H>(en, tabN, p.nxi, vv[u], p.kc, p.K, geq, heq);
-                const double og = row_off(p, k, 0), oh = H ? row_off(p, k, 1) : 0.0;
+                const double og = row_off<D>(p, vv[u], 0, vt), oh = H ? row_off<D>(p, vv[u], 1, vt) : 0.0;
                 go[k] = (TS)((1.0 - rn) * (gv[u] + og) + rn * geq - og);
                 if (H) ho[k] = (TS)((1.0 - rn) * (hv[u] + oh) + rn * heq - oh);
             }
@@ -937,7 +948,7 @@ template <int D, bool H, typename TS>
 __global__ __launch_bounds__(MAXNT) void k_init_raw(P<TS> p)
 {
     __shared__ double tab[NAXMAX];
-    __shared__ double vt[2 * NAXMAX];
+    __shared__ double vt[VT_SIZE];
     stage_vt(p, vt);
     const int c = blockIdx.x;
     const double U[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
@@ -956,8 +967,8 @@ __global__ __launch_bounds__(MAXNT) void k_init_raw(P<TS> p)
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
-        p.gb_out[ck] = (TS)(geq - row_off(p, k, 0));
-        if (H) p.hb_out[ck] = (TS)(heq - row_off(p, k, 1));
+        p.gb_out[ck] = (TS)(geq - row_off<D>(p, v, 0, vt));
+        if (H) p.hb_out[ck] = (TS)(heq - row_off<D>(p, v, 1, vt));
     }
 }
 
@@ -967,7 +978,7 @@ template <int D, bool H, typename TS>
 __global__ __launch_bounds__(MAXNT) void k_convert(P<TS> p, const double *dt_old, const double *dt_new, int mode)
 {
     __shared__ double tab[NAXMAX];
-    __shared__ double vt[2 * NAXMAX];
+    __shared__ double vt[VT_SIZE];
     stage_vt(p, vt);
     const int c = blockIdx.x;
     const int cas = p.ccase[c];
@@ -992,7 +1003,7 @@ __global__ __launch_bounds__(MAXNT) void k_convert(P<TS> p, const double *dt_old
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
-        const double og = row_off(p, k, 0), oh = H ? row_off(p, k, 1) : 0.0;
+        const double og = row_off<D>(p, v, 0, vt), oh = H ? row_off<D>(p, v, 1, vt) : 0.0;
         double g = ld(p.gb_in, ck) + og, h = H ? ld(p.hb_in, ck) + oh : 0.0;
         if (mode != 0) {
             g = (g - ro * geq) * inv1mr;
@@ -1014,7 +1025,7 @@ __global__ __launch_bounds__(256) void k_wall_init(P<TS> p, const int *wall_face
 {
     constexpr int NM = NMom<D>::v;
     __shared__ double tab[NAXMAX];
-    __shared__ double vt[2 * NAXMAX];
+    __shared__ double vt[VT_SIZE];
     stage_vt(p, vt);
     __shared__ double red[MAXWAVES * (NM + 1)];
     __shared__ double tot[NM + 1];
@@ -1055,7 +1066,7 @@ __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, co
                                                   double *g_out, double *h_out)
 {
     __shared__ double tab[NAXMAX];
-    __shared__ double vt[2 * NAXMAX];
+    __shared__ double vt[VT_SIZE];
     stage_vt(p, vt);
     const int c = cells[blockIdx.x];
     const int cas = p.ccase[c];
@@ -1077,7 +1088,7 @@ __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, co
         double geq, heq;
         eq_eval<D, H>(e, tab, p.nxi, v, p.kc, p.K, geq, heq);
         const size_t ck = (size_t)c * p.nvp + k;
-        double g = ld(p.gb_in, ck) + row_off(p, k, 0), h = H ? ld(p.hb_in, ck) + row_off(p, k, 1) : 0.0;
+        double g = ld(p.gb_in, ck) + row_off<D>(p, v, 0, vt), h = H ? ld(p.hb_in, ck) + row_off<D>(p, v, 1, vt) : 0.0;
         if (!raw) {
             g = (g - ro * geq) * inv1mr;
             h = (h - ro * heq) * inv1mr;
@@ -1103,7 +1114,6 @@ struct dugks_engine {
     int ns_max = 0;      // largest face count of a cell
     double R = 0, omega = 0, Tref = 0, mu_ref = 0, Pr = 0, xi_max = 0;
     std::vector<int> perm;          // device velocity -> reference velocity, -1 for padding
-    std::vector<double> vxyz;       // device-order velocities [nvp][3] (padding: 0)
     double off_rho = 0.0, off_T = 0.0;  // FP32 storage: rest Maxwellian of the row offsets
     double *moff = nullptr;
     std::vector<double> dcoef_int;  // internal deltaCoeffs (Courant number)
@@ -1224,7 +1234,7 @@ P<TS> params(dugks_engine *h)
 // ---- launch dispatch over (D, H, TS, NIT) ----
 
 
-size_t cell_lds(const dugks_engine *h) { return sizeof(double) * (3 + 2 * MAXS) * (size_t)h->nax; }
+size_t cell_lds(const dugks_engine *h) { return sizeof(double) * ((4 + 2 * MAXS) * (size_t)h->nax + 2); }
 
 dim3 block_grid(int n, int nb) { return dim3((unsigned)(((n + NXCD - 1) / NXCD) * NXCD), (unsigned)nb); }
 
@@ -1717,12 +1727,10 @@ int build(dugks_engine *h, const dugks_case *cs)
     const int nvp = h->nvp;
     std::vector<uint32_t> vidx(nvp, VPAD);
     h->perm.assign(nvp, -1);
-    h->vxyz.assign(3 * (size_t)nvp, 0.0);
     for (int k = 0; k < nv; ++k) {
         const int r = order[k];
         vidx[k] = (uint32_t)rid[r];
         h->perm[k] = r;
-        h->vxyz[3 * k] = rx[r]; h->vxyz[3 * k + 1] = ry[r]; h->vxyz[3 * k + 2] = rz[r];
     }
     std::vector<double> axis(h->nax), axw(h->nax);
     for (int a = 0; a < D; ++a)
@@ -1785,7 +1793,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
     int r = 0;
-    if ((r = dalloc(h, &h->vidx, nvp)) || (r = dalloc(h, &h->axw, h->nax)) || (r = dalloc(h, &h->moff, 2 * (size_t)nvp)) ||
+    if ((r = dalloc(h, &h->vidx, nvp)) || (r = dalloc(h, &h->axw, h->nax)) || (r = dalloc(h, &h->moff, (size_t)h->nax + 2)) ||
         (r = dalloc(h, &h->axis, h->nax)) || (r = dalloc(h, &h->cs, nc + 1)) || (r = dalloc(h, &h->slots, nslot)) ||
         (r = dalloc(h, &h->cV, nc)) || (r = dalloc(h, &h->ccase, nc)) || (r = dalloc(h, &h->dtc, h->n_cases)) ||
         (r = dalloc(h, &h->dtp, h->n_cases)) || (r = dalloc(h, &h->muc, h->n_cases)) ||
@@ -1860,20 +1868,19 @@ void release(dugks_engine *h)
 }
 
 // Row offsets of the FP32 storage: the rest Maxwellian g = rho (2 pi R T)^(-D/2) exp(-|xi|^2 / 2RT)
-// and h = g R T (K + 3 - D) at (off_rho, off_T); zero for FP64 storage and padding velocities.
+// and h = g R T (K + 3 - D) at (off_rho, off_T), as per-axis factors exp(-x_i^2 / 2RT) [nax] and
+// the prefactors of g and h; all zero for FP64 storage.
 int upload_offsets(dugks_engine *h)
 {
-    std::vector<double> m(2 * (size_t)h->nvp, 0.0);
+    std::vector<double> m((size_t)h->nax + 2, 0.0);
     if (h->storage == DUGKS_STORAGE_FP32) {
         const double RT = h->R * h->off_T, tp = 2.0 * PI * RT;
         const double pre = h->off_rho / (h->D == 3 ? tp * std::sqrt(tp) : tp);
-        for (int k = 0; k < h->nvp; ++k) {
-            if (h->perm[k] < 0) continue;
-            const double *x = &h->vxyz[3 * (size_t)k];
-            const double g = pre * std::exp(-(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) / (2.0 * RT));
-            m[k] = g;
-            m[h->nvp + k] = h->has_h ? g * RT * (h->K + 3 - h->D) : 0.0;
-        }
+        std::vector<double> ax((size_t)h->nax);
+        HIPCHK(h, hipMemcpy(ax.data(), h->axis, sizeof(double) * h->nax, hipMemcpyDeviceToHost));
+        for (int t = 0; t < h->nax; ++t) m[t] = std::exp(-(ax[t] * ax[t]) / (2.0 * RT));
+        m[h->nax] = pre;
+        m[h->nax + 1] = h->has_h ? pre * RT * (h->K + 3 - h->D) : 0.0;
     }
     return upload(h, h->moff, m.data(), m.size());
 }
