@@ -26,7 +26,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_model(case, storage_bytes: int, n_blocks: int):
+def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = None):
     """Algorithmic bytes per launch of each step kernel: every array the kernel must
     touch, read once and written once (the per-kernel table of DESIGN.md).
 
@@ -44,7 +44,7 @@ def bytes_model(case, storage_bytes: int, n_blocks: int):
     nm_f = (case.dim + 2 + case.dim + case.dim * (case.dim + 1) // 2) + 1
     nm_c = nm_f - 1
     fpart = nf * n_blocks * nm_f * 8
-    cpart = nc * n_blocks * nm_c * 8
+    cpart = nc * (n_blocks if n_blocks_cell is None else n_blocks_cell) * nm_c * 8
     kb = {
         "k_face": nd * s * (nc * nv + 2 * nw * nv) + fpart,
         "k_face_fin": fpart + nf * 8 * (3 + 8 + 16),
@@ -79,14 +79,15 @@ def run_gpu(case, storage: str, steps: int, warmup: int):
     e.step(nsteps=steps)          # returns after the engine stream has drained
     t1 = time.perf_counter()
     kms, n = e.profile()
-    n_blocks = e.blocks()[2]
+    n_blocks, n_blocks_cell = e.blocks()[2], e.cell_blocks()[2]
+    shapes = {"k_face": e.blocks()[:3], "k_cell": e.cell_blocks()}
     m = e.macro()
     ok = bool(np.all(np.isfinite(m["rho"])) and np.all(m["rho"] > 0))
     e.close()
     wall = t1 - t0
     upd = case.n_cells * case.nv * steps
     sb = 8 if storage == "fp64" else 4
-    kb, step_min = bytes_model(case, sb, n_blocks)
+    kb, step_min = bytes_model(case, sb, n_blocks, n_blocks_cell)
     kern = {k: (kms[k] / n, kb[k]) for k in kb}
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms, bytes_ = kern[dom]
@@ -100,6 +101,7 @@ def run_gpu(case, storage: str, steps: int, warmup: int):
             "step_model": {"bytes_per_step": step_min, "bytes_per_update": step_min / (case.n_cells * case.nv),
                            "GBs": round(step_min / (step_ms * 1e-3) / 1e9, 1),
                            "frac": round(step_min / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+    roof["launch_shapes"] = {k: {"threads": v[0], "iters": v[1], "velocity_blocks": v[2]} for k, v in shapes.items()}
     return dict(value=upd / wall / 1e9, ms_per_step=wall / steps * 1e3, kernel_ms=step_ms,
                 roofline=roof, finite=ok)
 

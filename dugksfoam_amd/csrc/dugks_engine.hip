@@ -152,7 +152,8 @@ static_assert(sizeof(FaceX) == 96, "facex size");
 template <typename TS>
 struct P {
     int D, nc, nf, nvp, nxi, nax, K, nt, niter, nw;
-    int nib, nb, vb;    // velocity blocks: nb blocks of vb = nt * nib velocities (niter = nb * nib)
+    int nib, nb, vb;    // k_face velocity blocks: nb blocks of vb = nt * nib velocities
+    int ntc, nibc, nbc, vbc;  // k_cell velocity blocks: nbc blocks of vbc = ntc * nibc velocities
     double R, Pr, omega, Tref, kc;  // kc = K + 3 - D
     const uint32_t *__restrict__ vidx;  // packed axis indices of each (sign-sorted) velocity
     const double *__restrict__ axw;     // per-axis quadrature weights, same layout as axis
@@ -721,7 +722,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
     __shared__ double tot[NC];
 
     int pos, b;
-    if (!xcd_block(p.nc, p.nb, pos, b)) return;
+    if (!xcd_block(p.nc, p.nbc, pos, b)) return;
     const int c = cptr(p.corder)[pos];
     const int nax = p.nax;
     double *vt = tabs;
@@ -766,9 +767,9 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
     double acc[NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) acc[m] = 0.0;
-    const unsigned k0 = (unsigned)b * p.vb + threadIdx.x;
-    for (int it = 0; it < p.nib; ++it) {
-        const unsigned k = k0 + it * p.nt;
+    const unsigned k0 = (unsigned)b * p.vbc + threadIdx.x;
+    for (int it = 0; it < p.nibc; ++it) {
+        const unsigned k = k0 + it * p.ntc;
         const Vel v = load_vel<D>(p, k, vt);
         const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
         const double gC = ld(rowC, k);                 // stored value (FP32: deviation)
@@ -848,7 +849,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
         acc_moments<D, H>(acc, ag, ah, v, Uc);
     }
     block_sum<NC>(acc, red, tot);
-    if ((int)threadIdx.x < NC) p.cpart[((size_t)c * p.nb + b) * NC + threadIdx.x] = tot[threadIdx.x];
+    if ((int)threadIdx.x < NC) p.cpart[((size_t)c * p.nbc + b) * NC + threadIdx.x] = tot[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -865,8 +866,8 @@ __global__ __launch_bounds__(256) void k_cell_fin(P<TS> p)
     const double dt = p.dtc[cas];
     double a[NC];
     for (int m = 0; m < NC; ++m) a[m] = 0.0;
-    const double *cp = p.cpart + (size_t)c * p.nb * NC;
-    for (int b = 0; b < p.nb; ++b)
+    const double *cp = p.cpart + (size_t)c * p.nbc * NC;
+    for (int b = 0; b < p.nbc; ++b)
         for (int m = 0; m < NC; ++m) a[m] += cp[b * NC + m];
     const double Uc[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
     double rho, U[3], T, tau, q[3];
@@ -1110,7 +1111,14 @@ struct dugks_engine {
     int D = 2, nc = 0, nfi = 0, nf = 0, nb = 0, nw = 0, nv = 0, nvp = 0, nxi = 0, nax = 0, K = 0;
     int storage = DUGKS_STORAGE_FP64, n_cases = 1;
     bool has_h = true;
-    int nt = 64, niter = 1, nib = 1, nvb = 1, vb = 64;  // velocity blocks: nvb x (vb = nt * nib)
+    int nt = 64, niter = 1, nib = 1, nvb = 1, vb = 64;  // k_face velocity blocks: nvb x (vb = nt * nib)
+    int ntc = 64, nibc = 1, nvbc = 1;                  // k_cell velocity blocks
+    int nb_cap = 1, nbc_cap = 1;                        // partial-sum buffers sized for these
+    // online launch-shape tuning: candidate (nt, nib) per kernel, all tiling the same nvp
+    std::vector<std::pair<int, int>> face_cand, cell_cand;
+    int tuning = 0, tune_i = -1;
+    double tune_best_face = 0.0, tune_best_cell = 0.0;
+    int tune_face = 0, tune_cell = 0;
     int ns_max = 0;      // largest face count of a cell
     double R = 0, omega = 0, Tref = 0, mu_ref = 0, Pr = 0, xi_max = 0;
     std::vector<int> perm;          // device velocity -> reference velocity, -1 for padding
@@ -1150,6 +1158,7 @@ struct dugks_engine {
     // buffer pointers of that side; dt is read from device memory, so no re-capture on change)
     int use_graphs = 1;
     hipGraphExec_t g2[2] = {nullptr, nullptr};
+    hipEvent_t tev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double last_ms = 0.0;
     int profiling = 0;
     std::vector<hipEvent_t> pev;     // NKERN + 1 events per profiled step
@@ -1214,6 +1223,7 @@ P<TS> params(dugks_engine *h)
     p.D = h->D; p.nc = h->nc; p.nf = h->nf; p.nvp = h->nvp; p.nxi = h->nxi; p.nax = h->nax; p.K = h->K;
     p.nt = h->nt; p.niter = h->niter; p.nw = h->nw;
     p.nib = h->nib; p.nb = h->nvb; p.vb = h->vb;
+    p.ntc = h->ntc; p.nibc = h->nibc; p.nbc = h->nvbc; p.vbc = h->ntc * h->nibc;
     p.R = h->R; p.Pr = h->Pr; p.omega = h->omega; p.Tref = h->Tref; p.kc = (double)(h->K + 3 - h->D);
     p.vidx = h->vidx; p.axw = h->axw; p.moff = h->moff;
     p.axis = h->axis; p.cs = h->cs; p.slots = h->slots; p.cV = h->cV; p.ccase = h->ccase;
@@ -1250,7 +1260,7 @@ void launch_step_ns(dugks_engine *h, hipEvent_t *ev)
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     hipLaunchKernelGGL((k_face_fin<D, TS>), dim3((h->nf + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
-    hipLaunchKernelGGL((k_cell<D, H, NS, TS>), block_grid(h->nc, h->nvb), dim3(h->nt), cell_lds(h), h->stream, p);
+    hipLaunchKernelGGL((k_cell<D, H, NS, TS>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
     if (ev) (void)hipEventRecord(ev[3], h->stream);
     hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[4], h->stream);
@@ -1399,6 +1409,53 @@ std::vector<int> morton_order(int n, const double *X, const int *group)
         return key[a] < key[b];
     });
     return ord;
+}
+
+// Launch shapes that tile the padded velocity count: (nt, nib) with nt a multiple of 64 up to
+// MAXNT, nib <= NIBMAX, nt * nib dividing nvp into at most 16 blocks.  The cell kernel's
+// default is one workgroup per cell (the largest nt that tiles nvp in NIBMAX iterations or
+// fewer), measured faster than the face kernel's blocking on the 48^3 x 24^3 cavity; the face
+// kernel keeps choose_blocks' shape.  Partial-sum buffers are sized for the largest block count.
+void set_face_shape(dugks_engine *h, int nt, int nib)
+{
+    h->nt = nt;
+    h->nib = nib;
+    h->vb = nt * nib;
+    h->nvb = h->nvp / h->vb;
+    h->niter = h->nvp / nt;
+}
+
+void set_cell_shape(dugks_engine *h, int nt, int nib)
+{
+    h->ntc = nt;
+    h->nibc = nib;
+    h->nvbc = h->nvp / (nt * nib);
+}
+
+void shape_candidates(dugks_engine *h)
+{
+    h->face_cand.clear();
+    for (int nb = 1; nb <= 16; ++nb) {
+        if (h->nvp % nb) continue;
+        const int vb = h->nvp / nb;
+        for (int t = 128; t <= MAXNT; t += 128) {
+            if (vb % t || vb / t > NIBMAX) continue;
+            h->face_cand.emplace_back(t, vb / t);
+        }
+    }
+    h->face_cand.emplace_back(h->nt, h->nib);   // choose_blocks' shape, always a candidate
+    std::sort(h->face_cand.begin(), h->face_cand.end());
+    h->face_cand.erase(std::unique(h->face_cand.begin(), h->face_cand.end()), h->face_cand.end());
+    h->cell_cand = h->face_cand;
+    int ct = 0;
+    for (int t = MAXNT; t >= 64 && !ct; t -= 64)
+        if (h->nvp % t == 0 && h->nvp / t <= NIBMAX) ct = t;
+    if (ct) set_cell_shape(h, ct, h->nvp / ct);
+    else set_cell_shape(h, h->nt, h->nib);
+    h->nb_cap = h->nvb;
+    h->nbc_cap = h->nvbc;
+    for (auto &c : h->face_cand) h->nb_cap = std::max(h->nb_cap, h->nvp / (c.first * c.second));
+    h->nbc_cap = h->nb_cap = std::max(h->nb_cap, h->nbc_cap);
 }
 
 double mag3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
@@ -1696,10 +1753,11 @@ int build(dugks_engine *h, const dugks_case *cs)
     choose_blocks(nv, h->nt, h->nib, h->nvb, h->nvp);
     h->vb = h->nt * h->nib;
     h->niter = h->nvb * h->nib;
+    shape_candidates(h);
     if (h->nt <= 0 || h->nt > MAXNT || h->nvp < nv)
         return fail(h, DUGKS_ERR_UNSUPPORTED, "velocity blocking failed");
     // one dispatch holds at most 2^32 - 1 work-items per grid dimension
-    if ((double)((std::max(nc, nf) + NXCD - 1) / NXCD * NXCD) * h->nt >= 4294967295.0)
+    if ((double)((std::max(nc, nf) + NXCD - 1) / NXCD * NXCD) * MAXNT >= 4294967295.0)
         return fail(h, DUGKS_ERR_UNSUPPORTED, "mesh too large for one dispatch per kernel");
     // ---- least-squares stencils (entry j = slot j; rows: cells, then walls) ----
     std::vector<CellSt> cst(nc);
@@ -1804,8 +1862,8 @@ int build(dugks_engine *h, const dugks_case *cs)
         (r = dalloc(h, &h->wall, nw)) || (r = dalloc(h, &h->wexp, (size_t)nw * h->nax)) ||
         (r = dalloc(h, &h->qwall, 3 * (size_t)nw)) || (r = dalloc(h, &h->swall, 9 * (size_t)nw)) ||
         (r = dalloc(h, &h->fx, nf)) || (r = dalloc(h, &h->cst, nc)) || (r = dalloc(h, &h->corder, nc)) ||
-        (r = dalloc(h, &h->forder, nf)) || (r = dalloc(h, &h->fpart, (size_t)nf * h->nvb * (NMom<3>::v + 1))) ||
-        (r = dalloc(h, &h->cpart, (size_t)nc * h->nvb * NMom<3>::v)) || (r = dalloc(h, &h->flag, 1)) || (r = dalloc(h, &h->wfslot, nw)))
+        (r = dalloc(h, &h->forder, nf)) || (r = dalloc(h, &h->fpart, (size_t)nf * h->nb_cap * (NMom<3>::v + 1))) ||
+        (r = dalloc(h, &h->cpart, (size_t)nc * h->nbc_cap * NMom<3>::v)) || (r = dalloc(h, &h->flag, 1)) || (r = dalloc(h, &h->wfslot, nw)))
         return r;
     const size_t es = ts_size(h);
     // one allocation per ping-pong side: nc cell rows, then nw wall rows (boundary gradient)
@@ -1831,6 +1889,14 @@ int build(dugks_engine *h, const dugks_case *cs)
     return 0;
 }
 
+void drop_graphs(dugks_engine *h)
+{
+    for (hipGraphExec_t &g : h->g2) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
 template <typename T>
 void dfree(T *&p)
 {
@@ -1853,9 +1919,10 @@ void release(dugks_engine *h)
         dfree(h->gb[i]); dfree(h->hb[i]);
         h->gB[i] = h->hB[i] = nullptr;
     }
-    for (hipGraphExec_t &g : h->g2) {
-        if (g) (void)hipGraphExecDestroy(g);
-        g = nullptr;
+    drop_graphs(h);
+    for (hipEvent_t &e : h->tev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
     }
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -1933,7 +2000,44 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
         }
     }
     int s0 = 0;
-    if (!h->profiling && h->use_graphs && nsteps >= 2) {
+    // online launch-shape tuning: each step runs the next candidate shape of k_face and of
+    // k_cell (after one warm-up step), timed with events; the fastest of each is kept
+    while (h->tuning && s0 < nsteps) {
+        const int nfc = (int)h->face_cand.size(), ncc = (int)h->cell_cand.size();
+        const int i = h->tune_i;
+        if (i >= 0) {
+            const auto &fc = h->face_cand[std::min(i, nfc - 1)];
+            const auto &cc = h->cell_cand[std::min(i, ncc - 1)];
+            set_face_shape(h, fc.first, fc.second);
+            set_cell_shape(h, cc.first, cc.second);
+        }
+        for (hipEvent_t &e : h->tev)
+            if (!e) HIPCHK(h, hipEventCreate(&e));
+        StepOp op{h, h->tev};
+        dispatch(h, op);
+        h->cur = 1 - h->cur;
+        ++s0;
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipEventSynchronize(h->tev[NKERN]));
+        float tf = 0.f, tc = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&tf, h->tev[0], h->tev[1]));
+        HIPCHK(h, hipEventElapsedTime(&tc, h->tev[2], h->tev[3]));
+        if (i >= 0 && i < nfc && (h->tune_best_face == 0.0 || tf < h->tune_best_face)) {
+            h->tune_best_face = tf;
+            h->tune_face = i;
+        }
+        if (i >= 0 && i < ncc && (h->tune_best_cell == 0.0 || tc < h->tune_best_cell)) {
+            h->tune_best_cell = tc;
+            h->tune_cell = i;
+        }
+        if (++h->tune_i >= std::max(nfc, ncc)) {
+            set_face_shape(h, h->face_cand[h->tune_face].first, h->face_cand[h->tune_face].second);
+            set_cell_shape(h, h->cell_cand[h->tune_cell].first, h->cell_cand[h->tune_cell].second);
+            h->tuning = 0;
+            drop_graphs(h);
+        }
+    }
+    if (!h->profiling && h->use_graphs && nsteps - s0 >= 2) {
         // replay a captured pair of steps; an odd last step runs eagerly below
         hipGraphExec_t &ex = h->g2[h->cur];
         if (!ex) {
@@ -2322,6 +2426,25 @@ int dugks_last_step_ms(dugks_engine *h, double *ms)
 {
     if (!h || !ms) return DUGKS_ERR_INVALID;
     *ms = h->last_ms;
+    return 0;
+}
+
+int dugks_set_autotune(dugks_engine *h, int32_t on)
+{
+    if (!h) return DUGKS_ERR_INVALID;
+    h->tuning = on ? 1 : 0;
+    h->tune_i = -1;
+    h->tune_best_face = h->tune_best_cell = 0.0;
+    h->tune_face = h->tune_cell = 0;
+    return 0;
+}
+
+int dugks_get_cell_launch(dugks_engine *h, int32_t *nt, int32_t *nib, int32_t *n_blocks)
+{
+    if (!h) return DUGKS_ERR_INVALID;
+    if (nt) *nt = h->ntc;
+    if (nib) *nib = h->nibc;
+    if (n_blocks) *n_blocks = h->nvbc;
     return 0;
 }
 

@@ -21,7 +21,7 @@ SYMBOLS = (
     "dugks_get_face_macro", "dugks_get_wall", "dugks_get_courant", "dugks_get_df", "dugks_get_sizes",
     "dugks_state_size", "dugks_save_state", "dugks_load_state", "dugks_last_step_ms", "dugks_destroy",
     "dugks_last_error", "dugks_abi_version", "dugks_set_profiling", "dugks_get_profile",
-    "dugks_get_launch", "dugks_set_graphs",
+    "dugks_get_launch", "dugks_set_graphs", "dugks_set_autotune", "dugks_get_cell_launch",
 )
 
 _lib = None
@@ -65,6 +65,8 @@ def load_library(path: str = LIB_PATH):
     L.dugks_get_profile.argtypes = [vp, dp, C.POINTER(C.c_int64)]
     L.dugks_get_launch.argtypes = [vp, ip, ip, ip, ip]
     L.dugks_set_graphs.argtypes = [vp, C.c_int32]
+    L.dugks_set_autotune.argtypes = [vp, C.c_int32]
+    L.dugks_get_cell_launch.argtypes = [vp, ip, ip, ip]
     _lib = L
     return L
 
@@ -191,10 +193,20 @@ class Engine:
         self._chk(load_library().dugks_set_graphs(self._h, 1 if on else 0))
 
     def blocks(self):
-        """(workgroup size, velocity iterations per thread, velocity blocks, padded Nv)."""
+        """Face kernel: (workgroup size, velocity iterations per thread, velocity blocks, padded Nv)."""
         v = [C.c_int32() for _ in range(4)]
         self._chk(load_library().dugks_get_launch(self._h, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+    def cell_blocks(self):
+        """Cell kernel: (workgroup size, velocity iterations per thread, velocity blocks)."""
+        v = [C.c_int32() for _ in range(3)]
+        self._chk(load_library().dugks_get_cell_launch(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune(self, on: bool = True):
+        """Time the candidate work decompositions on the next steps and keep the fastest."""
+        self._chk(load_library().dugks_set_autotune(self._h, 1 if on else 0))
 
     @property
     def last_step_ms(self) -> float:

@@ -148,7 +148,15 @@ int dugks_get_sizes(dugks_engine* h, int32_t* nv, int32_t* n_cells, int32_t* n_f
  * kernel; results are identical either way.  Profiling runs eagerly. */
 int dugks_set_graphs(dugks_engine* h, int32_t on);
 
-/* Work decomposition chosen at create (no reference counterpart; for measurement and
+/* Online launch-shape tuning (no reference counterpart): with autotune on, the next steps
+ * run one candidate work decomposition of the face kernel and of the cell kernel each (after
+ * one warm-up step), timed with HIP events, then keep the fastest of each.  The steps are
+ * real steps of the simulation; the candidates differ only in how velocity sums are split,
+ * i.e. in round-off.  dugks_get_launch / dugks_get_cell_launch report the shapes in use. */
+int dugks_set_autotune(dugks_engine* h, int32_t on);
+int dugks_get_cell_launch(dugks_engine* h, int32_t* nt, int32_t* nib, int32_t* n_blocks);
+
+/* Work decomposition of the face kernel (no reference counterpart; for measurement and
  * tuning): workgroup size, velocity iterations per thread in a block, number of
  * velocity blocks and the padded velocity count nt * nib * n_blocks.  NULL skips. */
 int dugks_get_launch(dugks_engine* h, int32_t* nt, int32_t* nib, int32_t* n_blocks, int32_t* nv_padded);

@@ -321,3 +321,27 @@ def test_step_graphs_match_eager():
             assert np.array_equal(ma[k], mb[k]), (n, k)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("which", ["cavity3d_6x12", "cavity2d_12x21", "channel_20x8"])
+def test_autotune_shapes_keep_parity(which):
+    """The online tuner runs every candidate work decomposition of both kernels on real
+    steps; the result still matches the oracle to the FP64 bar, and the shapes it keeps
+    tile the padded velocity count."""
+    mk, _ = _small_cases()[which]
+    c = mk()
+    e = Engine(c)
+    e.init_equilibrium()
+    e.autotune(True)
+    n = 24
+    e.step(nsteps=n)
+    o = _oracle(c)
+    o.step(nsteps=n)
+    mo, me = o.macro(), e.macro()
+    for k in mo:
+        assert rel_l2(me[k], mo[k]) <= FP64_TOL, (which, k, rel_l2(me[k], mo[k]))
+    nt, nib, nb, nvp = e.blocks()
+    ntc, nibc, nbc = e.cell_blocks()
+    assert nt * nib * nb == nvp and ntc * nibc * nbc == nvp
+    e.close()
+    o.close()
