@@ -1,0 +1,239 @@
+"""Writes structured box cases as dugksFoam case directories (OpenFOAM ASCII), the way
+blockMesh numbers a single hex block: points i + (nx+1)(j + (ny+1)k), cells i + nx(j + ny k),
+internal faces by owner then neighbour, boundary faces side by side in cell order.
+The cell, face and patch order equals that of dugksfoam_amd.cases.box_mesh, so a case read
+back by the front-end can be compared with the generator element by element.
+"""
+import os
+
+import numpy as np
+
+HEADER = "FoamFile\n{{\n    version     2.0;\n    format      ascii;\n    class       {cls};\n    object      {obj};\n}}\n\n"
+
+
+def _quad(axis, i, j, k):
+    """Quad of the plane x_axis = const through point (i,j,k), oriented along +axis."""
+    if axis == 0:
+        return [(i, j, k), (i, j + 1, k), (i, j + 1, k + 1), (i, j, k + 1)]
+    if axis == 1:
+        return [(i, j, k), (i, j, k + 1), (i + 1, j, k + 1), (i + 1, j, k)]
+    return [(i, j, k), (i + 1, j, k), (i + 1, j + 1, k), (i, j + 1, k)]
+
+
+def _write(path, cls, obj, body):
+    with open(path, "w") as f:
+        f.write("/* written by tests/foam_box.py */\n" + HEADER.format(cls=cls, obj=obj) + body)
+
+
+def write_box(path, nx, ny, nz=0, L=(1.0, 1.0, 1.0), dz2d=0.01, patches=None, fields=None,
+              dvm=None, xi=None, w=None, control=None):
+    """patches: list of (name, polyMesh type, [sides], extra) with sides among x0 x1 y0 y1 z0 z1;
+    fields: {'rho'|'U'|'T': (internal spec, {patch: boundary entry text})};
+    dvm: DVMProperties body; control: controlDict body."""
+    dim3 = nz > 0
+    nzc = nz if dim3 else 1
+    Lz = L[2] if dim3 else dz2d
+    dx = np.array([L[0] / nx, L[1] / ny, Lz / nzc])
+    npx, npy = nx + 1, ny + 1
+    pid = lambda i, j, k: i + npx * (j + npy * k)
+    I, J, K = np.meshgrid(np.arange(npx), np.arange(npy), np.arange(nzc + 1), indexing="ij")
+    pts = np.zeros((npx * npy * (nzc + 1), 3))
+    idx = pid(I.ravel(), J.ravel(), K.ravel())
+    pts[idx] = np.stack([I.ravel() * dx[0], J.ravel() * dx[1], K.ravel() * dx[2]], axis=1)
+    cid = lambda i, j, k: i + nx * (j + ny * k)
+    faces, owner, neighbour = [], [], []
+    for k in range(nzc):
+        for j in range(ny):
+            for i in range(nx):
+                c = cid(i, j, k)
+                if i < nx - 1:
+                    faces.append(_quad(0, i + 1, j, k)); owner.append(c); neighbour.append(c + 1)
+                if j < ny - 1:
+                    faces.append(_quad(1, i, j + 1, k)); owner.append(c); neighbour.append(c + nx)
+                if dim3 and k < nzc - 1:
+                    faces.append(_quad(2, i, j, k + 1)); owner.append(c); neighbour.append(c + nx * ny)
+
+    def side(name):
+        axis, hi = "xyz".index(name[0]), name[1] == "1"
+        out = []
+        for k in range(nzc):
+            for j in range(ny):
+                for i in range(nx):
+                    ijk = [i, j, k]
+                    n = [nx, ny, nzc][axis]
+                    if ijk[axis] != (n - 1 if hi else 0):
+                        continue
+                    corner = list(ijk)
+                    if hi:
+                        corner[axis] += 1
+                    q = _quad(axis, *corner)
+                    out.append((q if hi else q[::-1], cid(i, j, k)))
+        return out
+
+    bnd = []
+    for name, ptype, sides, extra in patches:
+        start = len(faces)
+        for s in sides:
+            for q, c in side(s):
+                faces.append(q); owner.append(c)
+        bnd.append((name, ptype, len(faces) - start, start, extra))
+
+    pm = os.path.join(path, "constant", "polyMesh")
+    os.makedirs(pm, exist_ok=True)
+    _write(os.path.join(pm, "points"), "vectorField", "points",
+           f"{len(pts)}\n(\n" + "\n".join(f"({float(p[0])!r} {float(p[1])!r} {float(p[2])!r})" for p in pts) + "\n)\n")
+    _write(os.path.join(pm, "faces"), "faceList", "faces",
+           f"{len(faces)}\n(\n" + "\n".join("4(" + " ".join(str(pid(*v)) for v in q) + ")" for q in faces) + "\n)\n")
+    _write(os.path.join(pm, "owner"), "labelList", "owner",
+           f"{len(owner)}\n(\n" + "\n".join(map(str, owner)) + "\n)\n")
+    _write(os.path.join(pm, "neighbour"), "labelList", "neighbour",
+           f"{len(neighbour)}\n(\n" + "\n".join(map(str, neighbour)) + "\n)\n")
+    body = f"{len(bnd)}\n(\n"
+    for name, ptype, n, start, extra in bnd:
+        body += f"    {name}\n    {{\n        type            {ptype};\n{extra}        nFaces          {n};\n        startFace       {start};\n    }}\n"
+    _write(os.path.join(pm, "boundary"), "polyBoundaryMesh", "boundary", body + ")\n")
+
+    os.makedirs(os.path.join(path, "0"), exist_ok=True)
+    for name, (internal, bf) in fields.items():
+        cls = "volVectorField" if name == "U" else "volScalarField"
+        body = f"dimensions      [0 0 0 0 0 0 0];\n\ninternalField   {internal};\n\nboundaryField\n{{\n"
+        for pname, entry in bf.items():
+            body += f"    {pname}\n    {{\n{entry}    }}\n"
+        _write(os.path.join(path, "0", name), cls, name, body + "}\n")
+    _write(os.path.join(path, "constant", "DVMProperties"), "dictionary", "DVMProperties", dvm)
+    for fname, arr in (("Xis", xi), ("weights", w)):
+        _write(os.path.join(path, "constant", fname), "scalarList", fname,
+               f"{len(arr)}\n(\n" + "\n".join(repr(float(v)) for v in arr) + "\n)\n")
+    os.makedirs(os.path.join(path, "system"), exist_ok=True)
+    _write(os.path.join(path, "system", "controlDict"), "dictionary", "controlDict", control)
+    return path
+
+
+def dvm_props(case):
+    return (f"fvDVMparas\n{{\n    xiMax   {float(case.xi_max)!r};\n    xiMin   {float(-case.xi_max)!r};\n"
+            f"    nDV     {case.xi.size};\n    macroFlux no;\n}}\n\n"
+            f"gasProperties\n{{\n    R       {float(case.R)!r};\n    omega   {float(case.omega)!r};\n"
+            f"    Tref    {float(case.T_ref)!r};\n    muRef   {float(case.mu_ref)!r};\n    Pr      {float(case.Pr)!r};\n"
+            f"    KInner  {case.K};\n}}\n")
+
+
+def control(deltaT, adjust=False, maxCo=0.5, endTime=1.0, writeInterval=1000, extra=""):
+    return (f"application dugksFoam;\nstartTime 0;\nendTime {float(endTime)!r};\ndeltaT {float(deltaT)!r};\n"
+            f"writeControl timeStep;\nwriteInterval {writeInterval};\ntimePrecision 9;\n"
+            f"adjustTimeStep {'yes' if adjust else 'no'};\nmaxCo {float(maxCo)!r};\n{extra}")
+
+
+def wall(U, T):
+    u = " ".join(repr(float(x)) for x in U)
+    return dict(rho="        type calculatedMaxwell;\n        value uniform 1.0;\n",
+                U=f"        type fixedValue;\n        value uniform ({u});\n",
+                T=f"        type fixedValue;\n        value uniform {float(T)!r};\n")
+
+
+def cavity_case(path, case, nz=0, dz2d=0.01, adjust=False, maxCo=0.5, **ctl):
+    """The lid-driven cavity of cases.cavity(n) as a case directory."""
+    n = case.meta["n"]
+    lid, fixed = wall(case.patches[0]["U"][0], case.T0[0]), wall((0, 0, 0), case.T0[0])
+    dim3 = nz > 0
+    side_fixed = ["x0", "x1", "y0"] + (["z0", "z1"] if dim3 else [])
+    patches = [("movingWall", "wall", ["y1"], ""), ("fixedWalls", "wall", side_fixed, "")]
+    if not dim3:
+        patches.append(("frontAndBack", "empty", ["z0", "z1"], ""))
+    fields = {}
+    for k in ("rho", "U", "T"):
+        bf = {"movingWall": lid[k], "fixedWalls": fixed[k]}
+        if not dim3:
+            bf["frontAndBack"] = "        type empty;\n"
+        internal = {"rho": f"uniform {float(case.rho0[0])!r}", "U": "uniform (0 0 0)", "T": f"uniform {float(case.T0[0])!r}"}[k]
+        fields[k] = (internal, bf)
+    return write_box(path, n, n, nz, dz2d=dz2d, patches=patches, fields=fields, dvm=dvm_props(case),
+                     xi=case.xi, w=case.xi_weights,
+                     control=control(case.dt, adjust=adjust, maxCo=maxCo, **ctl))
+
+
+def channel_case(path, case, nonuniform_rho=None):
+    """The periodic channel of cases.channel as a case directory (cyclic left/right)."""
+    nx, ny, Lx = case.meta["nx"], case.meta["ny"], case.meta["Lx"]
+    lower = case.patches[0]
+    Tlow = "nonuniform List<scalar> %d\n(\n%s\n)" % (lower["size"], "\n".join(repr(float(t)) for t in lower["T"]))
+    patches = [("lowerWall", "wall", ["y0"], ""), ("upperWall", "wall", ["y1"], ""),
+               ("left", "cyclic", ["x0"], "        neighbourPatch  right;\n"),
+               ("right", "cyclic", ["x1"], "        neighbourPatch  left;\n"),
+               ("frontAndBack", "empty", ["z0", "z1"], "")]
+    T0 = float(case.T0[0])
+    rho_int = f"uniform {float(case.rho0[0])!r}"
+    if nonuniform_rho is not None:
+        rho_int = "nonuniform List<scalar> %d\n(\n%s\n)" % (len(nonuniform_rho), "\n".join(repr(float(v)) for v in nonuniform_rho))
+    cyc = "        type cyclic;\n"
+    fields = dict(
+        rho=(rho_int, {"lowerWall": "        type calculatedMaxwell;\n        value uniform 1;\n",
+                       "upperWall": "        type calculatedMaxwell;\n        value uniform 1;\n",
+                       "left": cyc, "right": cyc, "frontAndBack": "        type empty;\n"}),
+        U=("uniform (0 0 0)", {"lowerWall": "        type fixedValue;\n        value uniform (0 0 0);\n",
+                               "upperWall": "        type fixedValue;\n        value uniform (0 0 0);\n",
+                               "left": cyc, "right": cyc, "frontAndBack": "        type empty;\n"}),
+        T=(f"uniform {T0!r}", {"lowerWall": f"        type fixedValue;\n        value {Tlow};\n",
+                               "upperWall": f"        type fixedValue;\n        value uniform {T0!r};\n",
+                               "left": cyc, "right": cyc, "frontAndBack": "        type empty;\n"}))
+    return write_box(path, nx, ny, 0, L=(Lx, 1.0, 1.0), patches=patches, fields=fields,
+                     dvm=dvm_props(case), xi=case.xi, w=case.xi_weights, control=control(case.dt))
+
+
+def write_fixture_case(path, npz_path, controls=None):
+    """The case directory stored by tests/golden/make_demo_fixture.py, written back out."""
+    z = np.load(npz_path)
+    pm = os.path.join(path, "constant", "polyMesh")
+    os.makedirs(pm, exist_ok=True)
+    pts = z["points"]
+    _write(os.path.join(pm, "points"), "vectorField", "points",
+           f"{len(pts)}\n(\n" + "\n".join(f"({float(p[0])!r} {float(p[1])!r} {float(p[2])!r})" for p in pts) + "\n)\n")
+    sizes, fp = z["face_sizes"], z["face_points"]
+    ends = np.cumsum(sizes)
+    lines = [f"{n}(" + " ".join(map(str, fp[e - n:e])) + ")" for n, e in zip(sizes, ends)]
+    _write(os.path.join(pm, "faces"), "faceList", "faces", f"{len(lines)}\n(\n" + "\n".join(lines) + "\n)\n")
+    for k in ("owner", "neighbour"):
+        a = z[k]
+        _write(os.path.join(pm, k), "labelList", k, f"{len(a)}\n(\n" + "\n".join(map(str, a)) + "\n)\n")
+    names, types = [str(s) for s in z["patch_names"]], [str(s) for s in z["patch_types"]]
+    body = f"{len(names)}\n(\n"
+    for name, t, n, s in zip(names, types, z["patch_n"], z["patch_start"]):
+        body += f"    {name}\n    {{\n        type {t};\n        nFaces {int(n)};\n        startFace {int(s)};\n    }}\n"
+    _write(os.path.join(pm, "boundary"), "polyBoundaryMesh", "boundary", body + ")\n")
+
+    def spec(v, vector):
+        v = np.asarray(v)
+        if vector:
+            return "nonuniform List<vector> %d\n(\n%s\n)" % (len(v), "\n".join(
+                "(" + " ".join(repr(float(x)) for x in r) + ")" for r in v))
+        return "nonuniform List<scalar> %d\n(\n%s\n)" % (len(v), "\n".join(repr(float(x)) for x in v))
+
+    os.makedirs(os.path.join(path, "0"), exist_ok=True)
+    for k, vec in (("rho", False), ("U", True), ("T", False)):
+        body = f"dimensions [0 0 0 0 0 0 0];\n\ninternalField {spec(z[k + '_internal'], vec)};\n\nboundaryField\n{{\n"
+        for name, pt in zip(names, [str(s) for s in z[k + "_patch_types"]]):
+            body += f"    {name}\n    {{\n        type {pt};\n"
+            if f"{k}_value_{name}" in z.files:
+                body += f"        value {spec(z[f'{k}_value_{name}'], vec)};\n"
+            body += "    }\n"
+        _write(os.path.join(path, "0", k), "volVectorField" if vec else "volScalarField", k, body + "}\n")
+    dvm = (f"fvDVMparas\n{{\n    xiMax {float(z['xi_max'])!r};\n    xiMin {-float(z['xi_max'])!r};\n"
+           f"    nDV {int(z['nDV'])};\n}}\n\ngasProperties\n{{\n    R {float(z['R'])!r};\n"
+           f"    omega {float(z['omega'])!r};\n    Tref {float(z['T_ref'])!r};\n    muRef {float(z['mu_ref'])!r};\n"
+           f"    Pr {float(z['Pr'])!r};\n    KInner {int(z['K'])};\n}}\n")
+    _write(os.path.join(path, "constant", "DVMProperties"), "dictionary", "DVMProperties", dvm)
+    dv = np.load(os.path.join(os.path.dirname(npz_path), "demo_cavity_dv.npz"))
+    for fname, arr in (("Xis", dv["xis"]), ("weights", dv["weights"])):
+        _write(os.path.join(path, "constant", fname), "scalarList", fname,
+               f"{len(arr)}\n(\n" + "\n".join(repr(float(v)) for v in arr) + "\n)\n")
+    ctl = {k[4:]: z[k][()] for k in z.files if k.startswith("ctl_")}
+    ctl.update(controls or {})
+    txt = "application dugksFoam;\n"
+    for k, v in ctl.items():
+        if isinstance(v, (bool, np.bool_)):
+            v = "yes" if v else "no"
+        elif isinstance(v, (float, np.floating)):
+            v = repr(float(v))
+        txt += f"{k} {v};\n"
+    os.makedirs(os.path.join(path, "system"), exist_ok=True)
+    _write(os.path.join(path, "system", "controlDict"), "dictionary", "controlDict", txt)
+    return path

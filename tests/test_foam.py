@@ -1,0 +1,243 @@
+"""Case front-end (dugksfoam_amd.foam) and application loop (dugksfoam_amd.solver).
+
+The reader is checked three ways: on box cases written in OpenFOAM format, where every
+array must equal the project's own generator (cases.box_mesh / cavity / channel); on the
+reference's demo cases when the reference tree is present (mesh counts of
+demo/cavity/constant/polyMesh/boundary:18-41, the 60x60 geometry up to cell numbering,
+DVMProperties and the 0/ fields); and through the solver loop, which must reproduce a
+hand-written loop of Courant number -> setDeltaT -> step on the same backend.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from dugksfoam_amd import cases, foam, solver
+import foam_box
+
+REF = "/root/reference/demo"
+
+
+def _same_case(a: cases.Case, b: cases.Case, tol=1e-13):
+    for k in ("C", "V", "Sf", "Cf", "weights", "delta"):
+        x, y = getattr(a, k), getattr(b, k)
+        assert x.shape == y.shape, k
+        assert np.abs(x - y).max() <= tol * max(1.0, np.abs(y).max()), k
+    assert (a.owner == b.owner).all() and (a.neighbour == b.neighbour).all()
+    assert [(p["name"], p["type"], p["start"], p["size"], p["neighbour_patch"]) for p in a.patches] == \
+           [(p["name"], p["type"], p["start"], p["size"], p["neighbour_patch"]) for p in b.patches]
+    for p, q in zip(a.patches, b.patches):
+        for k in ("U", "T"):
+            if q[k] is not None:
+                np.testing.assert_array_equal(p[k], q[k])
+    np.testing.assert_array_equal(a.xi, b.xi)
+    np.testing.assert_array_equal(a.xi_weights, b.xi_weights)
+    for k in ("xi_max", "R", "omega", "T_ref", "mu_ref", "Pr", "K", "dim"):
+        assert getattr(a, k) == getattr(b, k), k
+    np.testing.assert_array_equal(a.rho0, b.rho0)
+    np.testing.assert_array_equal(a.U0, b.U0)
+    np.testing.assert_array_equal(a.T0, b.T0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_box_cavity_reads_back_as_generated(tmp_path, dim):
+    c = cases.cavity(n=6, nv=8, dim=dim)
+    foam_box.cavity_case(str(tmp_path), c, nz=6 if dim == 3 else 0)
+    r = foam.read_case(str(tmp_path))
+    assert r.dim == dim and r.dt == c.dt
+    _same_case(r, c)
+
+
+def test_box_channel_cyclic_reads_back_as_generated(tmp_path):
+    c = cases.channel(nx=10, ny=4, Lx=2.5, nv=8)
+    rho = np.linspace(0.9, 1.1, c.n_cells)
+    c.rho0 = rho
+    foam_box.channel_case(str(tmp_path), c, nonuniform_rho=rho)
+    r = foam.read_case(str(tmp_path))
+    _same_case(r, c)
+    assert [p["type"] for p in r.patches] == [cases.PATCH_WALL, cases.PATCH_WALL, cases.PATCH_CYCLIC,
+                                               cases.PATCH_CYCLIC]
+
+
+def test_geometry_of_skewed_cells():
+    """Prism with a tilted top face: volume and centroid against the closed forms of a
+    trapezoidal prism; faces closed (sum of outward areas = 0)."""
+    # unit square base, top at z = 1 + x (a wedge-topped hexahedron)
+    P = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                  [0, 0, 1], [1, 0, 2], [1, 1, 2], [0, 1, 1]], dtype=float)
+    faces = [np.array(f) for f in ([0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [3, 7, 6, 2], [0, 4, 7, 3], [1, 2, 6, 5])]
+    Cf, Sf = foam.face_geometry(P, faces)
+    assert np.abs(Sf.sum(axis=0)).max() < 1e-15
+    C, V = foam.cell_geometry(Cf, Sf, np.zeros(6, np.int64), np.zeros(0, np.int64), 1)
+    # V = int_0^1 (1 + x) dx = 1.5; x-centroid = int x(1+x) / V = (1/2 + 1/3)/1.5
+    assert abs(V[0] - 1.5) < 1e-15
+    assert abs(C[0, 0] - (5.0 / 6.0) / 1.5) < 1e-15
+    assert abs(C[0, 1] - 0.5) < 1e-15
+    # z-centroid = int (1+x)^2/2 dx / V = (7/6)/1.5
+    assert abs(C[0, 2] - (7.0 / 6.0) / 1.5) < 1e-15
+
+
+def test_time_controls_follow_set_delta_t():
+    ctl = dict(maxCo=0.8, maxDeltaT=1.0)
+    # immediate reduction
+    assert foam.next_delta_t(8.0, 1e-3, ctl) == pytest.approx(1e-4, rel=1e-14)
+    # growth damped to 1 + 0.1 f, then capped at 1.2
+    assert foam.next_delta_t(0.7, 1e-3, ctl) == pytest.approx(1e-3 * (1 + 0.1 * 0.8 / 0.7), rel=1e-14)
+    assert foam.next_delta_t(0.1, 1e-3, ctl) == pytest.approx(1.2e-3, rel=1e-14)
+    assert foam.next_delta_t(0.1, 0.9, ctl) == 1.0
+
+
+def test_unsupported_patches_rejected(tmp_path):
+    c = cases.cavity(n=4, nv=6)
+    foam_box.cavity_case(str(tmp_path), c)
+    p = tmp_path / "0" / "rho"
+    p.write_text(p.read_text().replace("type calculatedMaxwell;\n        value uniform 1.0;",
+                                       "type symmetryMod;\n        value uniform 1.0;", 1))
+    with pytest.raises(NotImplementedError, match="symmetryMod"):
+        foam.read_case(str(tmp_path))
+
+
+def test_field_writer_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    s, v = rng.standard_normal(7), rng.standard_normal((7, 3))
+    bf = {"a": ("calculated", rng.standard_normal(2)), "b": ("empty", None), "c": ("fixedValue", np.ones(3))}
+    solver.write_field(str(tmp_path / "s"), "rho", s, bf, vector=False)
+    bv = {"a": ("calculated", rng.standard_normal((2, 3))), "b": ("empty", None)}
+    solver.write_field(str(tmp_path / "v"), "U", v, bv, vector=True)
+    si, sb = foam.read_field(str(tmp_path / "s"), 7, {"a": 2, "b": 0, "c": 3}, vector=False)
+    vi, vb = foam.read_field(str(tmp_path / "v"), 7, {"a": 2, "b": 0}, vector=True)
+    np.testing.assert_array_equal(si, s)
+    np.testing.assert_array_equal(vi, v)
+    np.testing.assert_array_equal(sb["a"]["value"], bf["a"][1])
+    np.testing.assert_array_equal(sb["c"]["value"], np.ones(3))
+    np.testing.assert_array_equal(vb["a"]["value"], bv["a"][1])
+    assert sb["b"]["type"] == "empty"
+
+
+class _OracleBackend:
+    """The oracle behind the solver's backend interface (test infrastructure only)."""
+
+    def __init__(self, case, orc):
+        self.o = orc.Oracle(case, threads=4)
+        self.steps = []
+
+    def init_equilibrium(self, rho, U, T):
+        self.o.init_equilibrium(rho, U, T)
+
+    def step(self, dt, n):
+        self.steps.append(dt)
+        self.o.step(dt=dt, nsteps=n)
+
+    def macro(self):
+        return self.o.macro()
+
+    def courant(self, dt):
+        return self.o.courant(dt)
+
+    def wall(self, p):
+        return self.o.wall(p)
+
+
+def test_solver_loop_matches_hand_loop(tmp_path, oracle_lib):
+    """Solver = dugksFoam.C's loop: Courant number, setDeltaT, evolution, writes."""
+    c = cases.cavity(n=6, nv=8)
+    foam_box.cavity_case(str(tmp_path), c, adjust=True, maxCo=0.6, writeInterval=3)
+    s = solver.Solver(str(tmp_path), backend=_OracleBackend(foam.read_case(str(tmp_path)), oracle_lib), log=None)
+    n = s.run(max_steps=7)
+    assert n == 7
+    assert sorted(d for d in os.listdir(tmp_path) if d[0].isdigit() and d != "0") == \
+           sorted(solver.time_name(t, 9) for t in np.cumsum(s.b.steps)[[2, 5, 6]])
+
+    # hand loop: the same steps from the same start
+    ctl = foam.read_controls(str(tmp_path))
+    case = foam.read_case(str(tmp_path))
+    o = oracle_lib.Oracle(case, threads=4)
+    o.init_equilibrium(case.rho0, case.U0, case.T0)
+    dt, dts = ctl["deltaT"], []
+    for _ in range(7):
+        co, _ = o.courant(dt)
+        dt = foam.next_delta_t(co, dt, ctl)
+        dts.append(dt)
+        o.step(dt=dt, nsteps=1)
+    assert dts == s.b.steps
+    assert dts[0] == case.dt          # read_case's dt is the reference's first step
+    m = o.macro()
+    for k in ("rho", "U", "T", "q"):
+        np.testing.assert_array_equal(m[k], s.fields()[k])
+    # the last write holds the final fields and the wall values
+    last = os.path.join(str(tmp_path), solver.time_name(s.t, 9))
+    sizes = {p["name"]: p["nFaces"] for p in case.meta["boundary"]}
+    rho, rb = foam.read_field(os.path.join(last, "rho"), case.n_cells, sizes, vector=False)
+    np.testing.assert_array_equal(rho, m["rho"])
+    np.testing.assert_array_equal(rb["movingWall"]["value"], o.wall(0)["rho_w"])
+    U, ub = foam.read_field(os.path.join(last, "U"), case.n_cells, sizes, vector=True)
+    np.testing.assert_array_equal(U, m["U"])
+    assert ub["frontAndBack"]["type"] == "empty"
+
+
+def test_solver_stops_at_convergence(tmp_path, oracle_lib):
+    """A closed box at rest with all walls at T0 does not move: Uchange = 0/0 -> nan; a
+    gas with a moving lid converges below a loose tolerance only after steps."""
+    c = cases.cavity(n=4, nv=6)
+    foam_box.cavity_case(str(tmp_path), c, extra="convergeTol 0.5;\nconvergeCheckSteps 2;\n")
+    s = solver.Solver(str(tmp_path), backend=_OracleBackend(foam.read_case(str(tmp_path)), oracle_lib), log=None)
+    n = s.run(max_steps=50, write=False)
+    assert n % 2 == 0 and n < 50
+    assert s.change["U"] <= 0.5
+
+
+# ---- the reference's own demo cases (read as text; skipped where the tree is absent) ----
+
+needs_ref = pytest.mark.skipif(not os.path.isdir(REF), reason="reference demo cases not present")
+
+
+@needs_ref
+def test_demo_cavity_mesh_and_properties():
+    c = foam.read_case(os.path.join(REF, "cavity"))
+    # demo/cavity/constant/polyMesh/boundary:18-41, neighbour header
+    assert (c.n_cells, c.n_internal_faces, c.dim) == (3600, 7080, 2)
+    assert [(p["name"], p["start"], p["size"]) for p in c.patches] == [("movingWall", 7080, 60), ("fixedWalls", 7140, 180)]
+    # same geometry as the 60 x 60 generator with dz = 0.1, up to cell numbering
+    d, C, V, own, nei, Sf, Cf, sides = cases.box_mesh(60, 60, dz2d=0.1)
+    idx = np.rint(c.C[:, 0] * 60 - 0.5).astype(int) + 60 * np.rint(c.C[:, 1] * 60 - 0.5).astype(int)
+    assert np.unique(idx).size == 3600
+    assert np.abs(c.C - C[idx]).max() < 1e-15
+    assert np.abs(c.V - V[idx]).max() < 1e-17
+    pairs = set(zip(idx[c.owner[:7080]].tolist(), idx[c.neighbour].tolist()))
+    assert pairs <= (set(zip(own.tolist(), nei.tolist())) | set(zip(nei.tolist(), own.tolist())))
+    assert np.allclose(c.weights[:7080], 0.5, rtol=0, atol=1e-13)   # points carry 16 digits
+    assert abs(c.V.sum() - 0.1) < 1e-15
+    # DVMProperties:27-30 and 0/rho:20
+    assert (c.R, c.mu_ref, c.omega, c.T_ref, c.K) == (208.244343891, 1.60281882485e-05, 0.81, 273.0, 0)
+    assert c.Pr == pytest.approx(2.0 / 3.0, rel=1e-14)
+    assert c.rho0[0] == 1.14423514927e-06 and c.T0[0] == 273.0
+    lid = c.patches[0]
+    assert (lid["U"] == [50.0, 0.0, 0.0]).all() and (lid["T"] == 273.0).all()
+    assert c.nv == 28 * 28 and c.xi_max == pytest.approx(1766.485662735688, rel=1e-15)
+    # setDeltaTvar.H with the demo's maxCo 0.8 from deltaT 4e-4, gas at rest
+    assert c.dt == pytest.approx(0.8 / (60 * math.sqrt(2) * c.xi_max), rel=1e-13)
+
+
+@needs_ref
+def test_demo_test_symmetry_cases():
+    full = foam.read_case(os.path.join(REF, "testSymmetry", "full"))
+    assert full.n_cells == 3600 and [p["size"] for p in full.patches] == [60, 180]
+    with pytest.raises(NotImplementedError, match="symmetryMod"):
+        foam.read_case(os.path.join(REF, "testSymmetry", "symmetry"))
+
+
+def test_demo_fixture_case(tmp_path):
+    """demo/cavity from its array fixture: counts of demo/cavity/constant/polyMesh/boundary:18-41,
+    the 60 x 60 geometry, DVMProperties:27-30, 0/rho:20, the lid of 0/U."""
+    foam_box.write_fixture_case(str(tmp_path), os.path.join(os.path.dirname(__file__), "golden", "demo_cavity_case.npz"))
+    c = foam.read_case(str(tmp_path))
+    assert (c.n_cells, c.n_internal_faces, c.dim, c.nv) == (3600, 7080, 2, 784)
+    assert [(p["name"], p["start"], p["size"]) for p in c.patches] == [("movingWall", 7080, 60), ("fixedWalls", 7140, 180)]
+    assert abs(c.V.sum() - 0.1) < 1e-15 and np.abs(c.V - 0.1 / 3600).max() < 1e-17
+    assert (c.R, c.mu_ref, c.omega, c.T_ref, c.K) == (208.244343891, 1.60281882485e-05, 0.81, 273.0, 0)
+    assert c.rho0[0] == 1.14423514927e-06 and (c.patches[0]["U"] == [50.0, 0.0, 0.0]).all()
+    assert c.dt == pytest.approx(0.8 / (60 * math.sqrt(2) * c.xi_max), rel=1e-13)
+    if os.path.isdir(REF):
+        _same_case(c, foam.read_case(os.path.join(REF, "cavity")), tol=0.0)
