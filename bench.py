@@ -26,16 +26,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = None):
+def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = None, face_values: bool = False):
     """Algorithmic bytes per launch of each step kernel: every array the kernel must
-    touch, read once and written once (the per-kernel table of DESIGN.md).
+    touch, read once and written once.
 
     k_face reads the resident gbar rows (each cell's row once, however many faces use it)
     and the lagged wall-gradient rows, writes the new wall-gradient rows and the block
-    partial sums; k_cell reads gbar and the wall rows and writes g~ and its partial sums;
+    partial sums, and in face-value mode the face values; k_cell reads gbar (and the wall
+    rows, or in face-value mode each face value once) and writes g~ and its partial sums;
     k_cell_s1 reads g~ and writes the next gbar; the two _fin kernels touch O(faces) and
     O(cells) data.  h is carried only when K + 3 - D != 0.  The whole-step figure is the
-    floor of any implementation: gbar read twice and written once (3 s per update)."""
+    floor of the scheme: gbar read twice and written once (3 s per update), plus the face
+    values written and read once in face-value mode."""
     s = storage_bytes
     nd = 1 if (case.K + 3 - case.dim) == 0 else 2
     nc, nv = case.n_cells, case.nv
@@ -45,14 +47,16 @@ def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = No
     nm_c = nm_f - 1
     fpart = nf * n_blocks * nm_f * 8
     cpart = nc * (n_blocks if n_blocks_cell is None else n_blocks_cell) * nm_c * 8
+    fvb = nd * s * nf * nv if face_values else 0
     kb = {
-        "k_face": nd * s * (nc * nv + 2 * nw * nv) + fpart,
+        "k_face": nd * s * (nc * nv + 2 * nw * nv) + fpart + fvb,
         "k_face_fin": fpart + nf * 8 * (3 + 8 + 16),
-        "k_cell": nd * s * (2 * nc * nv + nw * nv) + cpart + nf * 8 * 16,
+        "k_cell": (nd * s * 2 * nc * nv + fvb if face_values else nd * s * (2 * nc * nv + nw * nv))
+        + cpart + nf * 8 * 16,
         "k_cell_fin": cpart + nc * 8 * 9,
         "k_cell_s1": nd * s * 2 * nc * nv,
     }
-    step_min = nd * s * (3 * nc * nv + 3 * nw * nv)
+    step_min = nd * s * (3 * nc * nv + 3 * nw * nv) + 2 * fvb
     return kb, step_min
 
 
@@ -69,10 +73,16 @@ def traffic_from_profile(kernel_prefix: str, storage: str):
         return None
 
 
-def run_gpu(case, storage: str, steps: int, warmup: int):
+def run_gpu(case, storage: str, steps: int, warmup: int, tune: bool = True):
     from dugksfoam_amd.engine import Engine
     e = Engine(case, storage=storage)
     e.init_equilibrium()
+    tune_steps = 0
+    if tune:
+        # the engine's launch-shape tuner: real steps, one candidate shape per step, untimed
+        e.autotune(True)
+        tune_steps = e.autotune_steps_left()
+        e.step(nsteps=tune_steps)
     e.step(nsteps=warmup)
     e.set_profiling(True)
     t0 = time.perf_counter()
@@ -81,13 +91,14 @@ def run_gpu(case, storage: str, steps: int, warmup: int):
     kms, n = e.profile()
     n_blocks, n_blocks_cell = e.blocks()[2], e.cell_blocks()[2]
     shapes = {"k_face": e.blocks()[:3], "k_cell": e.cell_blocks()}
+    fv = e.face_values()
     m = e.macro()
     ok = bool(np.all(np.isfinite(m["rho"])) and np.all(m["rho"] > 0))
     e.close()
     wall = t1 - t0
     upd = case.n_cells * case.nv * steps
     sb = 8 if storage == "fp64" else 4
-    kb, step_min = bytes_model(case, sb, n_blocks, n_blocks_cell)
+    kb, step_min = bytes_model(case, sb, n_blocks, n_blocks_cell, fv)
     kern = {k: (kms[k] / n, kb[k]) for k in kb}
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms, bytes_ = kern[dom]
@@ -102,6 +113,8 @@ def run_gpu(case, storage: str, steps: int, warmup: int):
                            "GBs": round(step_min / (step_ms * 1e-3) / 1e9, 1),
                            "frac": round(step_min / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     roof["launch_shapes"] = {k: {"threads": v[0], "iters": v[1], "velocity_blocks": v[2]} for k, v in shapes.items()}
+    roof["launch_shapes"]["tuning_steps"] = tune_steps
+    roof["face_values"] = fv
     return dict(value=upd / wall / 1e9, ms_per_step=wall / steps * 1e3, kernel_ms=step_ms,
                 roofline=roof, finite=ok)
 
@@ -134,6 +147,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage form")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no tuning steps)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (rocprof counter calibration)")
     a = ap.parse_args()
@@ -148,13 +162,13 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs the DUGKS engine on a single MI355X; "
-              "multi-GPU velocity-space sharding is not built (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the DUGKS engine runs on a single MI355X; "
+              "velocity-space sharding over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
     from dugksfoam_amd import cases
     case = cases.config("cavity3d")
     log(f"[bench] cavity3d cells={case.n_cells} nv={case.nv} steps={a.steps} warmup={a.warmup}")
-    r64 = run_gpu(case, "fp64", a.steps, a.warmup)
+    r64 = run_gpu(case, "fp64", a.steps, a.warmup, tune=not a.no_tune)
     log(f"[bench] fp64 {r64['value']:.2f} G upd/s, {r64['ms_per_step']:.2f} ms/step")
     out = {
         "metric": "G cell-velocity updates/s (FP64 plain path; FP32-storage form in fp32)",
@@ -175,7 +189,7 @@ def main():
         "roofline": r64["roofline"],
     }
     if not a.no_fp32:
-        r32 = run_gpu(case, "fp32", a.steps, a.warmup)
+        r32 = run_gpu(case, "fp32", a.steps, a.warmup, tune=not a.no_tune)
         log(f"[bench] fp32 {r32['value']:.2f} G upd/s")
         out["fp32"] = {"value": round(r32["value"], 3), "ms_per_step": round(r32["ms_per_step"], 3),
                        "speedup_vs_fp64": round(r32["value"] / r64["value"], 3), "roofline": r32["roofline"]}

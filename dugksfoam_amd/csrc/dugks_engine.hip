@@ -66,7 +66,8 @@ constexpr int NIBMAX = 64;     // velocity iterations per thread in a block
 constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
 // Minimum resident waves per SIMD asked of the register allocator (measured on the 48^3 x 24^3
 // cavity: 6 for the FP64 k_face costs a few spill bytes and gains 4 %, the FP32 one spills more
-// and is held at 5; 5 for k_cell spills in its face loop and loses 35 %).
+// and is held at 5; 5 for k_cell spills in its face loop and loses 35 %; the face-value k_cell,
+// which rebuilds no gradients, runs best at 5 of 4, 5, 6, 8).
 #ifndef FACE_WAVES
 #define FACE_WAVES 6
 #endif
@@ -75,6 +76,9 @@ constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
 #endif
 #ifndef CELL_WAVES
 #define CELL_WAVES 4
+#endif
+#ifndef CELL_WAVES_FV
+#define CELL_WAVES_FV 5
 #endif
 
 // One (cell, face) incidence; a cell's slots are in ascending face order, the order in which the
@@ -181,6 +185,7 @@ struct P {
     TS *gb_out, *hb_out;
     const TS *gBg_in, *gBh_in;
     TS *gBg_out, *gBh_out;
+    TS *fvg, *fvh;      // stored face values [face][velocity] (face-value mode), else null
     int *flag;
 };
 
@@ -545,7 +550,7 @@ DEVI bool xcd_block(int n, int nb, int &pos, int &b)
 // across the lanes and written as the block's partial sums.  Wall faces also write the next
 // step's fixedGradient boundary gradient n . grad(gbar) (discreteVelocity.C:436-470).
 // ---------------------------------------------------------------------------------------------
-template <int D, bool H, int NS, typename TS>
+template <int D, bool H, int NS, typename TS, bool FV>
 __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 8 ? FACE_WAVES : FACE_WAVES_FP32, 8))) void k_face(P<TS> p)
 {
     constexpr int NM = NMom<D>::v + 1;  // + wall outGoing
@@ -611,8 +616,14 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
             }
         }
         if (!wall || s > 0) {
-            const double gf = face_value<D>(kind, s, gP, GP, dP, gN, GN, dN, wcy, hx) + row_off<D>(p, v, 0, vt);
-            const double hf = H ? face_value<D>(kind, s, hP, HP, dP, hN, HN, dN, wcy, hx) + row_off<D>(p, v, 1, vt) : 0.0;
+            const double gfd = face_value<D>(kind, s, gP, GP, dP, gN, GN, dN, wcy, hx);
+            const double hfd = H ? face_value<D>(kind, s, hP, HP, dP, hN, HN, dN, wcy, hx) : 0.0;
+            if (FV) {
+                p.fvg[(size_t)f * p.nvp + k] = (TS)gfd;
+                if (H) p.fvh[(size_t)f * p.nvp + k] = (TS)hfd;
+            }
+            const double gf = gfd + row_off<D>(p, v, 0, vt);
+            const double hf = H ? hfd + row_off<D>(p, v, 1, vt) : 0.0;
             acc_moments<D, H>(acc, gf, hf, v, Uh);
             if (wall) acc[NM - 1] = fma(v.w * s, gf, acc[NM - 1]);
         }
@@ -713,8 +724,8 @@ __global__ __launch_bounds__(256) void k_face_fin(P<TS> p)
 // output buffer, where k_cell_s1 turns it into the next gbar.
 // Dynamic LDS: [velocity table (3 nax + 2) | tabC | tabF x MAXS | tabW x MAXS], nax doubles each.
 // ---------------------------------------------------------------------------------------------
-template <int D, bool H, int NS, typename TS>
-__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVES, 8))) void k_cell(P<TS> p)
+template <int D, bool H, int NS, typename TS, bool FV>
+__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL_WAVES_FV : CELL_WAVES, 8))) void k_cell(P<TS> p)
 {
     constexpr int NC = NMom<D>::v;
     extern __shared__ double tabs[];
@@ -777,7 +788,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
         const double og = row_off<D>(p, v, 0, vt), oh = H ? row_off<D>(p, v, 1, vt) : 0.0;
         const double gCv = gC + og, hCv = hC + oh;      // gbar itself
         double GC[3] = {0, 0, 0}, HC[3] = {0, 0, 0};
-        grad_at<D, H, NS>(p, c, k, gC, hC, GC, HC);
+        if (!FV) grad_at<D, H, NS>(p, c, k, gC, hC, GC, HC);
         // g~ of this step from gbar (inverse of S1)
         double geq, heq;
         eq_eval<D, H>(ec, tabC, p.nxi, v, p.kc, p.K, geq, heq);
@@ -799,7 +810,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
             else needO = need_N(kind, s);
             double gO = 0.0, hO = 0.0;
             double GO[3] = {0, 0, 0}, HO[3] = {0, 0, 0};
-            if (__any(needO)) {
+            if (!FV && __any(needO)) {
                 const int o = S.other;
                 gO = ld(p.gb_in + (size_t)o * p.nvp, k);
                 if (H) hO = ld(p.hb_in + (size_t)o * p.nvp, k);
@@ -817,7 +828,11 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(CELL_WAVE
                 gf = fe.rho_w * m;
                 if (H) hf = gf * (p.R * fe.pad[0]) * p.kc;
             } else {
-                if (nei) {
+                if (FV) {
+                    // the value k_face rebuilt for this face (same arithmetic, stored)
+                    gf = ld(p.fvg + (size_t)S.face * p.nvp, k) + og;
+                    if (H) hf = ld(p.fvh + (size_t)S.face * p.nvp, k) + oh;
+                } else if (nei) {
                     gf = face_value<D>(KIND_OWN, s, gO, GO, dCo, gC, GC, dCs, 0.0, hx) + og;
                     if (H) hf = face_value<D>(KIND_OWN, s, hO, HO, dCo, hC, HC, dCs, 0.0, hx) + oh;
                 } else {
@@ -1149,6 +1164,8 @@ struct dugks_engine {
     double *fpart = nullptr, *cpart = nullptr;
     void *gb[2] = {nullptr, nullptr}, *hb[2] = {nullptr, nullptr};
     void *gB[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
+    int face_values = 0;  // face-value mode: k_face stores the face values, k_cell reads them
+    void *fvg = nullptr, *fvh = nullptr;
     int cur = 0;         // buffer holding the current state
     int raw = 1;         // state holds g~ (1) or gbar prepared with dt_prep (0)
     int initialised = 0;
@@ -1237,6 +1254,8 @@ P<TS> params(dugks_engine *h)
     p.gb_out = (TS *)h->gb[1 - h->cur]; p.hb_out = (TS *)h->hb[1 - h->cur];
     p.gBg_in = (const TS *)h->gB[h->cur]; p.gBh_in = (const TS *)h->hB[h->cur];
     p.gBg_out = (TS *)h->gB[1 - h->cur]; p.gBh_out = (TS *)h->hB[1 - h->cur];
+    p.fvg = h->face_values ? (TS *)h->fvg : nullptr;
+    p.fvh = h->face_values ? (TS *)h->fvh : nullptr;
     p.flag = h->flag;
     return p;
 }
@@ -1251,21 +1270,28 @@ dim3 block_grid(int n, int nb) { return dim3((unsigned)(((n + NXCD - 1) / NXCD) 
 constexpr int NKERN = 5;  // k_face, k_face_fin, k_cell, k_cell_fin, k_cell_s1
 constexpr int S1_NT = 256;  // workgroup size of k_cell_s1
 
-template <int D, bool H, int NS, typename TS>
-void launch_step_ns(dugks_engine *h, hipEvent_t *ev)
+template <int D, bool H, int NS, typename TS, bool FV>
+void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
 {
     P<TS> p = params<TS>(h);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    hipLaunchKernelGGL((k_face<D, H, NS, TS>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
+    hipLaunchKernelGGL((k_face<D, H, NS, TS, FV>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[1], h->stream);
     hipLaunchKernelGGL((k_face_fin<D, TS>), dim3((h->nf + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[2], h->stream);
-    hipLaunchKernelGGL((k_cell<D, H, NS, TS>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
+    hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
     if (ev) (void)hipEventRecord(ev[3], h->stream);
     hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[4], h->stream);
     hipLaunchKernelGGL((k_cell_s1<D, H, TS>), dim3(h->nc), dim3(S1_NT), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[5], h->stream);
+}
+
+template <int D, bool H, int NS, typename TS>
+void launch_step_ns(dugks_engine *h, hipEvent_t *ev)
+{
+    if (h->face_values) launch_step_fv<D, H, NS, TS, true>(h, ev);
+    else launch_step_fv<D, H, NS, TS, false>(h, ev);
 }
 
 // stencil sizes: 4 (triangles, quads) or 8 in 2-D, 6 (hexahedra, tetrahedra) or 8 in 3-D
@@ -1432,6 +1458,20 @@ void set_cell_shape(dugks_engine *h, int nt, int nib)
     h->nvbc = h->nvp / (nt * nib);
 }
 
+// Default cell-kernel shape: one workgroup per cell, the largest workgroup that tiles nvp in
+// NIBMAX iterations or fewer, up to 256 threads in face-value mode and MAXNT otherwise (the
+// fastest on the 48^3 x 24^3 cavity in each mode); else the face kernel's shape.
+void default_cell_shape(dugks_engine *h)
+{
+    int ct = 0;
+    for (int t = h->face_values ? 256 : MAXNT; t >= 64 && !ct; t -= 64)
+        if (h->nvp % t == 0 && h->nvp / t <= NIBMAX) ct = t;
+    for (int t = MAXNT; t >= 64 && !ct; t -= 64)
+        if (h->nvp % t == 0 && h->nvp / t <= NIBMAX) ct = t;
+    if (ct) set_cell_shape(h, ct, h->nvp / ct);
+    else set_cell_shape(h, h->nt, h->nib);
+}
+
 void shape_candidates(dugks_engine *h)
 {
     h->face_cand.clear();
@@ -1447,11 +1487,7 @@ void shape_candidates(dugks_engine *h)
     std::sort(h->face_cand.begin(), h->face_cand.end());
     h->face_cand.erase(std::unique(h->face_cand.begin(), h->face_cand.end()), h->face_cand.end());
     h->cell_cand = h->face_cand;
-    int ct = 0;
-    for (int t = MAXNT; t >= 64 && !ct; t -= 64)
-        if (h->nvp % t == 0 && h->nvp / t <= NIBMAX) ct = t;
-    if (ct) set_cell_shape(h, ct, h->nvp / ct);
-    else set_cell_shape(h, h->nt, h->nib);
+    default_cell_shape(h);
     h->nb_cap = h->nvb;
     h->nbc_cap = h->nvbc;
     for (auto &c : h->face_cand) h->nb_cap = std::max(h->nb_cap, h->nvp / (c.first * c.second));
@@ -1919,6 +1955,8 @@ void release(dugks_engine *h)
         dfree(h->gb[i]); dfree(h->hb[i]);
         h->gB[i] = h->hB[i] = nullptr;
     }
+    dfree(h->fvg); dfree(h->fvh);
+    h->face_values = 0;
     drop_graphs(h);
     for (hipEvent_t &e : h->tev) {
         if (e) (void)hipEventDestroy(e);
@@ -2122,6 +2160,15 @@ int dugks_create(const dugks_case *cs, dugks_engine **out)
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) r = fail(h, DUGKS_ERR_DEVICE, "no HIP device");
         else if (cs->device < 0 || cs->device >= ndev) r = fail(h, DUGKS_ERR_INVALID, "device ordinal");
         else r = build(h, cs);
+    }
+    if (!r) {
+        // face-value mode by default when its buffers take at most half of the free device
+        // memory; DUGKS_FACE_VALUES=0/1 in the environment forces it off/on
+        const char *fv = std::getenv("DUGKS_FACE_VALUES");
+        size_t fr = 0, tot = 0;
+        const size_t need = (size_t)h->nf * h->nvp * ts_size(h) * (h->has_h ? 2 : 1);
+        const bool fits = hipMemGetInfo(&fr, &tot) == hipSuccess && need <= fr / 2;
+        if (fv ? std::atoi(fv) != 0 : fits) r = dugks_set_face_values(h, 1);
     }
     if (r) {
         // the handle is returned without device resources so that dugks_last_error can
@@ -2439,6 +2486,21 @@ int dugks_set_autotune(dugks_engine *h, int32_t on)
     return 0;
 }
 
+int dugks_get_face_values(dugks_engine *h, int32_t *on)
+{
+    if (!h || !on) return DUGKS_ERR_INVALID;
+    *on = h->face_values;
+    return 0;
+}
+
+int dugks_autotune_steps_left(dugks_engine *h, int32_t *steps)
+{
+    if (!h || !steps) return DUGKS_ERR_INVALID;
+    const int n = std::max((int)h->face_cand.size(), (int)h->cell_cand.size());
+    *steps = h->tuning ? n - h->tune_i : 0;
+    return 0;
+}
+
 int dugks_get_cell_launch(dugks_engine *h, int32_t *nt, int32_t *nib, int32_t *n_blocks)
 {
     if (!h) return DUGKS_ERR_INVALID;
@@ -2452,6 +2514,29 @@ int dugks_set_graphs(dugks_engine *h, int32_t on)
 {
     if (!h) return DUGKS_ERR_INVALID;
     h->use_graphs = on ? 1 : 0;
+    return 0;
+}
+
+int dugks_set_face_values(dugks_engine *h, int32_t on)
+{
+    if (!h || !h->stream) return h ? fail(h, DUGKS_ERR_STATE, "no device resources") : DUGKS_ERR_INVALID;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (on && !h->fvg) {
+        const size_t bytes = (size_t)h->nf * h->nvp * ts_size(h);
+        int r;
+        if ((r = dallocb(h, &h->fvg, bytes))) return r;
+        if (h->has_h && (r = dallocb(h, &h->fvh, bytes))) {
+            dfree(h->fvg);
+            return r;
+        }
+    } else if (!on) {
+        dfree(h->fvg);
+        dfree(h->fvh);
+    }
+    h->face_values = on ? 1 : 0;
+    default_cell_shape(h);
+    drop_graphs(h);
     return 0;
 }
 

@@ -22,6 +22,7 @@ SYMBOLS = (
     "dugks_state_size", "dugks_save_state", "dugks_load_state", "dugks_last_step_ms", "dugks_destroy",
     "dugks_last_error", "dugks_abi_version", "dugks_set_profiling", "dugks_get_profile",
     "dugks_get_launch", "dugks_set_graphs", "dugks_set_autotune", "dugks_get_cell_launch",
+    "dugks_set_face_values", "dugks_get_face_values", "dugks_autotune_steps_left",
 )
 
 _lib = None
@@ -65,6 +66,9 @@ def load_library(path: str = LIB_PATH):
     L.dugks_get_profile.argtypes = [vp, dp, C.POINTER(C.c_int64)]
     L.dugks_get_launch.argtypes = [vp, ip, ip, ip, ip]
     L.dugks_set_graphs.argtypes = [vp, C.c_int32]
+    L.dugks_set_face_values.argtypes = [vp, C.c_int32]
+    L.dugks_autotune_steps_left.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.dugks_get_face_values.argtypes = [vp, C.POINTER(C.c_int32)]
     L.dugks_set_autotune.argtypes = [vp, C.c_int32]
     L.dugks_get_cell_launch.argtypes = [vp, ip, ip, ip]
     _lib = L
@@ -192,6 +196,15 @@ class Engine:
         """Replay captured two-step HIP graphs for multi-step calls (default on)."""
         self._chk(load_library().dugks_set_graphs(self._h, 1 if on else 0))
 
+    def set_face_values(self, on: bool = True):
+        """Face-value mode: stored face values instead of rebuilding them in the cell kernel."""
+        self._chk(load_library().dugks_set_face_values(self._h, int(bool(on))))
+
+    def face_values(self) -> bool:
+        n = C.c_int32()
+        self._chk(load_library().dugks_get_face_values(self._h, C.byref(n)))
+        return bool(n.value)
+
     def blocks(self):
         """Face kernel: (workgroup size, velocity iterations per thread, velocity blocks, padded Nv)."""
         v = [C.c_int32() for _ in range(4)]
@@ -207,6 +220,11 @@ class Engine:
     def autotune(self, on: bool = True):
         """Time the candidate work decompositions on the next steps and keep the fastest."""
         self._chk(load_library().dugks_set_autotune(self._h, 1 if on else 0))
+
+    def autotune_steps_left(self) -> int:
+        n = C.c_int32()
+        self._chk(load_library().dugks_autotune_steps_left(self._h, C.byref(n)))
+        return n.value
 
     @property
     def last_step_ms(self) -> float:

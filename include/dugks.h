@@ -154,7 +154,18 @@ int dugks_set_graphs(dugks_engine* h, int32_t on);
  * real steps of the simulation; the candidates differ only in how velocity sums are split,
  * i.e. in round-off.  dugks_get_launch / dugks_get_cell_launch report the shapes in use. */
 int dugks_set_autotune(dugks_engine* h, int32_t on);
+/* Steps still to run before the tuning started by dugks_set_autotune ends (0 when idle). */
+int dugks_autotune_steps_left(dugks_engine* h, int32_t* steps);
 int dugks_get_cell_launch(dugks_engine* h, int32_t* nt, int32_t* nib, int32_t* n_blocks);
+
+/* Face-value mode (no reference counterpart): with it on, the face kernel stores every face
+ * value it rebuilds ([face][velocity], storage precision) and the cell kernel reads them instead
+ * of rebuilding them from gbar and its gradients.  FP64 results are bit-identical either way;
+ * with FP32 storage the face values are rounded to FP32 like the other stored rows.  Costs
+ * n_faces x nv_padded x (4 or 8) bytes per carried distribution.  On by default when those
+ * buffers take at most half of the free device memory at create. */
+int dugks_set_face_values(dugks_engine* h, int32_t on);
+int dugks_get_face_values(dugks_engine* h, int32_t* on);
 
 /* Work decomposition of the face kernel (no reference counterpart; for measurement and
  * tuning): workgroup size, velocity iterations per thread in a block, number of
