@@ -140,6 +140,48 @@ def cpu_baseline(target_s: float = 12.0):
             "sample": f"FP64 oracle, 48^3 cells x 12^3 velocities, {n} step(s), {t:.1f} s"}
 
 
+SWEEP = (("cavity2d", 2000), ("delaunay", 150), ("cavity3d", 200), ("batch", 1000), ("channel", 2000))
+
+
+def sweep():
+    """Every workload of the scope table in FP64 and in the FP32-storage form: throughput over
+    the stated step count (launch-shape tuning steps included in the timed run) and the
+    relative L2 distance of the FP32-form fields from the FP64 fields.  The Delaunay cavity
+    runs 150 steps: the reference scheme diverges on that mesh near step 170."""
+    from dugksfoam_amd import cases
+    from dugksfoam_amd.engine import Engine
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    for name, n in SWEEP:
+        c = cases.config(name)
+        log(f"[sweep] {name} cells={c.n_cells} nv={c.nv} steps={n}")
+        out = {}
+        for st in ("fp64", "fp32"):
+            e = Engine(c, storage=st)
+            e.init_equilibrium()
+            e.autotune(True)
+            t0 = time.perf_counter()
+            done = 0
+            while done < n:
+                k = min(100, n - done)
+                e.step(nsteps=k)
+                done += k
+            m = e.macro()
+            out[st] = (m, time.perf_counter() - t0)
+            e.close()
+        m64, m32 = out["fp64"][0], out["fp32"][0]
+        r = {"workload": name, "cells": c.n_cells, "velocities": c.nv, "steps": n,
+             "fp64_G_updates_per_s": round(c.n_cells * c.nv * n / out["fp64"][1] / 1e9, 2),
+             "fp32_G_updates_per_s": round(c.n_cells * c.nv * n / out["fp32"][1] / 1e9, 2),
+             "fp32_vs_fp64_rel_l2": {k: float(f"{rel(m32[k], m64[k]):.3g}") for k in ("rho", "U", "T", "q")}}
+        if c.n_cases > 1:
+            cc = np.asarray(c.cell_case)
+            r["fp32_worst_case_rho_T"] = float(f"{max(max(rel(m32[k][cc == i], m64[k][cc == i]) for k in ('rho', 'T')) for i in range(c.n_cases)):.3g}")
+        print(json.dumps(r), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +190,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage form")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no tuning steps)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only run every workload in FP64 and FP32 storage (throughput, FP32 error)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (rocprof counter calibration)")
     a = ap.parse_args()
@@ -160,6 +204,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
+    if a.sweep:
+        sweep()
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the DUGKS engine runs on a single MI355X; "
