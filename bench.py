@@ -32,8 +32,10 @@ def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = No
 
     k_face reads the resident gbar rows (each cell's row once, however many faces use it)
     and the lagged wall-gradient rows, writes the new wall-gradient rows and the block
-    partial sums, and in face-value mode the face values; k_cell reads gbar (and the wall
-    rows, or in face-value mode each face value once) and writes g~ and its partial sums;
+    partial sums.  In face-value mode k_face_values does the same reads and writes the face
+    values instead of partial sums, and k_face_moments reads the face values and writes the
+    partial sums.  k_cell reads gbar (and the wall rows, or in face-value mode each face value
+    once) and writes g~ and its partial sums;
     k_cell_s1 reads g~ and writes the next gbar; the two _fin kernels touch O(faces) and
     O(cells) data.  h is carried only when K + 3 - D != 0.  The whole-step figure is the
     floor of the scheme: gbar read twice and written once (3 s per update), plus the face
@@ -48,14 +50,18 @@ def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = No
     fpart = nf * n_blocks * nm_f * 8
     cpart = nc * (n_blocks if n_blocks_cell is None else n_blocks_cell) * nm_c * 8
     fvb = nd * s * nf * nv if face_values else 0
-    kb = {
-        "k_face": nd * s * (nc * nv + 2 * nw * nv) + fpart + fvb,
+    if face_values:
+        kb = {"k_face_values": nd * s * (nc * nv + 2 * nw * nv) + fvb,
+              "k_face_moments": fvb + fpart}
+    else:
+        kb = {"k_face": nd * s * (nc * nv + 2 * nw * nv) + fpart}
+    kb.update({
         "k_face_fin": fpart + nf * 8 * (3 + 8 + 16),
         "k_cell": (nd * s * 2 * nc * nv + fvb if face_values else nd * s * (2 * nc * nv + nw * nv))
         + cpart + nf * 8 * 16,
         "k_cell_fin": cpart + nc * 8 * 9,
         "k_cell_s1": nd * s * 2 * nc * nv,
-    }
+    })
     step_min = nd * s * (3 * nc * nv + 3 * nw * nv) + 2 * fvb
     return kb, step_min
 

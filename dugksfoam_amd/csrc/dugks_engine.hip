@@ -80,6 +80,12 @@ constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
 #ifndef CELL_WAVES_FV
 #define CELL_WAVES_FV 5
 #endif
+#ifndef KC_PRELOAD
+#define KC_PRELOAD 1
+#endif
+#ifndef FM_UNR
+#define FM_UNR 2
+#endif
 
 // One (cell, face) incidence; a cell's slots are in ascending face order, the order in which the
 // reference's face loops touch that cell.
@@ -633,6 +639,154 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
 }
 
 // ---------------------------------------------------------------------------------------------
+// Face-value mode, S2 in two passes.
+// k_face_values: cell-wise, over one velocity block.  The least-squares gradient of the cell is
+// formed once per velocity, and the cell writes the upwind face value of every face it is upwind
+// of: internal faces it owns with xi.S >= -VSMALL (|xi.S| <= VSMALL is the mean of both
+// reconstructions, for which the neighbour's gradient is formed on demand), internal faces it
+// neighbours with xi.S < -VSMALL, its cyclic faces (the coupled value needs the partner cell's
+// gradient where xi.S <= VSMALL) and the outgoing half of its wall faces, whose lagged
+// fixedGradient rows n . grad(gbar) it writes as well.  Every face value is written exactly once,
+// with the arithmetic of k_face (same gradient, same face_value), so the stored values equal
+// k_face's.
+// ---------------------------------------------------------------------------------------------
+template <int D, bool H, int NS, typename TS>
+__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 8 ? FACE_WAVES : FACE_WAVES_FP32, 8))) void k_face_values(P<TS> p)
+{
+    __shared__ double vt[VT_SIZE];
+    int pos, b;
+    if (!xcd_block(p.nc, p.nb, pos, b)) return;
+    stage_vt(p, vt);
+    __syncthreads();
+    const int c = cptr(p.corder)[pos];
+    const int cas = cptr(p.ccase)[c];
+    const double dt = cptr(p.dtc)[cas];
+    const int s0 = cptr(p.cs)[c];
+    const int ns = cptr(p.cs)[c + 1] - s0;
+    const CONST_AS Slot *SL = cptr(p.slots) + s0;
+    const TS *__restrict__ rowC = p.gb_in + (size_t)c * p.nvp;
+    const TS *__restrict__ hrowC = H ? p.hb_in + (size_t)c * p.nvp : nullptr;
+    const unsigned k0 = (unsigned)b * p.vb + threadIdx.x;
+    for (int it = 0; it < p.nib; ++it) {
+        const unsigned k = k0 + it * p.nt;
+        const Vel v = load_vel<D>(p, k, vt);
+        const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
+        const double gC = ld(rowC, k);
+        const double hC = H ? ld(hrowC, k) : 0.0;
+        double GC[3] = {0, 0, 0}, HC[3] = {0, 0, 0};
+        grad_at<D, H, NS>(p, c, k, gC, hC, GC, HC);
+        for (int j = 0; j < ns; ++j) {
+            const CONST_AS Slot &S = SL[j];
+            const int kind = S.kind;
+            const double Sv[3] = {S.S[0], S.S[1], S.S[2]};
+            const double s = dotnc(v.x, v.y, v.z, Sv);
+            const double dCs[3] = {S.dCs[0], S.dCs[1], S.dCs[2]};
+            const double dCo[3] = {S.dCo[0], S.dCo[1], S.dCo[2]};
+            TS *__restrict__ fg = p.fvg + (size_t)S.face * p.nvp;
+            TS *__restrict__ fh = H ? p.fvh + (size_t)S.face * p.nvp : nullptr;
+            if (kind == KIND_WALL) {
+                const size_t bi = (size_t)S.bidx * p.nvp + k;
+                double ng = GC[0] * dCo[0] + GC[1] * dCo[1];
+                if (D == 3) ng += GC[2] * dCo[2];
+                p.gBg_out[bi] = (TS)ng;
+                if (H) {
+                    double nh = HC[0] * dCo[0] + HC[1] * dCo[1];
+                    if (D == 3) nh += HC[2] * dCo[2];
+                    p.gBh_out[bi] = (TS)nh;
+                }
+                if (s > 0) {
+                    fg[k] = (TS)face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx);
+                    if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx);
+                }
+                continue;
+            }
+            if (kind == KIND_NEI) {
+                // this cell is the face's neighbour N: upwind where xi.S < -VSMALL
+                if (s < -VSMALL) {
+                    const double z3[3] = {0, 0, 0};
+                    fg[k] = (TS)face_value<D>(KIND_OWN, s, 0.0, z3, dCo, gC, GC, dCs, 0.0, hx);
+                    if (H) fh[k] = (TS)face_value<D>(KIND_OWN, s, 0.0, z3, dCo, hC, HC, dCs, 0.0, hx);
+                }
+                continue;
+            }
+            // owner of an internal face or of a cyclic face
+            const bool mine = kind == KIND_CYC || !(s < -VSMALL);
+            const bool needO = kind == KIND_CYC ? !(s > VSMALL) : (mine && !(s >= VSMALL));
+            double gO = 0.0, hO = 0.0;
+            double GO[3] = {0, 0, 0}, HO[3] = {0, 0, 0};
+            if (__any(needO)) {
+                const int o = S.other;
+                gO = ld(p.gb_in + (size_t)o * p.nvp, k);
+                if (H) hO = ld(p.hb_in + (size_t)o * p.nvp, k);
+                grad_at<D, H, NS>(p, o, k, gO, hO, GO, HO);
+            }
+            if (mine) {
+                fg[k] = (TS)face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx);
+                if (H) fh[k] = (TS)face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx);
+            }
+        }
+    }
+}
+
+// k_face_moments: face-wise, over one velocity block: the S6 moment sums and the wall outGoing
+// sum of the stored face values (the moment part of k_face, same arithmetic and block split).
+template <int D, bool H, typename TS>
+__global__ __launch_bounds__(MAXNT) void k_face_moments(P<TS> p)
+{
+    constexpr int NM = NMom<D>::v + 1;  // + wall outGoing
+    __shared__ double red[MAXWAVES * NM];
+    __shared__ double tot[NM];
+    __shared__ double vt[VT_SIZE];
+    int pos, b;
+    if (!xcd_block(p.nf, p.nb, pos, b)) return;
+    stage_vt(p, vt);
+    __syncthreads();
+    const int f = cptr(p.forder)[pos];
+    const CONST_AS FaceX &fx = cptr(p.fx)[f];
+    const bool wall = fx.kind == KIND_WALL;
+    const double S[3] = {fx.S[0], fx.S[1], fx.S[2]};
+    double Uh[3];
+    if (wall) {
+        const CONST_AS WallC &W = cptr(p.wall)[fx.bidx];
+        Uh[0] = W.U[0]; Uh[1] = W.U[1]; Uh[2] = W.U[2];
+    } else {
+        Uh[0] = cptr(p.fU)[3 * f]; Uh[1] = cptr(p.fU)[3 * f + 1]; Uh[2] = cptr(p.fU)[3 * f + 2];
+    }
+    const TS *__restrict__ fg = p.fvg + (size_t)f * p.nvp;
+    const TS *__restrict__ fh = H ? p.fvh + (size_t)f * p.nvp : nullptr;
+    double acc[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) acc[m] = 0.0;
+    const unsigned k0 = (unsigned)b * p.vb + threadIdx.x;
+    // FM_UNR iterations per pass: their loads are issued before any is used; the moments are
+    // accumulated in iteration order, as in k_face
+    for (int it0 = 0; it0 < p.nib; it0 += FM_UNR) {
+        double gv[FM_UNR], hv[FM_UNR];
+#pragma unroll
+        for (int u = 0; u < FM_UNR; ++u) {
+            const unsigned k = k0 + (it0 + u < p.nib ? it0 + u : it0) * p.nt;
+            gv[u] = ld(fg, k);
+            hv[u] = H ? ld(fh, k) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < FM_UNR; ++u) {
+            if (it0 + u >= p.nib) break;
+            const unsigned k = k0 + (it0 + u) * p.nt;
+            const Vel v = load_vel<D>(p, k, vt);
+            const double s = dotnc(v.x, v.y, v.z, S);
+            if (!wall || s > 0) {
+                const double gf = gv[u] + row_off<D>(p, v, 0, vt);
+                const double hf = H ? hv[u] + row_off<D>(p, v, 1, vt) : 0.0;
+                acc_moments<D, H>(acc, gf, hf, v, Uh);
+                if (wall) acc[NM - 1] = fma(v.w * s, gf, acc[NM - 1]);
+            }
+        }
+    }
+    block_sum<NM>(acc, red, tot);
+    if ((int)threadIdx.x < NM) p.fpart[((size_t)f * p.nb + b) * NM + threadIdx.x] = tot[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------------------------
 // k_face_fin: one thread per face.  Sums the block partials in block order, then S3 wall density
 // rho_w = outGoing / |inComingByRho| (calculatedMaxwellFvPatchField.C:139-164) with the moments of
 // the S4 incoming Maxwellian added, the S6 face macros (fvDVM.C:456-522), the S7 equilibrium
@@ -797,7 +951,21 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
         // S8 (discreteVelocity.C:937-975): -1/3 g~ + 4/3 gbar, then the faces in face order
         double ag = -1.0 / 3 * gt + 4.0 / 3 * gCv;
         double ah = H ? (-1.0 / 3 * ht + 4.0 / 3 * hCv) : 0.0;
-        for (int j = 0; j < ns; ++j) {
+        // face-value mode: the values of all faces are requested before the first is used
+        constexpr bool PRE = FV && KC_PRELOAD;
+        double pg[PRE ? NS : 1], pgh[PRE ? NS : 1];
+        if (PRE) {
+#pragma unroll
+            for (int j = 0; j < (PRE ? NS : 1); ++j) {
+                const int fj = SL[j < ns ? j : 0].face;
+                pg[j] = ld(p.fvg + (size_t)fj * p.nvp, k);
+                pgh[j] = H ? ld(p.fvh + (size_t)fj * p.nvp, k) : 0.0;
+            }
+        }
+        constexpr int FUNR = PRE ? NS : 1;
+#pragma unroll FUNR
+        for (int j = 0; j < (PRE ? NS : ns); ++j) {
+            if (PRE && j >= ns) break;
             const CONST_AS Slot &S = SL[j];
             const int kind = S.kind;
             const double Sv[3] = {S.S[0], S.S[1], S.S[2]};
@@ -828,8 +996,11 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
                 gf = fe.rho_w * m;
                 if (H) hf = gf * (p.R * fe.pad[0]) * p.kc;
             } else {
-                if (FV) {
-                    // the value k_face rebuilt for this face (same arithmetic, stored)
+                if (PRE) {
+                    gf = pg[j] + og;
+                    if (H) hf = pgh[j] + oh;
+                } else if (FV) {
+                    // the stored face value (same arithmetic as the rebuilding branches)
                     gf = ld(p.fvg + (size_t)S.face * p.nvp, k) + og;
                     if (H) hf = ld(p.fvh + (size_t)S.face * p.nvp, k) + oh;
                 } else if (nei) {
@@ -1164,7 +1335,8 @@ struct dugks_engine {
     double *fpart = nullptr, *cpart = nullptr;
     void *gb[2] = {nullptr, nullptr}, *hb[2] = {nullptr, nullptr};
     void *gB[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
-    int face_values = 0;  // face-value mode: k_face stores the face values, k_cell reads them
+    int face_values = 0;  // face-value mode: the face values are stored, k_cell reads them
+    int face_pass_legacy = 0;  // face-value mode with the single face pass (k_face storing)
     void *fvg = nullptr, *fvh = nullptr;
     int cur = 0;         // buffer holding the current state
     int raw = 1;         // state holds g~ (1) or gbar prepared with dt_prep (0)
@@ -1267,7 +1439,10 @@ size_t cell_lds(const dugks_engine *h) { return sizeof(double) * ((4 + 2 * MAXS)
 
 dim3 block_grid(int n, int nb) { return dim3((unsigned)(((n + NXCD - 1) / NXCD) * NXCD), (unsigned)nb); }
 
-constexpr int NKERN = 5;  // k_face, k_face_fin, k_cell, k_cell_fin, k_cell_s1
+// Kernel slots of a step (profiling and tuning events): face pass 1 (k_face, or k_face_values in
+// face-value mode), face pass 2 (k_face_moments in face-value mode; empty otherwise), k_face_fin,
+// k_cell, k_cell_fin, k_cell_s1.
+constexpr int NKERN = 6;
 constexpr int S1_NT = 256;  // workgroup size of k_cell_s1
 
 template <int D, bool H, int NS, typename TS, bool FV>
@@ -1275,16 +1450,23 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
 {
     P<TS> p = params<TS>(h);
     if (ev) (void)hipEventRecord(ev[0], h->stream);
-    hipLaunchKernelGGL((k_face<D, H, NS, TS, FV>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
-    if (ev) (void)hipEventRecord(ev[1], h->stream);
-    hipLaunchKernelGGL((k_face_fin<D, TS>), dim3((h->nf + 255) / 256), dim3(256), 0, h->stream, p);
+    if (FV && !h->face_pass_legacy) {
+        hipLaunchKernelGGL((k_face_values<D, H, NS, TS>), block_grid(h->nc, h->nvb), dim3(h->nt), 0, h->stream, p);
+        if (ev) (void)hipEventRecord(ev[1], h->stream);
+        hipLaunchKernelGGL((k_face_moments<D, H, TS>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
+    } else {
+        hipLaunchKernelGGL((k_face<D, H, NS, TS, FV>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
+        if (ev) (void)hipEventRecord(ev[1], h->stream);
+    }
     if (ev) (void)hipEventRecord(ev[2], h->stream);
-    hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
+    hipLaunchKernelGGL((k_face_fin<D, TS>), dim3((h->nf + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[3], h->stream);
-    hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
+    hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
     if (ev) (void)hipEventRecord(ev[4], h->stream);
-    hipLaunchKernelGGL((k_cell_s1<D, H, TS>), dim3(h->nc), dim3(S1_NT), 0, h->stream, p);
+    hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[5], h->stream);
+    hipLaunchKernelGGL((k_cell_s1<D, H, TS>), dim3(h->nc), dim3(S1_NT), 0, h->stream, p);
+    if (ev) (void)hipEventRecord(ev[6], h->stream);
 }
 
 template <int D, bool H, int NS, typename TS>
@@ -2058,8 +2240,8 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
         HIPCHK(h, hipGetLastError());
         HIPCHK(h, hipEventSynchronize(h->tev[NKERN]));
         float tf = 0.f, tc = 0.f;
-        HIPCHK(h, hipEventElapsedTime(&tf, h->tev[0], h->tev[1]));
-        HIPCHK(h, hipEventElapsedTime(&tc, h->tev[2], h->tev[3]));
+        HIPCHK(h, hipEventElapsedTime(&tf, h->tev[0], h->tev[2]));
+        HIPCHK(h, hipEventElapsedTime(&tc, h->tev[3], h->tev[4]));
         if (i >= 0 && i < nfc && (h->tune_best_face == 0.0 || tf < h->tune_best_face)) {
             h->tune_best_face = tf;
             h->tune_face = i;
@@ -2169,6 +2351,8 @@ int dugks_create(const dugks_case *cs, dugks_engine **out)
         const size_t need = (size_t)h->nf * h->nvp * ts_size(h) * (h->has_h ? 2 : 1);
         const bool fits = hipMemGetInfo(&fr, &tot) == hipSuccess && need <= fr / 2;
         if (fv ? std::atoi(fv) != 0 : fits) r = dugks_set_face_values(h, 1);
+        const char *fl = std::getenv("DUGKS_FACE_PASS_LEGACY");
+        h->face_pass_legacy = fl && std::atoi(fl) ? 1 : 0;
     }
     if (r) {
         // the handle is returned without device resources so that dugks_last_error can

@@ -183,14 +183,17 @@ class Engine:
         """HIP-event timing of the two step kernels on the engine stream."""
         self._chk(load_library().dugks_set_profiling(self._h, 1 if on else 0))
 
-    KERNELS = ("k_face", "k_face_fin", "k_cell", "k_cell_fin", "k_cell_s1")
+    # kernel slots of dugks_get_profile, in face-value mode and otherwise
+    KERNELS_FV = ("k_face_values", "k_face_moments", "k_face_fin", "k_cell", "k_cell_fin", "k_cell_s1")
+    KERNELS = ("k_face", None, "k_face_fin", "k_cell", "k_cell_fin", "k_cell_s1")
 
     def profile(self):
         """({kernel: accumulated ms}, steps) since set_profiling(True)."""
         ms = np.zeros(len(self.KERNELS))
         n = C.c_int64()
         self._chk(load_library().dugks_get_profile(self._h, dptr(ms), C.byref(n)))
-        return dict(zip(self.KERNELS, ms.tolist())), n.value
+        names = self.KERNELS_FV if self.face_values() else self.KERNELS
+        return {k: v for k, v in zip(names, ms.tolist()) if k}, n.value
 
     def set_graphs(self, on: bool = True):
         """Replay captured two-step HIP graphs for multi-step calls (default on)."""

@@ -184,11 +184,13 @@ int dugks_last_step_ms(dugks_engine* h, double* ms);
 
 /* Per-kernel timing (no reference counterpart; for measurement).  With
  * profiling on, every step records HIP events on the engine stream around its
- * five kernels; dugks_get_profile returns in ms[5] the accumulated device time
- * of k_face (S2 reconstruction + S6 face-moment sums), k_face_fin (S3 wall
- * density, face macros), k_cell (S7 relaxation, S8 update, S9 moment sums),
- * k_cell_fin (S9 cell macros) and k_cell_s1 (S1 of the next step), and the
- * number of steps since profiling was switched on. */
+ * kernels; dugks_get_profile returns in ms[6] the accumulated device time of
+ * face pass 1 (k_face: S2 reconstruction + S6 face-moment sums; in face-value
+ * mode k_face_values: S2 face values, cell-wise), face pass 2 (k_face_moments:
+ * S6 face-moment sums of the stored values; 0 outside face-value mode),
+ * k_face_fin (S3 wall density, face macros), k_cell (S7 relaxation, S8 update,
+ * S9 moment sums), k_cell_fin (S9 cell macros) and k_cell_s1 (S1 of the next
+ * step), and the number of steps since profiling was switched on. */
 int dugks_set_profiling(dugks_engine* h, int32_t on);
 int dugks_get_profile(dugks_engine* h, double* ms, int64_t* steps);
 
