@@ -58,6 +58,7 @@ constexpr int KIND_OWN = 0;   // internal face, this cell is the owner
 constexpr int KIND_NEI = 1;   // internal face, this cell is the neighbour
 constexpr int KIND_WALL = 2;  // Maxwell wall (fixedGradient gBarPvol, maxwellWall gSurf)
 constexpr int KIND_CYC = 3;   // cyclic (coupled) patch face
+constexpr int KIND_SYM = 4;   // symmetry plane (symmetryMod rho, DVMsymmetry gSurf, fixedGradient gBarPvol)
 constexpr int MAXS = 8;       // faces per cell handled by k_cell
 constexpr int NAXMAX = 192;   // per-axis exp table entries (3 axes x 64 points)
 constexpr int MAXNT = 512;     // workgroup size cap (256 VGPRs per lane available)
@@ -93,9 +94,9 @@ struct alignas(16) Slot {
     double L[3];    // least-squares vector of this cell for this face
     double S[3];    // face area vector
     double dCs[3];  // Cf - C(this cell)
-    double dCo[3];  // Cf - C(other cell) | wall: unit normal Sf/|Sf|
-    double aux;     // wall: 1/deltaCoeffs | cyclic: patch weight w
-    int32_t face, other, kind, bidx;
+    double dCo[3];  // Cf - C(other cell) | wall, symmetry: unit normal Sf/|Sf|
+    double aux;     // wall, symmetry: 1/deltaCoeffs | cyclic: patch weight w
+    int32_t face, other, kind, bidx;  // other: the other cell | symmetry: mirror axis
     double pad;
 };
 static_assert(sizeof(Slot) == 128, "slot size");
@@ -166,6 +167,7 @@ struct P {
     int ntc, nibc, nbc, vbc;  // k_cell velocity blocks: nbc blocks of vbc = ntc * nibc velocities
     double R, Pr, omega, Tref, kc;  // kc = K + 3 - D
     const uint32_t *__restrict__ vidx;  // packed axis indices of each (sign-sorted) velocity
+    const uint32_t *__restrict__ mir;   // [axis][velocity]: the velocity mirrored across that axis
     const double *__restrict__ axw;     // per-axis quadrature weights, same layout as axis
     const double *__restrict__ axis;
     const int *__restrict__ cs;
@@ -522,7 +524,7 @@ DEVI double face_value(int kind, double s, double gP, const double *GP, const do
 DEVI bool need_P(int kind, double s) { return kind != KIND_OWN || !(s < -VSMALL); }
 DEVI bool need_N(int kind, double s)
 {
-    if (kind == KIND_WALL) return false;
+    if (kind == KIND_WALL || kind == KIND_SYM) return false;
     if (kind == KIND_CYC) return !(s > VSMALL);
     return !(s >= VSMALL);
 }
@@ -684,7 +686,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
             const double dCo[3] = {S.dCo[0], S.dCo[1], S.dCo[2]};
             TS *__restrict__ fg = p.fvg + (size_t)S.face * p.nvp;
             TS *__restrict__ fh = H ? p.fvh + (size_t)S.face * p.nvp : nullptr;
-            if (kind == KIND_WALL) {
+            if (kind == KIND_WALL || kind == KIND_SYM) {
                 const size_t bi = (size_t)S.bidx * p.nvp + k;
                 double ng = GC[0] * dCo[0] + GC[1] * dCo[1];
                 if (D == 3) ng += GC[2] * dCo[2];
@@ -697,6 +699,21 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                 if (s > 0) {
                     fg[k] = (TS)face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx);
                     if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx);
+                }
+                if (kind == KIND_SYM && __any(!(s > 0))) {
+                    // incoming at a symmetry plane (discreteVelocity.C:746-790): the outgoing
+                    // value of the velocity mirrored across the plane, rebuilt at its index
+                    const unsigned km = p.mir[(size_t)S.other * p.nvp + k];
+                    const Vel vm = load_vel<D>(p, km, vt);
+                    const double hxm[3] = {0.5 * vm.x * dt, 0.5 * vm.y * dt, 0.5 * vm.z * dt};
+                    const double gm = ld(rowC, km);
+                    const double hm = H ? ld(hrowC, km) : 0.0;
+                    double GM[3] = {0, 0, 0}, HM[3] = {0, 0, 0};
+                    grad_at<D, H, NS>(p, c, km, gm, hm, GM, HM);
+                    if (!(s > 0)) {
+                        fg[k] = (TS)face_value<D>(KIND_WALL, s, gm, GM, dCs, 0.0, GM, dCo, 0.0, hxm);
+                        if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hm, HM, dCs, 0.0, HM, dCo, 0.0, hxm);
+                    }
                 }
                 continue;
             }
@@ -973,7 +990,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
             // roles: this cell is the face owner P except on KIND_NEI faces
             const bool nei = kind == KIND_NEI;
             bool needO;
-            if (kind == KIND_WALL) needO = false;
+            if (kind == KIND_WALL || kind == KIND_SYM) needO = false;
             else if (nei) needO = need_P(KIND_OWN, s);
             else needO = need_N(kind, s);
             double gO = 0.0, hO = 0.0;
@@ -1010,16 +1027,19 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
                     gf = face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx) + og;
                     if (H) hf = face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx) + oh;
                 }
-                // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium
-                EqC e;
-                e.U[0] = fe.U[0]; e.U[1] = fe.U[1]; e.U[2] = fe.U[2];
-                e.invRT = fe.invRT; e.pre = fe.pre; e.RT = fe.RT;
-                e.qc[0] = fe.qc[0]; e.qc[1] = fe.qc[1]; e.qc[2] = fe.qc[2];
-                double gq, hq;
-                eq_eval<D, H>(e, tabF + j * nax, p.nxi, v, p.kc, p.K, gq, hq);
-                const double r = fe.r;
-                gf = (1.0 - r) * gf + r * gq;
-                if (H) hf = (1.0 - r) * hf + r * hq;
+                // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium; on a
+                // symmetry plane only the outgoing values (the patch's own operators are no-ops)
+                if (kind != KIND_SYM || s > 0) {
+                    EqC e;
+                    e.U[0] = fe.U[0]; e.U[1] = fe.U[1]; e.U[2] = fe.U[2];
+                    e.invRT = fe.invRT; e.pre = fe.pre; e.RT = fe.RT;
+                    e.qc[0] = fe.qc[0]; e.qc[1] = fe.qc[1]; e.qc[2] = fe.qc[2];
+                    double gq, hq;
+                    eq_eval<D, H>(e, tabF + j * nax, p.nxi, v, p.kc, p.K, gq, hq);
+                    const double r = fe.r;
+                    gf = (1.0 - r) * gf + r * gq;
+                    if (H) hf = (1.0 - r) * hf + r * hq;
+                }
             }
             const double tg = div_rn(s * gf * dt, Vc, invV);
             if (nei) ag += tg;
@@ -1318,7 +1338,8 @@ struct dugks_engine {
     std::vector<int> ncell_slots;
 
     // device buffers
-    uint32_t *vidx = nullptr;
+    uint32_t *vidx = nullptr, *mir = nullptr;
+    int has_sym = 0;     // symmetry patches present (they need face-value mode)
     double *axw = nullptr;
     double *axis = nullptr;
     int *cs = nullptr, *ccase = nullptr, *flag = nullptr, *wfslot = nullptr;
@@ -1414,7 +1435,7 @@ P<TS> params(dugks_engine *h)
     p.nib = h->nib; p.nb = h->nvb; p.vb = h->vb;
     p.ntc = h->ntc; p.nibc = h->nibc; p.nbc = h->nvbc; p.vbc = h->ntc * h->nibc;
     p.R = h->R; p.Pr = h->Pr; p.omega = h->omega; p.Tref = h->Tref; p.kc = (double)(h->K + 3 - h->D);
-    p.vidx = h->vidx; p.axw = h->axw; p.moff = h->moff;
+    p.vidx = h->vidx; p.mir = h->mir; p.axw = h->axw; p.moff = h->moff;
     p.axis = h->axis; p.cs = h->cs; p.slots = h->slots; p.cV = h->cV; p.ccase = h->ccase;
     p.dtc = h->dtc; p.muc = h->muc;
     p.crho = h->crho; p.cU = h->cU; p.cT = h->cT; p.cq = h->cq; p.ctau = h->ctau;
@@ -1704,7 +1725,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     const int D = h->D;
 
     // ---- boundary bookkeeping ----
-    std::vector<int> bkind(nb, 0), bpartner(nb, -1), bpatch(nb, -1), bwall(nb, -1);
+    std::vector<int> bkind(nb, 0), bpartner(nb, -1), bpatch(nb, -1), bwall(nb, -1), bsym(nb, -1);
     std::vector<double> Ub(3 * (size_t)nb, 0.0), Tb(nb, 0.0);
     h->patch_type.assign(cs->n_patches, 0);
     h->patch_start.assign(cs->n_patches, 0);
@@ -1715,12 +1736,26 @@ int build(dugks_engine *h, const dugks_case *cs)
         const dugks_patch &pp = cs->patches[pi];
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf)
             return fail(h, DUGKS_ERR_INVALID, "patch " + std::to_string(pi) + " face range outside the boundary");
-        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC)
+        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_SYMMETRY)
             return fail(h, DUGKS_ERR_UNSUPPORTED, "patch " + std::to_string(pi) + ": unsupported type");
         h->patch_type[pi] = pp.type; h->patch_start[pi] = pp.start; h->patch_size[pi] = pp.size;
+        int sym_axis = -1;
         if (pp.type == DUGKS_PATCH_WALL) {
             if (!pp.U || !pp.T) return fail(h, DUGKS_ERR_INVALID, "wall patch without U/T");
             h->patch_wall0[pi] = nw;
+        } else if (pp.type == DUGKS_PATCH_SYMMETRY) {
+            // the mirror axis comes from the patch's first face normal (discreteVelocity.C:766-768)
+            h->has_sym = 1;
+            if (pp.size) {
+                const double *S0 = cs->Sf + 3 * (size_t)pp.start;
+                const double m = mag3(S0);
+                sym_axis = 0;
+                for (int d = 1; d < 3; ++d)
+                    if (std::fabs(S0[d]) > std::fabs(S0[sym_axis])) sym_axis = d;
+                if (!(m > 0) || !(std::fabs(S0[sym_axis]) / m > 1.0 - 1e-9) || sym_axis >= cs->dim)
+                    return fail(h, DUGKS_ERR_UNSUPPORTED,
+                                "symmetry patch " + std::to_string(pi) + " must be normal to a solution axis");
+            }
         } else {
             if (pp.neighbour_patch < 0 || pp.neighbour_patch >= cs->n_patches ||
                 cs->patches[pp.neighbour_patch].size != pp.size ||
@@ -1736,6 +1771,11 @@ int build(dugks_engine *h, const dugks_case *cs)
                 for (int d = 0; d < 3; ++d) Ub[3 * b + d] = pp.U[3 * i + d];
                 Tb[b] = pp.T[i];
                 if (!(Tb[b] > 0)) return fail(h, DUGKS_ERR_INVALID, "wall temperature must be > 0");
+                bwall[b] = nw++;
+            } else if (pp.type == DUGKS_PATCH_SYMMETRY) {
+                // a fixedGradient row like a wall's; the wall constants are placeholders
+                Tb[b] = cs->T_ref;
+                bsym[b] = sym_axis;
                 bwall[b] = nw++;
             } else {
                 bpartner[b] = cs->patches[pp.neighbour_patch].start - nfi + i;
@@ -1883,9 +1923,10 @@ int build(dugks_engine *h, const dugks_case *cs)
             } else {
                 const int b = f - nfi;
                 for (int d = 0; d < 3; ++d) s.L[d] = pvec[3 * f + d];
-                if (bkind[b] == DUGKS_PATCH_WALL) {
-                    s.kind = KIND_WALL;
+                if (bkind[b] == DUGKS_PATCH_WALL || bkind[b] == DUGKS_PATCH_SYMMETRY) {
+                    s.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : KIND_SYM;
                     s.bidx = bwall[b];
+                    if (s.kind == KIND_SYM) s.other = bsym[b];
                     for (int d = 0; d < 3; ++d) s.dCo[d] = cs->Sf[3 * f + d] / magSf[f];
                     const double *dl = cs->delta + 3 * b;
                     s.aux = 1.0 / (1.0 / std::sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]));
@@ -1919,8 +1960,8 @@ int build(dugks_engine *h, const dugks_case *cs)
             for (int d = 0; d < 3; ++d) x.dN[d] = cs->Cf[3 * f + d] - Cc[3 * x.N + d];
         } else {
             const int b = f - nfi;
-            if (bkind[b] == DUGKS_PATCH_WALL) {
-                x.kind = KIND_WALL;
+            if (bkind[b] == DUGKS_PATCH_WALL || bkind[b] == DUGKS_PATCH_SYMMETRY) {
+                x.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : KIND_SYM;
                 x.bidx = bwall[b];
                 for (int d = 0; d < 3; ++d) x.dN[d] = cs->Sf[3 * f + d] / magSf[f];
             } else {
@@ -1990,7 +2031,7 @@ int build(dugks_engine *h, const dugks_case *cs)
             if (j >= ns) continue;
             const Slot &q = sl[cnt[c] + j];
             for (int d = 0; d < 3; ++d) t.L[j][d] = q.L[d];
-            if (q.kind == KIND_WALL) {
+            if (q.kind == KIND_WALL || q.kind == KIND_SYM) {
                 t.off[j] = (int64_t)(nc + q.bidx) * h->nvp;
                 t.a[j] = q.aux;
                 t.wmask |= 1 << j;
@@ -2007,6 +2048,22 @@ int build(dugks_engine *h, const dugks_case *cs)
         const int r = order[k];
         vidx[k] = (uint32_t)rid[r];
         h->perm[k] = r;
+    }
+    // mirrored velocities (symmXtgID / symmYtgID / symmZtgID, fvDVM.C:152-166, 183-195) in device order
+    std::vector<uint32_t> mir(3 * (size_t)nvp);
+    {
+        std::vector<int> inv(nv);
+        for (int k = 0; k < nv; ++k) inv[h->perm[k]] = k;
+        for (int k = 0; k < nvp; ++k)
+            for (int a = 0; a < 3; ++a) mir[(size_t)a * nvp + k] = (uint32_t)k;
+        for (int k = 0; k < nv; ++k) {
+            const int r = h->perm[k];
+            const int ix = r % nxi, iy = (r / nxi) % nxi, iz = D == 3 ? r / (nxi * nxi) : 0;
+            const int nz = D == 3 ? nxi : 1;
+            mir[k] = (uint32_t)inv[(iz * nxi + iy) * nxi + (nxi - 1 - ix)];
+            mir[(size_t)nvp + k] = (uint32_t)inv[(iz * nxi + (nxi - 1 - iy)) * nxi + ix];
+            mir[2 * (size_t)nvp + k] = (uint32_t)inv[((nz - 1 - iz) * nxi + iy) * nxi + ix];
+        }
     }
     std::vector<double> axis(h->nax), axw(h->nax);
     for (int a = 0; a < D; ++a)
@@ -2069,7 +2126,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
     int r = 0;
-    if ((r = dalloc(h, &h->vidx, nvp)) || (r = dalloc(h, &h->axw, h->nax)) || (r = dalloc(h, &h->moff, (size_t)h->nax + 2)) ||
+    if ((r = dalloc(h, &h->vidx, nvp)) || (r = dalloc(h, &h->mir, 3 * (size_t)nvp)) || (r = dalloc(h, &h->axw, h->nax)) || (r = dalloc(h, &h->moff, (size_t)h->nax + 2)) ||
         (r = dalloc(h, &h->axis, h->nax)) || (r = dalloc(h, &h->cs, nc + 1)) || (r = dalloc(h, &h->slots, nslot)) ||
         (r = dalloc(h, &h->cV, nc)) || (r = dalloc(h, &h->ccase, nc)) || (r = dalloc(h, &h->dtc, h->n_cases)) ||
         (r = dalloc(h, &h->dtp, h->n_cases)) || (r = dalloc(h, &h->muc, h->n_cases)) ||
@@ -2094,7 +2151,7 @@ int build(dugks_engine *h, const dugks_case *cs)
             h->hB[i] = (char *)h->hb[i] + row;
         }
     }
-    if ((r = upload(h, h->vidx, vidx.data(), nvp)) || (r = upload(h, h->axw, axw.data(), h->nax)) ||
+    if ((r = upload(h, h->vidx, vidx.data(), nvp)) || (r = upload(h, h->mir, mir.data(), 3 * (size_t)nvp)) || (r = upload(h, h->axw, axw.data(), h->nax)) ||
         (r = upload(h, h->axis, axis.data(), h->nax)) || (r = upload(h, h->cs, cnt.data(), nc + 1)) ||
         (r = upload(h, h->slots, sl.data(), nslot)) || (r = upload(h, h->cV, cs->V, nc)) ||
         (r = upload(h, h->ccase, ccase.data(), nc)) || (r = upload(h, h->muc, muc.data(), h->n_cases)) ||
@@ -2127,7 +2184,7 @@ void release(dugks_engine *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->vidx); dfree(h->axw); dfree(h->moff); dfree(h->axis);
+    dfree(h->vidx); dfree(h->mir); dfree(h->axw); dfree(h->moff); dfree(h->axis);
     dfree(h->cs); dfree(h->ccase); dfree(h->flag); dfree(h->wfslot); dfree(h->slots); dfree(h->cV);
     dfree(h->dtc); dfree(h->dtp); dfree(h->muc); dfree(h->crho); dfree(h->cU); dfree(h->cT); dfree(h->cq);
     dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->feq);
@@ -2350,9 +2407,9 @@ int dugks_create(const dugks_case *cs, dugks_engine **out)
         size_t fr = 0, tot = 0;
         const size_t need = (size_t)h->nf * h->nvp * ts_size(h) * (h->has_h ? 2 : 1);
         const bool fits = hipMemGetInfo(&fr, &tot) == hipSuccess && need <= fr / 2;
-        if (fv ? std::atoi(fv) != 0 : fits) r = dugks_set_face_values(h, 1);
+        if (h->has_sym || (fv ? std::atoi(fv) != 0 : fits)) r = dugks_set_face_values(h, 1);
         const char *fl = std::getenv("DUGKS_FACE_PASS_LEGACY");
-        h->face_pass_legacy = fl && std::atoi(fl) ? 1 : 0;
+        h->face_pass_legacy = fl && std::atoi(fl) && !h->has_sym ? 1 : 0;
     }
     if (r) {
         // the handle is returned without device resources so that dugks_last_error can
@@ -2403,6 +2460,8 @@ int dugks_init_equilibrium(dugks_engine *h, const double *rho, const double *U, 
                 const int f = s.face;
                 if (s.kind == KIND_WALL) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];
+                } else if (s.kind == KIND_SYM) {
+                    for (int d = 0; d < 3; ++d) fU[3 * f + d] = U[3 * c + d];   // U zeroGradient there
                 } else {
                     // linear interpolation w U_P + (1 - w) U_N (cyclic: patch weight)
                     const double wf = h->face_w[f];
@@ -2704,6 +2763,7 @@ int dugks_set_graphs(dugks_engine *h, int32_t on)
 int dugks_set_face_values(dugks_engine *h, int32_t on)
 {
     if (!h || !h->stream) return h ? fail(h, DUGKS_ERR_STATE, "no device resources") : DUGKS_ERR_INVALID;
+    if (!on && h->has_sym) return fail(h, DUGKS_ERR_UNSUPPORTED, "symmetry patches need face-value mode");
     HIPCHK(h, hipSetDevice(h->device));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (on && !h->fvg) {

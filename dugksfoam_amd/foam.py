@@ -5,8 +5,9 @@ read_case(path) builds a Case (dugksfoam_amd.cases.Case) from the files a dugksF
   constant/polyMesh/{points,faces,owner,neighbour,boundary}   the mesh (polyMesh)
   0/{rho,U,T}                     initial fields (uniform or nonuniform internalField) and the
                                   boundary types: rho calculatedMaxwell + U, T fixedValue is a
-                                  diffuse Maxwell wall (doc/usage.tex:61-75); empty patches are
-                                  omitted (fvPatch size 0); cyclic patches are coupled pairs
+                                  diffuse Maxwell wall (doc/usage.tex:61-75); rho symmetryMod is a
+                                  mirror plane; empty patches are omitted (fvPatch size 0);
+                                  cyclic patches are coupled pairs
   constant/DVMProperties          fvDVMparas {nDV, xiMax, xiMin}, gasProperties
                                   {R, omega, Tref, muRef, Pr, KInner = 0} (fvDVM.C:913-933)
   constant/Xis, constant/weights  the 1-D discrete-velocity set (fvDVM.C:56-117)
@@ -16,7 +17,8 @@ Geometry follows OpenFOAM's primitiveMesh: face centres and area vectors by tria
 decomposition about the point average of the face; cell centres and volumes by pyramid
 decomposition about the average of the cell's face centres.  Interpolation weights and
 boundary deltas are those of cases._assemble (surfaceInterpolation::makeWeights, the
-patch-normal fvPatch::delta).  Patch types other than walls, cyclics and empty are rejected.
+patch-normal fvPatch::delta).  Patch types other than walls, symmetryMod planes, cyclics and
+empty are rejected.
 """
 from __future__ import annotations
 
@@ -295,6 +297,9 @@ def read_case(path: str, dt: float = None) -> cases.Case:
         if b["type"] == "cyclic":
             plist.append(cases.Patch(b["name"], cases.PATCH_CYCLIC, owner[sl], Sf_all[sl], Cf_all[sl],
                                      neighbour_patch=index[b["neighbourPatch"]]))
+        elif rtype == "symmetryMod":
+            # DVMsymmetry DF patch (bcMap, discreteVelocity.C:279): mirror plane
+            plist.append(cases.Patch(b["name"], cases.PATCH_SYMMETRY, owner[sl], Sf_all[sl], Cf_all[sl]))
         elif rtype == "calculatedMaxwell":
             if U_b[b["name"]]["type"] != "fixedValue" or T_b[b["name"]]["type"] != "fixedValue":
                 raise NotImplementedError(f"patch {b['name']}: Maxwell walls need fixedValue U and T")

@@ -48,7 +48,10 @@ enum dugks_status {
  * it in dugksFoam (the DF-side type follows bcMap, discreteVelocity.C:274-281). */
 enum dugks_patch_type {
     DUGKS_PATCH_WALL = 1,      /* rho calculatedMaxwell, U/T fixedValue: diffuse Maxwell wall */
-    DUGKS_PATCH_CYCLIC = 2     /* OpenFOAM cyclic (coupled) patch pair                      */
+    DUGKS_PATCH_CYCLIC = 2,    /* OpenFOAM cyclic (coupled) patch pair                      */
+    DUGKS_PATCH_SYMMETRY = 3   /* rho symmetryMod (gSurf DVMsymmetry): axis-aligned mirror
+                                  plane; incoming values = the mirrored velocity's outgoing
+                                  value (fvDVM.C:375-454, discreteVelocity.C:733-817)       */
 };
 
 /* Storage precision of the resident distributions. */

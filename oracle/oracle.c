@@ -14,6 +14,9 @@
  *      commented-out non-coupled boundary blocks enabled)
  *   S3 updateMaxwellWallRho     fvDVM.C:347-367, calculatedMaxwellFvPatchField.C:139-164
  *   S4 updateGHbarSurfMaxwellWallIn  discreteVelocity.C:693-731
+ *   S5 updateGHbarSurfSymmetryIn     fvDVM.C:375-454, discreteVelocity.C:733-817
+ *      (symmetryMod rho patches, DVMsymmetry gSurf: incoming = the mirrored
+ *      velocity's outgoing face value; gBarPvol fixedGradient as on walls)
  *   S6 updateMacroSurf          fvDVM.C:456-582
  *   S7 updateGHsurf             discreteVelocity.C:819-932
  *   S8 updateGHtildeVol         discreteVelocity.C:934-978
@@ -69,6 +72,10 @@ typedef struct oracle {
     int *bkind;      /* [nb] patch type                      */
     int *bpatch;     /* [nb] patch index                     */
     int *bpartner;   /* [nb] cyclic partner boundary index   */
+    int *bmirror;    /* [nb] symmetry faces: mirror axis     */
+    int has_sym;     /* symmetry patches present             */
+    double *gt0, *ht0, *gradb0_g, *gradb0_h; /* start-of-step copies read by sym_recon */
+    int *mirror;     /* [3][nv] velocity mirrored across axis a (fvDVM.C:152-166, 183-195) */
     double *Ub, *Tb; /* [nb*3], [nb] wall values             */
     double *nhat;    /* [nb*3] Sf/|Sf|                       */
     double *dcoef;   /* [nb] 1/|delta|                       */
@@ -279,9 +286,24 @@ oracle *oracle_create(const dugks_case *cs)
             } else {
                 o->bpartner[b] = -1;
                 for (int d = 0; d < 3; ++d) o->Ub[3*b + d] = pp->U ? pp->U[3*i + d] : 0.0;
-                o->Tb[b] = pp->T[i];
+                o->Tb[b] = pp->T ? pp->T[i] : cs->T_ref;
             }
         }
+    }
+    /* symmetry patches: the mirror axis from the patch's first face normal
+     * (discreteVelocity.C:766-768: round(|n . (symX, symY, symZ)|) picks the mirrored
+     * velocity of the axis the plane is normal to) */
+    o->bmirror = (int *)xmalloc(sizeof(int)*(nb + 1));
+    for (int p = 0; p < cs->n_patches; ++p) {
+        const dugks_patch *pp = cs->patches + p;
+        if (pp->type != DUGKS_PATCH_SYMMETRY || pp->size == 0) continue;
+        const double *S0 = cs->Sf + 3*pp->start;
+        double m = sqrt(magsqr3(S0));
+        int ax = 0;
+        for (int d = 1; d < 3; ++d) if (fabs(S0[d]) > fabs(S0[ax])) ax = d;
+        if (!(fabs(S0[ax])/m > 1.0 - 1e-9) || ax >= cs->dim) { oracle_destroy(o); return NULL; }
+        for (int i = 0; i < pp->size; ++i) o->bmirror[pp->start - nfi + i] = ax;
+        o->has_sym = 1;
     }
     for (int b = 0; b < nb; ++b) {
         if (o->bkind[b] == 0) { oracle_destroy(o); return NULL; }
@@ -323,6 +345,15 @@ oracle *oracle_create(const dugks_case *cs)
                 ++i;
             }
         }
+        /* mirror ids symmXtgID, symmYtgID, symmZtgID (fvDVM.C:152-166, 183-195) */
+        o->mirror = (int *)xmalloc(sizeof(int)*3*nv);
+        for (int k = 0; k < nv; ++k) {
+            int ix = k % n, iy = (k / n) % n, iz = o->dim == 3 ? k / (n*n) : 0;
+            int nz = o->dim == 3 ? n : 1;
+            o->mirror[k] = (iz*n + iy)*n + (n - 1 - ix);
+            o->mirror[nv + k] = (iz*n + (n - 1 - iy))*n + ix;
+            o->mirror[2*nv + k] = ((nz - 1 - iz)*n + iy)*n + ix;
+        }
     }
     o->n_cases = cs->n_cases > 1 ? cs->n_cases : 1;
     o->dt_case = (double *)xmalloc(sizeof(double)*o->n_cases);
@@ -363,7 +394,7 @@ oracle *oracle_create(const dugks_case *cs)
 void oracle_destroy(oracle *o)
 {
     if (!o) return;
-    void *ptrs[] = {o->owner, o->nei, o->C, o->V, o->Sf, o->Cf, o->w, o->delta, o->magSf, o->bkind, o->bpatch,
+    void *ptrs[] = {o->gt0, o->ht0, o->gradb0_g, o->gradb0_h, o->bmirror, o->mirror, o->owner, o->nei, o->C, o->V, o->Sf, o->Cf, o->w, o->delta, o->magSf, o->bkind, o->bpatch,
                     o->bpartner, o->Ub, o->Tb, o->nhat, o->dcoef, o->pstart, o->psize, o->ptype, o->pvec, o->nvec,
                     o->cstart, o->cface, o->xi, o->wv, o->cell_case, o->mu_case, o->dt_case, o->gt, o->ht,
                     o->gradb_g, o->gradb_h, o->rho, o->U, o->T, o->q, o->tau, o->incoming, o->rho_w, o->rho_f,
@@ -429,6 +460,8 @@ int oracle_init_equilibrium(oracle *o, const double *rho, const double *U, const
             if (o->bkind[b] == DUGKS_PATCH_CYCLIC) {
                 int N = o->owner[o->nfi + o->bpartner[b]];
                 o->U_f[3*f + d] = o->w[f]*o->U[3*o->owner[f] + d] + (1 - o->w[f])*o->U[3*N + d];
+            } else if (o->bkind[b] == DUGKS_PATCH_SYMMETRY) {
+                o->U_f[3*f + d] = o->U[3*o->owner[f] + d];   /* U zeroGradient on the patch */
             } else {
                 o->U_f[3*f + d] = o->Ub[3*b + d];
             }
@@ -436,6 +469,71 @@ int oracle_init_equilibrium(oracle *o, const double *rho, const double *U, const
     }
     o->initialised = 1;
     return 0;
+}
+
+/* S1 value of cell c for velocity k (discreteVelocity.C:393-406) from the start-of-step g~ (the
+ * last pass of a step overwrites g~ velocity by velocity, and a mirrored velocity may already
+ * be done) */
+static void gbar_at(const oracle *o, int k, int c, double *g, double *h)
+{
+    const double *xi = o->xi + 3*k;
+    double dt = dt_of_cell(o, c);
+    double rf = 1.5*dt/(2.0*o->tau[c] + dt);
+    double geq, heq;
+    shakhov(o, o->rho[c], o->U + 3*c, o->T[c], o->q + 3*c, xi, &geq, &heq);
+    *g = (1.0 - rf)*o->gt0[(size_t)k*o->nc + c] + rf*geq;
+    *h = (1.0 - rf)*o->ht0[(size_t)k*o->nc + c] + rf*heq;
+}
+
+/* Face value of velocity k at the boundary face b reconstructed from its owner cell, evaluated
+ * locally with the arithmetic of compute_k (S1 of the owner's stencil, the fixedGradient
+ * boundary values, the least-squares gradient in face order, the upwind reconstruction): the
+ * value the mirrored velocity's own pass gives that face, which the symmetry patch copies. */
+static void sym_recon(const oracle *o, int k, int b, double *gout, double *hout)
+{
+    const int nfi = o->nfi, f0 = nfi + b, c = o->owner[f0];
+    const double *xi = o->xi + 3*k;
+    const double *gbk = o->gradb0_g + (size_t)k*o->nb, *hbk = o->gradb0_h + (size_t)k*o->nb;
+    double gc, hc;
+    gbar_at(o, k, c, &gc, &hc);
+    double g[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+    for (int e = o->cstart[c]; e < o->cstart[c + 1]; ++e) {
+        int f = o->cface[e];
+        if (f < nfi) {
+            int own = o->owner[f], nei = o->nei[f];
+            double go, ho, gn, hn;
+            if (own == c) { go = gc; ho = hc; gbar_at(o, k, nei, &gn, &hn); }
+            else { gn = gc; hn = hc; gbar_at(o, k, own, &go, &ho); }
+            double dg = gn - go, dh = hn - ho;
+            if (own == c) {
+                const double *L = o->pvec + 3*f;
+                for (int d = 0; d < 3; ++d) { g[d] += L[d]*dg; h[d] += L[d]*dh; }
+            } else {
+                const double *L = o->nvec + 3*f;
+                for (int d = 0; d < 3; ++d) { g[d] -= L[d]*dg; h[d] -= L[d]*dh; }
+            }
+        } else {
+            int bb = f - nfi;
+            const double *L = o->pvec + 3*f;
+            double dg, dh;
+            if (o->bkind[bb] == DUGKS_PATCH_CYCLIC) {
+                double gn, hn;
+                gbar_at(o, k, o->owner[nfi + o->bpartner[bb]], &gn, &hn);
+                dg = gn - gc;
+                dh = hn - hc;
+            } else {
+                dg = (gc + gbk[bb]/o->dcoef[bb]) - gc;
+                dh = (hc + hbk[bb]/o->dcoef[bb]) - hc;
+            }
+            for (int d = 0; d < 3; ++d) { g[d] += L[d]*dg; h[d] += L[d]*dh; }
+        }
+    }
+    double dt = dt_of_cell(o, c);
+    double hx[3] = {0.5*xi[0]*dt, 0.5*xi[1]*dt, 0.5*xi[2]*dt};
+    const double *Cf = o->Cf + 3*f0;
+    double dO[3] = {Cf[0] - o->C[3*c] - hx[0], Cf[1] - o->C[3*c + 1] - hx[1], Cf[2] - o->C[3*c + 2] - hx[2]};
+    *gout = gc + dot3(g, dO);
+    *hout = hc + dot3(h, dO);
 }
 
 /* One k: S1 (gBarPvol) for all cells, boundary values, S2 gradient and the
@@ -462,7 +560,7 @@ static void compute_k(oracle *o, int k, int want_faces)
     #pragma omp for schedule(static)
     for (int b = 0; b < nb; ++b) {
         int f = nfi + b, P = o->owner[f];
-        if (o->bkind[b] == DUGKS_PATCH_WALL) {
+        if (o->bkind[b] != DUGKS_PATCH_CYCLIC) { /* fixedGradient: walls, symmetry */
             o->phib_g[b] = o->gbar[P] + gbk[b]/o->dcoef[b];
             o->phib_h[b] = o->hbar[P] + hbk[b]/o->dcoef[b];
         } else { /* cyclic */
@@ -510,7 +608,7 @@ static void compute_k(oracle *o, int k, int want_faces)
     #pragma omp for schedule(static)
     for (int b = 0; b < nb; ++b) {
         int f = nfi + b, P = o->owner[f];
-        if (o->bkind[b] == DUGKS_PATCH_WALL) {
+        if (o->bkind[b] != DUGKS_PATCH_CYCLIC) {
             o->newgb_g[b] = dot3(o->grad_g + 3*P, o->nhat + 3*b);
             o->newgb_h[b] = dot3(o->grad_h + 3*P, o->nhat + 3*b);
         } else {
@@ -553,6 +651,16 @@ static void compute_k(oracle *o, int k, int want_faces)
                     o->hs[f] = o->hbar[own] + dot3(o->grad_h + 3*own, dO);
                 } else {
                     o->gs[f] = 0.0; o->hs[f] = 0.0; /* set by S4 */
+                }
+            } else if (o->bkind[b] == DUGKS_PATCH_SYMMETRY) {
+                if (s > 0) {
+                    /* outgoing, discreteVelocity.C:673-689 */
+                    o->gs[f] = o->gbar[own] + dot3(o->grad_g + 3*own, dO);
+                    o->hs[f] = o->hbar[own] + dot3(o->grad_h + 3*own, dO);
+                } else {
+                    /* incoming: the face value of the mirrored velocity (S5,
+                     * discreteVelocity.C:746-790); xi.S == 0 mirrors onto itself */
+                    sym_recon(o, o->mirror[o->bmirror[b]*o->nv + k], b, &o->gs[f], &o->hs[f]);
                 }
             } else { /* cyclic, discreteVelocity.C:628-672 */
                 /* mesh C boundary values are the face centres */
@@ -607,6 +715,15 @@ static int do_step(oracle *o)
     const int nc = o->nc, nfi = o->nfi, nf = o->nf, nb = o->nb, nv = o->nv;
     const int D = o->dim, K = o->K;
     int bad = 0;
+    if (o->has_sym) {
+        const size_t a = (size_t)nv*nc, bsz = (size_t)nv*nb;
+        if (!o->gt0) {
+            o->gt0 = (double *)xmalloc(sizeof(double)*a); o->ht0 = (double *)xmalloc(sizeof(double)*a);
+            o->gradb0_g = (double *)xmalloc(sizeof(double)*bsz); o->gradb0_h = (double *)xmalloc(sizeof(double)*bsz);
+        }
+        memcpy(o->gt0, o->gt, sizeof(double)*a); memcpy(o->ht0, o->ht, sizeof(double)*a);
+        memcpy(o->gradb0_g, o->gradb_g, sizeof(double)*bsz); memcpy(o->gradb0_h, o->gradb_h, sizeof(double)*bsz);
+    }
     int nth = o->nthreads;
     (void)nth;
     #pragma omp parallel num_threads(nth)
@@ -763,7 +880,7 @@ static int do_step(oracle *o)
             double *gbk = o->gradb_g + (size_t)k*nb, *hbk = o->gradb_h + (size_t)k*nb;
             #pragma omp for schedule(static)
             for (int b = 0; b < nb; ++b) {
-                if (o->bkind[b] == DUGKS_PATCH_WALL) { gbk[b] = o->newgb_g[b]; hbk[b] = o->newgb_h[b]; }
+                if (o->bkind[b] != DUGKS_PATCH_CYCLIC) { gbk[b] = o->newgb_g[b]; hbk[b] = o->newgb_h[b]; }
             }
         }
         /* ---- S9: primitive variables, tau (fvDVM.C:694-707) ---- */

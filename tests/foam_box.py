@@ -221,7 +221,7 @@ def write_fixture_case(path, npz_path, controls=None):
            f"    omega {float(z['omega'])!r};\n    Tref {float(z['T_ref'])!r};\n    muRef {float(z['mu_ref'])!r};\n"
            f"    Pr {float(z['Pr'])!r};\n    KInner {int(z['K'])};\n}}\n")
     _write(os.path.join(path, "constant", "DVMProperties"), "dictionary", "DVMProperties", dvm)
-    dv = np.load(os.path.join(os.path.dirname(npz_path), "demo_cavity_dv.npz"))
+    dv = z if "xis" in z.files else np.load(os.path.join(os.path.dirname(npz_path), "demo_cavity_dv.npz"))
     for fname, arr in (("Xis", dv["xis"]), ("weights", dv["weights"])):
         _write(os.path.join(path, "constant", fname), "scalarList", fname,
                f"{len(arr)}\n(\n" + "\n".join(repr(float(v)) for v in arr) + "\n)\n")

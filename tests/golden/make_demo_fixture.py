@@ -1,12 +1,16 @@
 """Stores the reference's demo/cavity case as array fixtures, for machines without the
 reference tree.
 
-    python tests/golden/make_demo_fixture.py [/root/reference/demo/cavity]
+    python tests/golden/make_demo_fixture.py [/root/reference/demo]
 
 writes
-  demo_cavity_dv.npz    constant/Xis and constant/weights (28-point half-range Gauss-Hermite)
-  demo_cavity_case.npz  the polyMesh (points, faces, owner, neighbour, boundary), the 0/ fields
-                        (internal and patch values), DVMProperties and the time controls
+  demo_cavity_dv.npz      demo/cavity's constant/Xis and constant/weights (28-point half-range
+                          Gauss-Hermite)
+  demo_cavity_case.npz    demo/cavity: the polyMesh (points, faces, owner, neighbour, boundary),
+                          the 0/ fields (internal and patch values), DVMProperties, Xis/weights
+                          and the time controls
+  demo_symmetry_case.npz  demo/testSymmetry/symmetry, the same way (half cavity with a
+                          symmetryMod plane)
 
 The files are read with the project's own case reader (dugksfoam_amd.foam); nothing of the
 reference is executed.  tests/foam_box.write_fixture_case writes the arrays back out as an
@@ -23,7 +27,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from dugksfoam_amd import foam  # noqa: E402
 
 
-def main(src):
+def main(src, name):
     pm = os.path.join(src, "constant", "polyMesh")
     points = foam.read_points(os.path.join(pm, "points"))
     faces = foam.read_faces(os.path.join(pm, "faces"))
@@ -49,12 +53,16 @@ def main(src):
     for k in ("xi_max", "R", "omega", "T_ref", "mu_ref", "Pr", "K"):
         out[k] = np.array(getattr(case, k))
     out["nDV"] = np.array(case.xi.size)
+    out["xis"], out["weights"] = case.xi, case.xi_weights
     for k, v in case.meta["controls"].items():
         out[f"ctl_{k}"] = np.array(v)
-    np.savez_compressed(os.path.join(HERE, "demo_cavity_case.npz"), **out)
-    np.savez_compressed(os.path.join(HERE, "demo_cavity_dv.npz"), xis=case.xi, weights=case.xi_weights)
-    print("wrote demo_cavity_case.npz, demo_cavity_dv.npz:", case.n_cells, "cells,", len(faces), "faces")
+    np.savez_compressed(os.path.join(HERE, f"{name}_case.npz"), **out)
+    if name == "demo_cavity":
+        np.savez_compressed(os.path.join(HERE, "demo_cavity_dv.npz"), xis=case.xi, weights=case.xi_weights)
+    print(f"wrote {name}_case.npz:", case.n_cells, "cells,", len(faces), "faces")
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/demo/cavity")
+    demo = sys.argv[1] if len(sys.argv) > 1 else "/root/reference/demo"
+    main(os.path.join(demo, "cavity"), "demo_cavity")
+    main(os.path.join(demo, "testSymmetry", "symmetry"), "demo_symmetry")

@@ -10,7 +10,7 @@ import math
 
 import numpy as np
 
-from dugksfoam_amd.cases import PATCH_CYCLIC, PATCH_WALL
+from dugksfoam_amd.cases import PATCH_CYCLIC, PATCH_SYMMETRY, PATCH_WALL
 
 
 class NumpyDUGKS:
@@ -28,15 +28,27 @@ class NumpyDUGKS:
         self.partner = np.full(self.nb, -1, np.int64)
         self.Ub = np.zeros((self.nb, 3))
         self.Tb = np.ones(self.nb)
+        self.axis = np.zeros(self.nb, np.int64)
         for p in c.patches:
             b = np.arange(p["start"], p["start"] + p["size"]) - nfi
             self.kind[b] = p["type"]
             if p["type"] == PATCH_WALL:
                 self.Ub[b] = p["U"]
                 self.Tb[b] = p["T"]
+            elif p["type"] == PATCH_SYMMETRY:
+                # mirror axis: the normal of the patch's first face (discreteVelocity.C:766-768)
+                self.axis[b] = int(np.argmax(np.abs(c.Sf[p["start"]])))
             else:
                 q = c.patches[p["neighbour_patch"]]
                 self.partner[b] = np.arange(q["start"], q["start"] + q["size"]) - nfi
+        # velocities mirrored across each axis (symmXtgID, symmYtgID, symmZtgID, fvDVM.C:152-195)
+        n = c.xi.shape[0]
+        k = np.arange(self.w.size)
+        ix, iy, iz = k % n, (k // n) % n, (k // (n * n) if self.D == 3 else 0 * k)
+        nz = n if self.D == 3 else 1
+        self.mirror = np.stack([(iz * n + iy) * n + (n - 1 - ix), (iz * n + (n - 1 - iy)) * n + ix,
+                                ((nz - 1 - iz) * n + iy) * n + ix])
+        self.sym = np.nonzero(self.kind == PATCH_SYMMETRY)[0]
         self.wall = np.nonzero(self.kind == PATCH_WALL)[0]
         self.cyc = np.nonzero(self.kind == PATCH_CYCLIC)[0]
         self.bP = self.own[nfi:]
@@ -131,7 +143,8 @@ class NumpyDUGKS:
         wb = c.weights[nfi:]
         phib_g = np.empty((xi.shape[0], self.nb))
         phib_h = np.empty_like(phib_g)
-        wl, cy = self.wall, self.cyc
+        cy = self.cyc
+        wl = np.concatenate([self.wall, self.sym])   # fixedGradient gBarPvol patches
         phib_g[:, wl] = gbar[:, self.bP[wl]] + self.gradB_g[:, wl] / self.dcoef[wl]
         phib_h[:, wl] = hbar[:, self.bP[wl]] + self.gradB_h[:, wl] / self.dcoef[wl]
         lsq_b_g = phib_g.copy()
@@ -171,6 +184,11 @@ class NumpyDUGKS:
         hin = gin * (R * self.Tb)[None, :] * (K + 3 - D)
         gb = np.where(sb > 0, gP, gin)
         hb = np.where(sb > 0, hP, hin)
+        for b in self.sym:
+            # symmetry plane: incoming = the mirrored velocity's outgoing reconstruction
+            m = self.mirror[self.axis[b]]
+            gb[:, b] = np.where(sb[:, b] > 0, gP[:, b], gP[m, b])
+            hb[:, b] = np.where(sb[:, b] > 0, hP[:, b], hP[m, b])
         if cy.size:
             Gb_g = wb[cy][None, :, None] * Gg[:, self.bP[cy]] + (1 - wb[cy])[None, :, None] * Gg[:, self.bN[cy]]
             Gb_h = wb[cy][None, :, None] * Gh[:, self.bP[cy]] + (1 - wb[cy])[None, :, None] * Gh[:, self.bN[cy]]

@@ -94,8 +94,8 @@ def test_unsupported_patches_rejected(tmp_path):
     foam_box.cavity_case(str(tmp_path), c)
     p = tmp_path / "0" / "rho"
     p.write_text(p.read_text().replace("type calculatedMaxwell;\n        value uniform 1.0;",
-                                       "type symmetryMod;\n        value uniform 1.0;", 1))
-    with pytest.raises(NotImplementedError, match="symmetryMod"):
+                                       "type farField;\n        value uniform 1.0;", 1))
+    with pytest.raises(NotImplementedError, match="farField"):
         foam.read_case(str(tmp_path))
 
 
@@ -224,8 +224,14 @@ def test_demo_cavity_mesh_and_properties():
 def test_demo_test_symmetry_cases():
     full = foam.read_case(os.path.join(REF, "testSymmetry", "full"))
     assert full.n_cells == 3600 and [p["size"] for p in full.patches] == [60, 180]
-    with pytest.raises(NotImplementedError, match="symmetryMod"):
-        foam.read_case(os.path.join(REF, "testSymmetry", "symmetry"))
+    sym = foam.read_case(os.path.join(REF, "testSymmetry", "symmetry"))
+    assert sym.n_cells == 1800
+    assert [(p["name"], p["type"], p["size"]) for p in sym.patches] == [
+        ("movingWall", cases.PATCH_WALL, 30), ("fixedWalls", cases.PATCH_WALL, 90),
+        ("symmetryWall", cases.PATCH_SYMMETRY, 60)]
+    s = sym.patches[2]
+    n = sym.Sf[s["start"]:s["start"] + s["size"]]
+    assert np.all(n[:, 0] > 0) and np.abs(n[:, 1:]).max() < 1e-15 * np.abs(n[:, 0]).max()
 
 
 def test_demo_fixture_case(tmp_path):

@@ -1,8 +1,8 @@
 """GPU: dugksFoam case directories through the front-end and the application loop.
 
 The reference's demo/cavity (from its array fixture: 3600 cells, 28^2 velocities, lid 50 m/s,
-rarefied argon-like gas, adjustTimeStep at maxCo 0.8) and a cyclic channel written in
-OpenFOAM format run through dugksfoam_amd.solver.Solver on the HIP engine and on the FP64
+rarefied argon-like gas, adjustTimeStep at maxCo 0.8), its demo/testSymmetry/symmetry (the
+left half with a symmetryMod plane) and a cyclic channel written in OpenFOAM format run through dugksfoam_amd.solver.Solver on the HIP engine and on the FP64
 oracle.  Bar: relative L2 <= max(1e-12, 2 x floor) on rho, U, T, q, where floor is the
 distance between the oracle and the oracle with every velocity sum reversed (an equally
 valid FP64 evaluation, run live), and the same time-step sequence to 1e-13 (1e-8 with FP32
@@ -97,6 +97,12 @@ def demo_dir(tmp_path_factory):
 
 def test_demo_cavity_fp64(demo_dir):
     _compare("demo_cavity", demo_dir, (1, 10, 60))
+
+
+def test_demo_symmetry_fp64(tmp_path):
+    """demo/testSymmetry/symmetry: half cavity with a symmetryMod plane."""
+    d = foam_box.write_fixture_case(str(tmp_path), os.path.join(GOLDEN, "demo_symmetry_case.npz"))
+    _compare("demo_symmetry", d, (1, 10, 40))
 
 
 def test_demo_cavity_fp32_storage(demo_dir):

@@ -39,6 +39,9 @@ def _small_cases():
         "channel_20x8": (lambda: cases.channel(20, 8, 1.25, 0.5, nv=12), 10),
         "delaunay_300": (lambda: cases.delaunay_cavity(300, 12, nv=12), 6),
         "batch_4": (lambda: cases.batch(n_cases=4, nmin=4, nmax=8, nv=8, seed=5), 5),
+        "symcavity_6x12": (lambda: cases.symmetry_cavity(6, 12, nv=12), 10),
+        "symcavity_odd": (lambda: cases.symmetry_cavity(5, 10, nv=9), 8),
+        "symcavity_left": (lambda: cases.symmetry_cavity(5, 10, nv=10, plane="x0"), 8),
     }
 
 
@@ -117,7 +120,7 @@ def test_courant(pair):
     assert b[1] == pytest.approx(a[1], rel=1e-12)
 
 
-@pytest.mark.parametrize("which", ["cavity2d_12x21", "cavity3d_6x12", "delaunay_300", "channel_20x8"])
+@pytest.mark.parametrize("which", ["cavity2d_12x21", "cavity3d_6x12", "delaunay_300", "channel_20x8", "symcavity_6x12"])
 def test_fp32_storage(which):
     mk, n = _small_cases()[which]
     c = mk()
@@ -327,6 +330,15 @@ def test_face_value_mode(which):
     switch on a running engine."""
     mk, n = _small_cases()[which]
     c = mk()
+    if any(p["type"] == cases.PATCH_SYMMETRY for p in c.patches):
+        # symmetry planes are handled in face-value mode only
+        e = Engine(c)
+        assert e.face_values()
+        with pytest.raises(DugksError) as x:
+            e.set_face_values(False)
+        assert x.value.code == -6
+        e.close()
+        return
     for st, tol in (("fp64", 0.0), ("fp32", 1e-6)):
         a, b = Engine(c, storage=st), Engine(c, storage=st)
         a.set_face_values(True)
