@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-DUGKS_ABI_VERSION = 1
+DUGKS_ABI_VERSION = 2
 STORAGE_FP64 = 0
 STORAGE_FP32 = 1
 
@@ -20,7 +20,7 @@ _ip = C.POINTER(C.c_int32)
 
 class DugksPatch(C.Structure):
     _fields_ = [("type", C.c_int32), ("start", C.c_int32), ("size", C.c_int32),
-                ("neighbour_patch", C.c_int32), ("U", _dp), ("T", _dp)]
+                ("neighbour_patch", C.c_int32), ("U", _dp), ("T", _dp), ("rho", _dp)]
 
 
 class DugksCase(C.Structure):
@@ -70,6 +70,7 @@ class CaseABI:
             pats[j].neighbour_patch = int(p["neighbour_patch"])
             pats[j].U = d(p["U"]) if p.get("U") is not None else _dp()
             pats[j].T = d(p["T"]) if p.get("T") is not None else _dp()
+            pats[j].rho = d(p["rho"]) if p.get("rho") is not None else _dp()
         keep.append(pats)
         s = DugksCase()
         s.abi_version = DUGKS_ABI_VERSION

@@ -28,6 +28,8 @@ import numpy as np
 PATCH_WALL = 1
 PATCH_CYCLIC = 2
 PATCH_SYMMETRY = 3
+PATCH_FARFIELD = 4
+PATCH_FIXED = 5
 
 # Boltzmann constant and argon data of demo/cavity/para.py:13-20
 K_BOLTZMANN = 1.38066e-23
@@ -135,6 +137,7 @@ class Patch:
     U: Optional[np.ndarray] = None   # [n,3]
     T: Optional[np.ndarray] = None   # [n]
     neighbour_patch: int = -1
+    rho: Optional[np.ndarray] = None  # [n] far-field / fixed patches
 
 
 @dataclasses.dataclass
@@ -217,7 +220,8 @@ def _assemble(name, dim, C, V, own_i, nei_i, Sf_i, Cf_i, patches: List[Patch]) -
         pl.append(dict(name=p.name, type=p.type, start=start, size=n,
                        neighbour_patch=p.neighbour_patch,
                        U=None if p.U is None else np.ascontiguousarray(p.U, dtype=np.float64),
-                       T=None if p.T is None else np.ascontiguousarray(p.T, dtype=np.float64)))
+                       T=None if p.T is None else np.ascontiguousarray(p.T, dtype=np.float64),
+                       rho=None if p.rho is None else np.ascontiguousarray(p.rho, dtype=np.float64)))
         start += n
     owner = np.concatenate(own).astype(np.int32)
     Sf = np.concatenate(Sf).astype(np.float64)
@@ -411,6 +415,29 @@ def symmetry_cavity(nx: int = 30, ny: int = 60, nv: int = 28, kn: float = 0.075,
     dt = courant_dt(max(2.0 * nx, float(ny)), d, xi_max, co)
     return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
                    meta=dict(kn=kn, u_lid=u_lid, nx=nx, ny=ny, nv=nv))
+
+
+def open_channel(nx: int = 20, ny: int = 8, Lx: float = 2.0, nv: int = 16, kn: float = 0.2,
+                 rho_in: float = 1.05, u_in: float = 30.0, R: float = 208.0, T0: float = 273.0,
+                 rho0: float = 1.0, xi_sigma: float = 4.0, Pr: float = 2.0 / 3.0, omega: float = 0.5,
+                 K: int = 0, co: float = 0.5) -> Case:
+    """Open 2-D channel [0, Lx] x [0, 1]: diffuse walls at y = 0, 1 (T0); the inlet x = 0 a fixed
+    boundary Maxwellian (rho fixedValue: rho_in, U = (u_in, 0, 0), T0); the outlet x = Lx a far
+    field (rho farField: rho0, T0, incoming velocity from the outlet cells)."""
+    d, C, V, own, nei, Sf, Cf, sides = box_mesh(nx, ny, 0, Lx, 1.0)
+    walls = _wall("walls", _concat_sides([sides["y0"], sides["y1"]]), (0, 0, 0), T0)
+    n_in, n_out = sides["x0"]["owner"].size, sides["x1"]["owner"].size
+    inlet = Patch("inlet", PATCH_FIXED, sides["x0"]["owner"], sides["x0"]["Sf"], sides["x0"]["Cf"],
+                  U=np.tile([u_in, 0.0, 0.0], (n_in, 1)), T=np.full(n_in, T0), rho=np.full(n_in, rho_in))
+    outlet = Patch("outlet", PATCH_FARFIELD, sides["x1"]["owner"], sides["x1"]["Sf"], sides["x1"]["Cf"],
+                   T=np.full(n_out, T0), rho=np.full(n_out, rho0))
+    geo = _assemble(f"openchannel_{nx}x{ny}_{nv}", d, C, V, own, nei, Sf, Cf, [walls, inlet, outlet])
+    xi_max = xi_sigma * math.sqrt(R * T0)
+    xi, w = dv_newton_cotes(xi_max, nv)
+    mu = mu_from_kn(kn, R, T0, rho0)
+    dt = courant_dt(max(nx / Lx, float(ny)), d, xi_max, co)
+    return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
+                   meta=dict(kn=kn, nx=nx, ny=ny, nv=nv, rho_in=rho_in, u_in=u_in))
 
 
 def channel(nx: int = 200, ny: int = 40, Lx: float = 5.0, Ly: float = 1.0, nv: int = 32,

@@ -59,6 +59,10 @@ constexpr int KIND_NEI = 1;   // internal face, this cell is the neighbour
 constexpr int KIND_WALL = 2;  // Maxwell wall (fixedGradient gBarPvol, maxwellWall gSurf)
 constexpr int KIND_CYC = 3;   // cyclic (coupled) patch face
 constexpr int KIND_SYM = 4;   // symmetry plane (symmetryMod rho, DVMsymmetry gSurf, fixedGradient gBarPvol)
+constexpr int KIND_FAR = 5;   // incoming Maxwellian: far field (farField rho) or fixed (fixedValue rho)
+
+// Boundary faces with a lagged fixedGradient gBarPvol row; open faces relax only outgoing values.
+__host__ __device__ constexpr inline bool grad_row(int kind) { return kind == KIND_WALL || kind == KIND_SYM || kind == KIND_FAR; }
 constexpr int MAXS = 8;       // faces per cell handled by k_cell
 constexpr int NAXMAX = 192;   // per-axis exp table entries (3 axes x 64 points)
 constexpr int MAXNT = 512;     // workgroup size cap (256 VGPRs per lane available)
@@ -122,7 +126,8 @@ struct alignas(16) WallC {
     double incoming;
     double pre;     // 1 / (2 pi R T_w)^(D/2)
     double A[14];   // rho, rhoU(D), rhoE, Q(D), P(D(D+1)/2) of the incoming part
-    double pad[2];
+    double rho_b;   // far-field / fixed faces: boundary density
+    double mode;    // far-field / fixed faces: 0 incoming U from the owner cell, 1 the fixed U
 };
 static_assert(sizeof(WallC) == 176, "wallc size");
 
@@ -524,7 +529,7 @@ DEVI double face_value(int kind, double s, double gP, const double *GP, const do
 DEVI bool need_P(int kind, double s) { return kind != KIND_OWN || !(s < -VSMALL); }
 DEVI bool need_N(int kind, double s)
 {
-    if (kind == KIND_WALL || kind == KIND_SYM) return false;
+    if (grad_row(kind)) return false;
     if (kind == KIND_CYC) return !(s > VSMALL);
     return !(s >= VSMALL);
 }
@@ -686,7 +691,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
             const double dCo[3] = {S.dCo[0], S.dCo[1], S.dCo[2]};
             TS *__restrict__ fg = p.fvg + (size_t)S.face * p.nvp;
             TS *__restrict__ fh = H ? p.fvh + (size_t)S.face * p.nvp : nullptr;
-            if (kind == KIND_WALL || kind == KIND_SYM) {
+            if (grad_row(kind)) {
                 const size_t bi = (size_t)S.bidx * p.nvp + k;
                 double ng = GC[0] * dCo[0] + GC[1] * dCo[1];
                 if (D == 3) ng += GC[2] * dCo[2];
@@ -699,6 +704,20 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                 if (s > 0) {
                     fg[k] = (TS)face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx);
                     if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx);
+                }
+                if (kind == KIND_FAR && !(s > 0)) {
+                    // incoming: rho_b M(U, T_b) with the owner cell's velocity (farField,
+                    // discreteVelocity.C:588-602) or the fixed boundary one (mixed, frozen at the
+                    // initial boundary Maxwellian, discreteVelocity.C:312-344)
+                    const CONST_AS WallC &W = cptr(p.wall)[S.bidx];
+                    const double Ui[3] = {W.mode == 0.0 ? cptr(p.cU)[3 * c] : W.U[0],
+                                          W.mode == 0.0 ? cptr(p.cU)[3 * c + 1] : W.U[1],
+                                          W.mode == 0.0 ? cptr(p.cU)[3 * c + 2] : W.U[2]};
+                    const double cx = Ui[0] - v.x, cy = Ui[1] - v.y, cz = Ui[2] - v.z;
+                    const double RT = p.R * W.T;
+                    const double gin = W.rho_b * (W.pre * exp(-(cx * cx + cy * cy + cz * cz) / (2.0 * RT)));
+                    fg[k] = (TS)(gin - row_off<D>(p, v, 0, vt));
+                    if (H) fh[k] = (TS)(gin * RT * p.kc - row_off<D>(p, v, 1, vt));
                 }
                 if (kind == KIND_SYM && __any(!(s > 0))) {
                     // incoming at a symmetry plane (discreteVelocity.C:746-790): the outgoing
@@ -990,7 +1009,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
             // roles: this cell is the face owner P except on KIND_NEI faces
             const bool nei = kind == KIND_NEI;
             bool needO;
-            if (kind == KIND_WALL || kind == KIND_SYM) needO = false;
+            if (grad_row(kind)) needO = false;
             else if (nei) needO = need_P(KIND_OWN, s);
             else needO = need_N(kind, s);
             double gO = 0.0, hO = 0.0;
@@ -1027,9 +1046,9 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
                     gf = face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx) + og;
                     if (H) hf = face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx) + oh;
                 }
-                // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium; on a
-                // symmetry plane only the outgoing values (the patch's own operators are no-ops)
-                if (kind != KIND_SYM || s > 0) {
+                // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium; on symmetry,
+                // far-field and fixed patches only the outgoing values (their operators are no-ops)
+                if (kind == KIND_OWN || kind == KIND_NEI || kind == KIND_CYC || s > 0) {
                     EqC e;
                     e.U[0] = fe.U[0]; e.U[1] = fe.U[1]; e.U[2] = fe.U[2];
                     e.invRT = fe.invRT; e.pre = fe.pre; e.RT = fe.RT;
@@ -1339,7 +1358,7 @@ struct dugks_engine {
 
     // device buffers
     uint32_t *vidx = nullptr, *mir = nullptr;
-    int has_sym = 0;     // symmetry patches present (they need face-value mode)
+    int has_sym = 0;     // symmetry, far-field or fixed patches present (they need face-value mode)
     double *axw = nullptr;
     double *axis = nullptr;
     int *cs = nullptr, *ccase = nullptr, *flag = nullptr, *wfslot = nullptr;
@@ -1726,7 +1745,7 @@ int build(dugks_engine *h, const dugks_case *cs)
 
     // ---- boundary bookkeeping ----
     std::vector<int> bkind(nb, 0), bpartner(nb, -1), bpatch(nb, -1), bwall(nb, -1), bsym(nb, -1);
-    std::vector<double> Ub(3 * (size_t)nb, 0.0), Tb(nb, 0.0);
+    std::vector<double> Ub(3 * (size_t)nb, 0.0), Tb(nb, 0.0), rhob(nb, 0.0);
     h->patch_type.assign(cs->n_patches, 0);
     h->patch_start.assign(cs->n_patches, 0);
     h->patch_size.assign(cs->n_patches, 0);
@@ -1736,8 +1755,15 @@ int build(dugks_engine *h, const dugks_case *cs)
         const dugks_patch &pp = cs->patches[pi];
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf)
             return fail(h, DUGKS_ERR_INVALID, "patch " + std::to_string(pi) + " face range outside the boundary");
-        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_SYMMETRY)
+        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_SYMMETRY &&
+            pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_FIXED)
             return fail(h, DUGKS_ERR_UNSUPPORTED, "patch " + std::to_string(pi) + ": unsupported type");
+        const bool far = pp.type == DUGKS_PATCH_FARFIELD || pp.type == DUGKS_PATCH_FIXED;
+        if (far) {
+            if (!pp.rho || !pp.T || (pp.type == DUGKS_PATCH_FIXED && !pp.U))
+                return fail(h, DUGKS_ERR_INVALID, "far-field / fixed patch without rho, T (and U)");
+            h->has_sym = 1;
+        }
         h->patch_type[pi] = pp.type; h->patch_start[pi] = pp.start; h->patch_size[pi] = pp.size;
         int sym_axis = -1;
         if (pp.type == DUGKS_PATCH_WALL) {
@@ -1756,7 +1782,7 @@ int build(dugks_engine *h, const dugks_case *cs)
                     return fail(h, DUGKS_ERR_UNSUPPORTED,
                                 "symmetry patch " + std::to_string(pi) + " must be normal to a solution axis");
             }
-        } else {
+        } else if (pp.type == DUGKS_PATCH_CYCLIC) {
             if (pp.neighbour_patch < 0 || pp.neighbour_patch >= cs->n_patches ||
                 cs->patches[pp.neighbour_patch].size != pp.size ||
                 cs->patches[pp.neighbour_patch].type != DUGKS_PATCH_CYCLIC)
@@ -1776,6 +1802,14 @@ int build(dugks_engine *h, const dugks_case *cs)
                 // a fixedGradient row like a wall's; the wall constants are placeholders
                 Tb[b] = cs->T_ref;
                 bsym[b] = sym_axis;
+                bwall[b] = nw++;
+            } else if (far) {
+                // a fixedGradient row; the wall record carries rho_b, T_b (and U_b)
+                if (pp.type == DUGKS_PATCH_FIXED)
+                    for (int d = 0; d < 3; ++d) Ub[3 * b + d] = pp.U[3 * i + d];
+                Tb[b] = pp.T[i];
+                rhob[b] = pp.rho[i];
+                if (!(Tb[b] > 0) || !(rhob[b] > 0)) return fail(h, DUGKS_ERR_INVALID, "boundary rho and T must be > 0");
                 bwall[b] = nw++;
             } else {
                 bpartner[b] = cs->patches[pp.neighbour_patch].start - nfi + i;
@@ -1923,8 +1957,8 @@ int build(dugks_engine *h, const dugks_case *cs)
             } else {
                 const int b = f - nfi;
                 for (int d = 0; d < 3; ++d) s.L[d] = pvec[3 * f + d];
-                if (bkind[b] == DUGKS_PATCH_WALL || bkind[b] == DUGKS_PATCH_SYMMETRY) {
-                    s.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : KIND_SYM;
+                if (bkind[b] != DUGKS_PATCH_CYCLIC) {
+                    s.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : (bkind[b] == DUGKS_PATCH_SYMMETRY ? KIND_SYM : KIND_FAR);
                     s.bidx = bwall[b];
                     if (s.kind == KIND_SYM) s.other = bsym[b];
                     for (int d = 0; d < 3; ++d) s.dCo[d] = cs->Sf[3 * f + d] / magSf[f];
@@ -1960,8 +1994,8 @@ int build(dugks_engine *h, const dugks_case *cs)
             for (int d = 0; d < 3; ++d) x.dN[d] = cs->Cf[3 * f + d] - Cc[3 * x.N + d];
         } else {
             const int b = f - nfi;
-            if (bkind[b] == DUGKS_PATCH_WALL || bkind[b] == DUGKS_PATCH_SYMMETRY) {
-                x.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : KIND_SYM;
+            if (bkind[b] != DUGKS_PATCH_CYCLIC) {
+                x.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : (bkind[b] == DUGKS_PATCH_SYMMETRY ? KIND_SYM : KIND_FAR);
                 x.bidx = bwall[b];
                 for (int d = 0; d < 3; ++d) x.dN[d] = cs->Sf[3 * f + d] / magSf[f];
             } else {
@@ -2031,7 +2065,7 @@ int build(dugks_engine *h, const dugks_case *cs)
             if (j >= ns) continue;
             const Slot &q = sl[cnt[c] + j];
             for (int d = 0; d < 3; ++d) t.L[j][d] = q.L[d];
-            if (q.kind == KIND_WALL || q.kind == KIND_SYM) {
+            if (grad_row(q.kind)) {
                 t.off[j] = (int64_t)(nc + q.bidx) * h->nvp;
                 t.a[j] = q.aux;
                 t.wmask |= 1 << j;
@@ -2112,6 +2146,8 @@ int build(dugks_engine *h, const dugks_case *cs)
         W.T = Tb[b];
         double tp = 2.0 * PI * (cs->R * Tb[b]);
         W.pre = D == 3 ? 1.0 / (tp * std::sqrt(tp)) : 1.0 / tp;
+        W.rho_b = rhob[b];
+        W.mode = bkind[b] == DUGKS_PATCH_FIXED ? 1.0 : 0.0;
         wc[bwall[b]] = W;
     }
     for (int i = 0; i < nw; ++i) {
@@ -2460,8 +2496,10 @@ int dugks_init_equilibrium(dugks_engine *h, const double *rho, const double *U, 
                 const int f = s.face;
                 if (s.kind == KIND_WALL) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];
-                } else if (s.kind == KIND_SYM) {
+                } else if (s.kind == KIND_SYM || (s.kind == KIND_FAR && wc[s.bidx].mode == 0.0)) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = U[3 * c + d];   // U zeroGradient there
+                } else if (s.kind == KIND_FAR) {
+                    for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];
                 } else {
                     // linear interpolation w U_P + (1 - w) U_N (cyclic: patch weight)
                     const double wf = h->face_w[f];
@@ -2763,7 +2801,8 @@ int dugks_set_graphs(dugks_engine *h, int32_t on)
 int dugks_set_face_values(dugks_engine *h, int32_t on)
 {
     if (!h || !h->stream) return h ? fail(h, DUGKS_ERR_STATE, "no device resources") : DUGKS_ERR_INVALID;
-    if (!on && h->has_sym) return fail(h, DUGKS_ERR_UNSUPPORTED, "symmetry patches need face-value mode");
+    if (!on && h->has_sym)
+        return fail(h, DUGKS_ERR_UNSUPPORTED, "symmetry, far-field and fixed patches need face-value mode");
     HIPCHK(h, hipSetDevice(h->device));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (on && !h->fvg) {

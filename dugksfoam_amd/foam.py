@@ -6,8 +6,9 @@ read_case(path) builds a Case (dugksfoam_amd.cases.Case) from the files a dugksF
   0/{rho,U,T}                     initial fields (uniform or nonuniform internalField) and the
                                   boundary types: rho calculatedMaxwell + U, T fixedValue is a
                                   diffuse Maxwell wall (doc/usage.tex:61-75); rho symmetryMod is a
-                                  mirror plane; empty patches are omitted (fvPatch size 0);
-                                  cyclic patches are coupled pairs
+                                  mirror plane; rho farField / fixedValue with T fixedValue are
+                                  open boundaries with an incoming Maxwellian; empty patches are
+                                  omitted (fvPatch size 0); cyclic patches are coupled pairs
   constant/DVMProperties          fvDVMparas {nDV, xiMax, xiMin}, gasProperties
                                   {R, omega, Tref, muRef, Pr, KInner = 0} (fvDVM.C:913-933)
   constant/Xis, constant/weights  the 1-D discrete-velocity set (fvDVM.C:56-117)
@@ -17,8 +18,8 @@ Geometry follows OpenFOAM's primitiveMesh: face centres and area vectors by tria
 decomposition about the point average of the face; cell centres and volumes by pyramid
 decomposition about the average of the cell's face centres.  Interpolation weights and
 boundary deltas are those of cases._assemble (surfaceInterpolation::makeWeights, the
-patch-normal fvPatch::delta).  Patch types other than walls, symmetryMod planes, cyclics and
-empty are rejected.
+patch-normal fvPatch::delta).  Patch types other than walls, symmetryMod planes, farField and
+fixedValue open boundaries, cyclics and empty are rejected.
 """
 from __future__ import annotations
 
@@ -297,6 +298,18 @@ def read_case(path: str, dt: float = None) -> cases.Case:
         if b["type"] == "cyclic":
             plist.append(cases.Patch(b["name"], cases.PATCH_CYCLIC, owner[sl], Sf_all[sl], Cf_all[sl],
                                      neighbour_patch=index[b["neighbourPatch"]]))
+        elif rtype in ("farField", "fixedValue"):
+            # farField / mixed DF patches (bcMap, discreteVelocity.C:275-281): incoming Maxwellian
+            # of the boundary rho and T; the far field takes U from the boundary cells
+            if T_b[b["name"]]["type"] != "fixedValue" or rho_b[b["name"]]["value"] is None:
+                raise NotImplementedError(f"patch {b['name']}: {rtype} rho needs a rho value and fixedValue T")
+            kw = dict(T=T_b[b["name"]]["value"], rho=rho_b[b["name"]]["value"])
+            if rtype == "fixedValue":
+                if U_b[b["name"]]["value"] is None:
+                    raise NotImplementedError(f"patch {b['name']}: fixedValue rho needs a U value")
+                kw["U"] = U_b[b["name"]]["value"]
+            kind = cases.PATCH_FARFIELD if rtype == "farField" else cases.PATCH_FIXED
+            plist.append(cases.Patch(b["name"], kind, owner[sl], Sf_all[sl], Cf_all[sl], **kw))
         elif rtype == "symmetryMod":
             # DVMsymmetry DF patch (bcMap, discreteVelocity.C:279): mirror plane
             plist.append(cases.Patch(b["name"], cases.PATCH_SYMMETRY, owner[sl], Sf_all[sl], Cf_all[sl]))

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define DUGKS_ABI_VERSION 1
+#define DUGKS_ABI_VERSION 2
 
 /* status codes */
 enum dugks_status {
@@ -49,9 +49,15 @@ enum dugks_status {
 enum dugks_patch_type {
     DUGKS_PATCH_WALL = 1,      /* rho calculatedMaxwell, U/T fixedValue: diffuse Maxwell wall */
     DUGKS_PATCH_CYCLIC = 2,    /* OpenFOAM cyclic (coupled) patch pair                      */
-    DUGKS_PATCH_SYMMETRY = 3   /* rho symmetryMod (gSurf DVMsymmetry): axis-aligned mirror
+    DUGKS_PATCH_SYMMETRY = 3,  /* rho symmetryMod (gSurf DVMsymmetry): axis-aligned mirror
                                   plane; incoming values = the mirrored velocity's outgoing
                                   value (fvDVM.C:375-454, discreteVelocity.C:733-817)       */
+    DUGKS_PATCH_FARFIELD = 4,  /* rho farField, T fixedValue (gSurf farField): incoming
+                                  values rho_b M(U of the owner cell, T_b)
+                                  (discreteVelocity.C:574-604)                               */
+    DUGKS_PATCH_FIXED = 5      /* rho fixedValue (gSurf mixed): incoming values frozen at the
+                                  boundary Maxwellian rho_b M(U_b, T_b)
+                                  (discreteVelocity.C:312-344, 556-573, 1046-1060)           */
 };
 
 /* Storage precision of the resident distributions. */
@@ -65,8 +71,9 @@ typedef struct dugks_patch {
     int32_t start;             /* first face, global face numbering                  */
     int32_t size;              /* number of faces                                    */
     int32_t neighbour_patch;   /* cyclic partner patch index, else -1                */
-    const double* U;           /* [size][3] boundary U (wall velocity), may be NULL for cyclic */
-    const double* T;           /* [size]    boundary T (wall temperature), NULL for cyclic     */
+    const double* U;           /* [size][3] boundary U (wall, fixed), else may be NULL          */
+    const double* T;           /* [size]    boundary T (wall, far field, fixed), else NULL      */
+    const double* rho;         /* [size]    boundary rho (far field, fixed), else NULL          */
 } dugks_patch;
 
 typedef struct dugks_case {

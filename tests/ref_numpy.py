@@ -10,7 +10,7 @@ import math
 
 import numpy as np
 
-from dugksfoam_amd.cases import PATCH_CYCLIC, PATCH_SYMMETRY, PATCH_WALL
+from dugksfoam_amd.cases import PATCH_CYCLIC, PATCH_FARFIELD, PATCH_FIXED, PATCH_SYMMETRY, PATCH_WALL
 
 
 class NumpyDUGKS:
@@ -29,12 +29,18 @@ class NumpyDUGKS:
         self.Ub = np.zeros((self.nb, 3))
         self.Tb = np.ones(self.nb)
         self.axis = np.zeros(self.nb, np.int64)
+        self.rhob = np.zeros(self.nb)
         for p in c.patches:
             b = np.arange(p["start"], p["start"] + p["size"]) - nfi
             self.kind[b] = p["type"]
             if p["type"] == PATCH_WALL:
                 self.Ub[b] = p["U"]
                 self.Tb[b] = p["T"]
+            elif p["type"] in (PATCH_FARFIELD, PATCH_FIXED):
+                self.Tb[b] = p["T"]
+                self.rhob[b] = p["rho"]
+                if p["type"] == PATCH_FIXED:
+                    self.Ub[b] = p["U"]
             elif p["type"] == PATCH_SYMMETRY:
                 # mirror axis: the normal of the patch's first face (discreteVelocity.C:766-768)
                 self.axis[b] = int(np.argmax(np.abs(c.Sf[p["start"]])))
@@ -49,6 +55,7 @@ class NumpyDUGKS:
         self.mirror = np.stack([(iz * n + iy) * n + (n - 1 - ix), (iz * n + (n - 1 - iy)) * n + ix,
                                 ((nz - 1 - iz) * n + iy) * n + ix])
         self.sym = np.nonzero(self.kind == PATCH_SYMMETRY)[0]
+        self.far = np.nonzero((self.kind == PATCH_FARFIELD) | (self.kind == PATCH_FIXED))[0]
         self.wall = np.nonzero(self.kind == PATCH_WALL)[0]
         self.cyc = np.nonzero(self.kind == PATCH_CYCLIC)[0]
         self.bP = self.own[nfi:]
@@ -144,7 +151,7 @@ class NumpyDUGKS:
         phib_g = np.empty((xi.shape[0], self.nb))
         phib_h = np.empty_like(phib_g)
         cy = self.cyc
-        wl = np.concatenate([self.wall, self.sym])   # fixedGradient gBarPvol patches
+        wl = np.concatenate([self.wall, self.sym, self.far])   # fixedGradient gBarPvol patches
         phib_g[:, wl] = gbar[:, self.bP[wl]] + self.gradB_g[:, wl] / self.dcoef[wl]
         phib_h[:, wl] = hbar[:, self.bP[wl]] + self.gradB_h[:, wl] / self.dcoef[wl]
         lsq_b_g = phib_g.copy()
@@ -184,6 +191,13 @@ class NumpyDUGKS:
         hin = gin * (R * self.Tb)[None, :] * (K + 3 - D)
         gb = np.where(sb > 0, gP, gin)
         hb = np.where(sb > 0, hP, hin)
+        if self.far.size:
+            # far field: rho_b M(U of the outlet cell, T_b); fixed: rho_b M(U_b, T_b)
+            fb = self.far
+            Uin = np.where((self.kind[fb] == PATCH_FARFIELD)[:, None], self.U[self.bP[fb]], self.Ub[fb])
+            Mf = self.rhob[fb][None, :] * self._maxwell_by_rho(Uin, self.Tb[fb])
+            gb[:, fb] = np.where(sb[:, fb] > 0, gP[:, fb], Mf)
+            hb[:, fb] = np.where(sb[:, fb] > 0, hP[:, fb], Mf * (R * self.Tb[fb])[None, :] * (K + 3 - D))
         for b in self.sym:
             # symmetry plane: incoming = the mirrored velocity's outgoing reconstruction
             m = self.mirror[self.axis[b]]

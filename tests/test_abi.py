@@ -42,7 +42,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_abi_version(lib):
-    assert lib.dugks_abi_version() == abi.DUGKS_ABI_VERSION == 1
+    assert lib.dugks_abi_version() == abi.DUGKS_ABI_VERSION == 2
 
 
 def test_struct_layout_matches_header(tmp_path):
@@ -84,7 +84,7 @@ def test_create_validates_arguments(lib):
     assert r == -1
     assert b"abi_version" in lib.dugks_last_error(h)
     lib.dugks_destroy(h)
-    a.struct.abi_version = 1
+    a.struct.abi_version = abi.DUGKS_ABI_VERSION
     a.struct.dim = 4
     r = lib.dugks_create(C.cast(a.ref, C.c_void_p), C.byref(h))
     assert r == -6

@@ -94,8 +94,8 @@ def test_unsupported_patches_rejected(tmp_path):
     foam_box.cavity_case(str(tmp_path), c)
     p = tmp_path / "0" / "rho"
     p.write_text(p.read_text().replace("type calculatedMaxwell;\n        value uniform 1.0;",
-                                       "type farField;\n        value uniform 1.0;", 1))
-    with pytest.raises(NotImplementedError, match="farField"):
+                                       "type zeroGradient;\n        value uniform 1.0;", 1))
+    with pytest.raises(NotImplementedError, match="zeroGradient"):
         foam.read_case(str(tmp_path))
 
 
@@ -247,3 +247,32 @@ def test_demo_fixture_case(tmp_path):
     assert c.dt == pytest.approx(0.8 / (60 * math.sqrt(2) * c.xi_max), rel=1e-13)
     if os.path.isdir(REF):
         _same_case(c, foam.read_case(os.path.join(REF, "cavity")), tol=0.0)
+
+
+def test_open_boundaries_read_back(tmp_path):
+    """rho fixedValue (U, T fixedValue) and rho farField (T fixedValue) patches read back as the
+    open channel's fixed inlet and far-field outlet."""
+    c = cases.open_channel(8, 4, nv=8)
+    wall = foam_box.wall((0, 0, 0), c.T0[0])
+    inlet, outlet = c.patches[1], c.patches[2]
+    empty = "        type empty;\n"
+    u_in = " ".join(repr(float(x)) for x in inlet["U"][0])
+    fields = dict(
+        rho=(f"uniform {float(c.rho0[0])!r}", {
+            "walls": wall["rho"], "frontAndBack": empty,
+            "inlet": f"        type fixedValue;\n        value uniform {float(inlet['rho'][0])!r};\n",
+            "outlet": f"        type farField;\n        value uniform {float(outlet['rho'][0])!r};\n"}),
+        U=("uniform (0 0 0)", {"walls": wall["U"], "frontAndBack": empty,
+                               "inlet": f"        type fixedValue;\n        value uniform ({u_in});\n",
+                               "outlet": "        type zeroGradient;\n"}),
+        T=(f"uniform {float(c.T0[0])!r}", {"walls": wall["T"], "frontAndBack": empty,
+                                           "inlet": wall["T"], "outlet": wall["T"]}))
+    patches = [("walls", "wall", ["y0", "y1"], ""), ("inlet", "patch", ["x0"], ""),
+               ("outlet", "patch", ["x1"], ""), ("frontAndBack", "empty", ["z0", "z1"], "")]
+    foam_box.write_box(str(tmp_path), 8, 4, 0, L=(2.0, 1.0, 1.0), patches=patches, fields=fields,
+                       dvm=foam_box.dvm_props(c), xi=c.xi, w=c.xi_weights, control=foam_box.control(c.dt))
+    r = foam.read_case(str(tmp_path))
+    _same_case(r, c)
+    for p, q in zip(r.patches, c.patches):
+        if q["rho"] is not None:
+            np.testing.assert_array_equal(p["rho"], q["rho"])

@@ -42,6 +42,7 @@ def _small_cases():
         "symcavity_6x12": (lambda: cases.symmetry_cavity(6, 12, nv=12), 10),
         "symcavity_odd": (lambda: cases.symmetry_cavity(5, 10, nv=9), 8),
         "symcavity_left": (lambda: cases.symmetry_cavity(5, 10, nv=10, plane="x0"), 8),
+        "openchannel_12x6": (lambda: cases.open_channel(12, 6, nv=12), 10),
     }
 
 
@@ -120,7 +121,8 @@ def test_courant(pair):
     assert b[1] == pytest.approx(a[1], rel=1e-12)
 
 
-@pytest.mark.parametrize("which", ["cavity2d_12x21", "cavity3d_6x12", "delaunay_300", "channel_20x8", "symcavity_6x12"])
+@pytest.mark.parametrize("which", ["cavity2d_12x21", "cavity3d_6x12", "delaunay_300", "channel_20x8", "symcavity_6x12",
+                                   "openchannel_12x6"])
 def test_fp32_storage(which):
     mk, n = _small_cases()[which]
     c = mk()
@@ -330,8 +332,8 @@ def test_face_value_mode(which):
     switch on a running engine."""
     mk, n = _small_cases()[which]
     c = mk()
-    if any(p["type"] == cases.PATCH_SYMMETRY for p in c.patches):
-        # symmetry planes are handled in face-value mode only
+    if any(p["type"] in (cases.PATCH_SYMMETRY, cases.PATCH_FARFIELD, cases.PATCH_FIXED) for p in c.patches):
+        # symmetry, far-field and fixed patches are handled in face-value mode only
         e = Engine(c)
         assert e.face_values()
         with pytest.raises(DugksError) as x:
