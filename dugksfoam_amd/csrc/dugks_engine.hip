@@ -160,7 +160,7 @@ struct alignas(16) FaceX {
     double dP[3];   // Cf - C(owner)
     double dN[3];   // Cf - C(neighbour) | wall: unit normal Sf/|Sf|
     double w;       // cyclic: patch weight
-    int32_t P, N;   // owner; neighbour or coupled partner cell (-1 on walls)
+    int32_t P, N;   // owner; neighbour or coupled partner cell; symmetry: mirror axis; else -1
     int32_t kind, bidx;  // KIND_OWN (internal), KIND_WALL, KIND_CYC; wall index
 };
 static_assert(sizeof(FaceX) == 96, "facex size");
@@ -199,6 +199,8 @@ struct P {
     const TS *gBg_in, *gBh_in;
     TS *gBg_out, *gBh_out;
     TS *fvg, *fvh;      // stored face values [face][velocity] (face-value mode), else null
+    int nopen;          // symmetry, far-field and fixed faces
+    const int *__restrict__ open;  // their face indices
     int *flag;
 };
 
@@ -403,17 +405,25 @@ DEVI void acc_moments(double *a, double g, double h, const Vel &v, const double 
         ch[d] = xi[d] - Uh[d];
         cc = fma(ch[d], ch[d], cc);
     }
-    double A = cc * g;
-    if (H) A += h;
+    const double hwg = hw * g;
+    double t[3];
 #pragma unroll
-    for (int d = 0; d < D; ++d) a[D + 2 + d] = fma(hw * ch[d], A, a[D + 2 + d]);
-    double hwg = hw * g;
+    for (int d = 0; d < D; ++d) t[d] = hwg * ch[d];
+    if (H) {
+        const double A = cc * g + h;
+#pragma unroll
+        for (int d = 0; d < D; ++d) a[D + 2 + d] = fma(hw * ch[d], A, a[D + 2 + d]);
+    } else {
+        // Q_d = sum (w g / 2) c_d |c|^2 with the shared products t_d = (w g / 2) c_d
+#pragma unroll
+        for (int d = 0; d < D; ++d) a[D + 2 + d] = fma(t[d], cc, a[D + 2 + d]);
+    }
     int m = 2 * D + 2;
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
         for (int j = i; j < D; ++j) {
-            a[m] = fma(hwg * ch[i], ch[j], a[m]);
+            a[m] = fma(t[i], ch[j], a[m]);
             ++m;
         }
 }
@@ -653,7 +663,8 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
 // reconstructions, for which the neighbour's gradient is formed on demand), internal faces it
 // neighbours with xi.S < -VSMALL, its cyclic faces (the coupled value needs the partner cell's
 // gradient where xi.S <= VSMALL) and the outgoing half of its wall faces, whose lagged
-// fixedGradient rows n . grad(gbar) it writes as well.  Every face value is written exactly once,
+// fixedGradient rows n . grad(gbar) it writes as well (so also for symmetry, far-field and fixed
+// faces, whose incoming halves k_face_open writes).  Every face value is written exactly once,
 // with the arithmetic of k_face (same gradient, same face_value), so the stored values equal
 // k_face's.
 // ---------------------------------------------------------------------------------------------
@@ -705,35 +716,6 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                     fg[k] = (TS)face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx);
                     if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx);
                 }
-                if (kind == KIND_FAR && !(s > 0)) {
-                    // incoming: rho_b M(U, T_b) with the owner cell's velocity (farField,
-                    // discreteVelocity.C:588-602) or the fixed boundary one (mixed, frozen at the
-                    // initial boundary Maxwellian, discreteVelocity.C:312-344)
-                    const CONST_AS WallC &W = cptr(p.wall)[S.bidx];
-                    const double Ui[3] = {W.mode == 0.0 ? cptr(p.cU)[3 * c] : W.U[0],
-                                          W.mode == 0.0 ? cptr(p.cU)[3 * c + 1] : W.U[1],
-                                          W.mode == 0.0 ? cptr(p.cU)[3 * c + 2] : W.U[2]};
-                    const double cx = Ui[0] - v.x, cy = Ui[1] - v.y, cz = Ui[2] - v.z;
-                    const double RT = p.R * W.T;
-                    const double gin = W.rho_b * (W.pre * exp(-(cx * cx + cy * cy + cz * cz) / (2.0 * RT)));
-                    fg[k] = (TS)(gin - row_off<D>(p, v, 0, vt));
-                    if (H) fh[k] = (TS)(gin * RT * p.kc - row_off<D>(p, v, 1, vt));
-                }
-                if (kind == KIND_SYM && __any(!(s > 0))) {
-                    // incoming at a symmetry plane (discreteVelocity.C:746-790): the outgoing
-                    // value of the velocity mirrored across the plane, rebuilt at its index
-                    const unsigned km = p.mir[(size_t)S.other * p.nvp + k];
-                    const Vel vm = load_vel<D>(p, km, vt);
-                    const double hxm[3] = {0.5 * vm.x * dt, 0.5 * vm.y * dt, 0.5 * vm.z * dt};
-                    const double gm = ld(rowC, km);
-                    const double hm = H ? ld(hrowC, km) : 0.0;
-                    double GM[3] = {0, 0, 0}, HM[3] = {0, 0, 0};
-                    grad_at<D, H, NS>(p, c, km, gm, hm, GM, HM);
-                    if (!(s > 0)) {
-                        fg[k] = (TS)face_value<D>(KIND_WALL, s, gm, GM, dCs, 0.0, GM, dCo, 0.0, hxm);
-                        if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hm, HM, dCs, 0.0, HM, dCo, 0.0, hxm);
-                    }
-                }
                 continue;
             }
             if (kind == KIND_NEI) {
@@ -760,6 +742,61 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                 fg[k] = (TS)face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx);
                 if (H) fh[k] = (TS)face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx);
             }
+        }
+    }
+}
+
+// k_face_open: the incoming values (xi.S <= 0) of symmetry, far-field and fixed faces, one
+// workgroup per (face, velocity block).  Symmetry plane (discreteVelocity.C:746-790): the outgoing
+// value of the velocity mirrored across the plane, rebuilt at its index from the owner cell's gbar
+// and gradient.  Far field (discreteVelocity.C:588-602): rho_b M(U of the owner cell, T_b).
+// Fixed (mixed, frozen at the initial boundary Maxwellian, discreteVelocity.C:312-344):
+// rho_b M(U_b, T_b).  h = g R T_b (K + 3 - D) for the Maxwellians.
+// ---------------------------------------------------------------------------------------------
+template <int D, bool H, int NS, typename TS>
+__global__ __launch_bounds__(MAXNT) void k_face_open(P<TS> p)
+{
+    __shared__ double vt[VT_SIZE];
+    const int i = blockIdx.x, b = blockIdx.y;
+    if (i >= p.nopen || b >= p.nb) return;
+    stage_vt(p, vt);
+    __syncthreads();
+    const int f = cptr(p.open)[i];
+    const CONST_AS FaceX &fx = cptr(p.fx)[f];
+    const int c = fx.P;
+    const double dt = cptr(p.dtc)[cptr(p.ccase)[c]];
+    const double S[3] = {fx.S[0], fx.S[1], fx.S[2]};
+    const double dP[3] = {fx.dP[0], fx.dP[1], fx.dP[2]};
+    const CONST_AS WallC &W = cptr(p.wall)[fx.bidx];
+    const bool sym = fx.kind == KIND_SYM;
+    const double Ui[3] = {W.mode == 0.0 ? cptr(p.cU)[3 * c] : W.U[0], W.mode == 0.0 ? cptr(p.cU)[3 * c + 1] : W.U[1],
+                          W.mode == 0.0 ? cptr(p.cU)[3 * c + 2] : W.U[2]};
+    const double RT = p.R * W.T;
+    const TS *__restrict__ rowC = p.gb_in + (size_t)c * p.nvp;
+    const TS *__restrict__ hrowC = H ? p.hb_in + (size_t)c * p.nvp : nullptr;
+    TS *__restrict__ fg = p.fvg + (size_t)f * p.nvp;
+    TS *__restrict__ fh = H ? p.fvh + (size_t)f * p.nvp : nullptr;
+    const unsigned k0 = (unsigned)b * p.vb + threadIdx.x;
+    for (int it = 0; it < p.nib; ++it) {
+        const unsigned k = k0 + it * p.nt;
+        const Vel v = load_vel<D>(p, k, vt);
+        const double s = dotnc(v.x, v.y, v.z, S);
+        if (s > 0) continue;
+        if (sym) {
+            const unsigned km = p.mir[(size_t)fx.N * p.nvp + k];
+            const Vel vm = load_vel<D>(p, km, vt);
+            const double hxm[3] = {0.5 * vm.x * dt, 0.5 * vm.y * dt, 0.5 * vm.z * dt};
+            const double gm = ld(rowC, km);
+            const double hm = H ? ld(hrowC, km) : 0.0;
+            double GM[3] = {0, 0, 0}, HM[3] = {0, 0, 0};
+            grad_at<D, H, NS>(p, c, km, gm, hm, GM, HM);
+            fg[k] = (TS)(gm + dotg<D>(GM, dP, hxm));
+            if (H) fh[k] = (TS)(hm + dotg<D>(HM, dP, hxm));
+        } else {
+            const double cx = Ui[0] - v.x, cy = Ui[1] - v.y, cz = Ui[2] - v.z;
+            const double gin = W.rho_b * (W.pre * exp(-(cx * cx + cy * cy + cz * cz) / (2.0 * RT)));
+            fg[k] = (TS)(gin - row_off<D>(p, v, 0, vt));
+            if (H) fh[k] = (TS)(gin * RT * p.kc - row_off<D>(p, v, 1, vt));
         }
     }
 }
@@ -1359,6 +1396,8 @@ struct dugks_engine {
     // device buffers
     uint32_t *vidx = nullptr, *mir = nullptr;
     int has_sym = 0;     // symmetry, far-field or fixed patches present (they need face-value mode)
+    int nopen = 0;       // number of such faces
+    int *open = nullptr; // their face indices
     double *axw = nullptr;
     double *axis = nullptr;
     int *cs = nullptr, *ccase = nullptr, *flag = nullptr, *wfslot = nullptr;
@@ -1466,6 +1505,8 @@ P<TS> params(dugks_engine *h)
     p.gb_out = (TS *)h->gb[1 - h->cur]; p.hb_out = (TS *)h->hb[1 - h->cur];
     p.gBg_in = (const TS *)h->gB[h->cur]; p.gBh_in = (const TS *)h->hB[h->cur];
     p.gBg_out = (TS *)h->gB[1 - h->cur]; p.gBh_out = (TS *)h->hB[1 - h->cur];
+    p.nopen = h->nopen;
+    p.open = h->open;
     p.fvg = h->face_values ? (TS *)h->fvg : nullptr;
     p.fvh = h->face_values ? (TS *)h->fvh : nullptr;
     p.flag = h->flag;
@@ -1492,6 +1533,8 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
     if (ev) (void)hipEventRecord(ev[0], h->stream);
     if (FV && !h->face_pass_legacy) {
         hipLaunchKernelGGL((k_face_values<D, H, NS, TS>), block_grid(h->nc, h->nvb), dim3(h->nt), 0, h->stream, p);
+        if (h->nopen)
+            hipLaunchKernelGGL((k_face_open<D, H, NS, TS>), dim3(h->nopen, h->nvb), dim3(h->nt), 0, h->stream, p);
         if (ev) (void)hipEventRecord(ev[1], h->stream);
         hipLaunchKernelGGL((k_face_moments<D, H, TS>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
     } else {
@@ -1977,6 +2020,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     }
     // ---- per-face data of k_face ----
     std::vector<FaceX> fxs(nf);
+    std::vector<int> open_faces;
     for (int f = 0; f < nf; ++f) {
         FaceX x;
         std::memset(&x, 0, sizeof x);
@@ -1997,6 +2041,8 @@ int build(dugks_engine *h, const dugks_case *cs)
             if (bkind[b] != DUGKS_PATCH_CYCLIC) {
                 x.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : (bkind[b] == DUGKS_PATCH_SYMMETRY ? KIND_SYM : KIND_FAR);
                 x.bidx = bwall[b];
+                if (x.kind == KIND_SYM) x.N = bsym[b];
+                if (x.kind != KIND_WALL) open_faces.push_back(f);
                 for (int d = 0; d < 3; ++d) x.dN[d] = cs->Sf[3 * f + d] / magSf[f];
             } else {
                 x.kind = KIND_CYC;
@@ -2195,6 +2241,9 @@ int build(dugks_engine *h, const dugks_case *cs)
         (r = upload(h, h->corder, corder.data(), nc)) || (r = upload(h, h->forder, forder.data(), nf)) || (r = upload(h, h->wfslot, wfslot.data(), nw)))
         return r;
     if (nw && (r = upload(h, h->wall, wc.data(), nw))) return r;
+    h->nopen = (int)open_faces.size();
+    if (h->nopen && ((r = dalloc(h, &h->open, open_faces.size())) || (r = upload(h, h->open, open_faces.data(), open_faces.size()))))
+        return r;
     HIPCHK(h, hipMemset(h->flag, 0, sizeof(int)));
     h->dt_prep.assign(h->n_cases, 0.0);
     return 0;
@@ -2220,7 +2269,7 @@ void release(dugks_engine *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->vidx); dfree(h->mir); dfree(h->axw); dfree(h->moff); dfree(h->axis);
+    dfree(h->vidx); dfree(h->mir); dfree(h->open); dfree(h->axw); dfree(h->moff); dfree(h->axis);
     dfree(h->cs); dfree(h->ccase); dfree(h->flag); dfree(h->wfslot); dfree(h->slots); dfree(h->cV);
     dfree(h->dtc); dfree(h->dtp); dfree(h->muc); dfree(h->crho); dfree(h->cU); dfree(h->cT); dfree(h->cq);
     dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->feq);
