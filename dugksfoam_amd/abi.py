@@ -20,7 +20,8 @@ _ip = C.POINTER(C.c_int32)
 
 class DugksPatch(C.Structure):
     _fields_ = [("type", C.c_int32), ("start", C.c_int32), ("size", C.c_int32),
-                ("neighbour_patch", C.c_int32), ("U", _dp), ("T", _dp), ("rho", _dp)]
+                ("neighbour_patch", C.c_int32), ("U", _dp), ("T", _dp), ("rho", _dp),
+                ("pressure", C.c_double)]
 
 
 class DugksCase(C.Structure):
@@ -71,6 +72,7 @@ class CaseABI:
             pats[j].U = d(p["U"]) if p.get("U") is not None else _dp()
             pats[j].T = d(p["T"]) if p.get("T") is not None else _dp()
             pats[j].rho = d(p["rho"]) if p.get("rho") is not None else _dp()
+            pats[j].pressure = float(p.get("pressure") or 0.0)
         keep.append(pats)
         s = DugksCase()
         s.abi_version = DUGKS_ABI_VERSION

@@ -30,6 +30,8 @@ PATCH_CYCLIC = 2
 PATCH_SYMMETRY = 3
 PATCH_FARFIELD = 4
 PATCH_FIXED = 5
+PATCH_PRESSURE_IN = 6
+PATCH_PRESSURE_OUT = 7
 
 # Boltzmann constant and argon data of demo/cavity/para.py:13-20
 K_BOLTZMANN = 1.38066e-23
@@ -138,6 +140,7 @@ class Patch:
     T: Optional[np.ndarray] = None   # [n]
     neighbour_patch: int = -1
     rho: Optional[np.ndarray] = None  # [n] far-field / fixed patches
+    pressure: float = 0.0             # pressureIn / pressureOut patches
 
 
 @dataclasses.dataclass
@@ -221,7 +224,8 @@ def _assemble(name, dim, C, V, own_i, nei_i, Sf_i, Cf_i, patches: List[Patch]) -
                        neighbour_patch=p.neighbour_patch,
                        U=None if p.U is None else np.ascontiguousarray(p.U, dtype=np.float64),
                        T=None if p.T is None else np.ascontiguousarray(p.T, dtype=np.float64),
-                       rho=None if p.rho is None else np.ascontiguousarray(p.rho, dtype=np.float64)))
+                       rho=None if p.rho is None else np.ascontiguousarray(p.rho, dtype=np.float64),
+                       pressure=float(p.pressure)))
         start += n
     owner = np.concatenate(own).astype(np.int32)
     Sf = np.concatenate(Sf).astype(np.float64)
@@ -438,6 +442,29 @@ def open_channel(nx: int = 20, ny: int = 8, Lx: float = 2.0, nv: int = 16, kn: f
     dt = courant_dt(max(nx / Lx, float(ny)), d, xi_max, co)
     return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
                    meta=dict(kn=kn, nx=nx, ny=ny, nv=nv, rho_in=rho_in, u_in=u_in))
+
+
+def pressure_channel(nx: int = 20, ny: int = 8, Lx: float = 2.0, nv: int = 16, kn: float = 0.2,
+                     p_ratio: float = 1.2, R: float = 208.0, T0: float = 273.0, rho0: float = 1.0,
+                     xi_sigma: float = 4.0, Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0,
+                     co: float = 0.5) -> Case:
+    """Pressure-driven 2-D channel [0, Lx] x [0, 1]: diffuse walls at y = 0, 1 (T0); a pressureIn
+    patch at x = 0 (p_ratio rho0 R T0, T0) and a pressureOut patch at x = Lx (rho0 R T0)."""
+    d, C, V, own, nei, Sf, Cf, sides = box_mesh(nx, ny, 0, Lx, 1.0)
+    walls = _wall("walls", _concat_sides([sides["y0"], sides["y1"]]), (0, 0, 0), T0)
+    n_in, n_out = sides["x0"]["owner"].size, sides["x1"]["owner"].size
+    p0 = rho0 * R * T0
+    inlet = Patch("inlet", PATCH_PRESSURE_IN, sides["x0"]["owner"], sides["x0"]["Sf"], sides["x0"]["Cf"],
+                  T=np.full(n_in, T0), pressure=p_ratio * p0)
+    outlet = Patch("outlet", PATCH_PRESSURE_OUT, sides["x1"]["owner"], sides["x1"]["Sf"], sides["x1"]["Cf"],
+                   T=np.full(n_out, T0), pressure=p0)
+    geo = _assemble(f"pchannel_{nx}x{ny}_{nv}", d, C, V, own, nei, Sf, Cf, [walls, inlet, outlet])
+    xi_max = xi_sigma * math.sqrt(R * T0)
+    xi, w = dv_newton_cotes(xi_max, nv)
+    mu = mu_from_kn(kn, R, T0, rho0)
+    dt = courant_dt(max(nx / Lx, float(ny)), d, xi_max, co)
+    return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
+                   meta=dict(kn=kn, nx=nx, ny=ny, nv=nv, p_ratio=p_ratio))
 
 
 def channel(nx: int = 200, ny: int = 40, Lx: float = 5.0, Ly: float = 1.0, nv: int = 32,

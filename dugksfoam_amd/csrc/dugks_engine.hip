@@ -126,8 +126,9 @@ struct alignas(16) WallC {
     double incoming;
     double pre;     // 1 / (2 pi R T_w)^(D/2)
     double A[14];   // rho, rhoU(D), rhoE, Q(D), P(D(D+1)/2) of the incoming part
-    double rho_b;   // far-field / fixed faces: boundary density
-    double mode;    // far-field / fixed faces: 0 incoming U from the owner cell, 1 the fixed U
+    double rho_b;   // open faces: boundary density
+    double mode;    // open faces: 0 far field, 1 fixed (incoming U = U above; else the owner
+                    // cell's), 2 pressure inlet, 3 pressure outlet (rho_b, T reset every step)
 };
 static_assert(sizeof(WallC) == 176, "wallc size");
 
@@ -199,7 +200,8 @@ struct P {
     const TS *gBg_in, *gBh_in;
     TS *gBg_out, *gBh_out;
     TS *fvg, *fvh;      // stored face values [face][velocity] (face-value mode), else null
-    int nopen;          // symmetry, far-field and fixed faces
+    int nopen;          // symmetry, far-field, fixed and pressure faces
+    const double *__restrict__ bpress;  // [wall record] pressureIn / pressureOut pressure
     const int *__restrict__ open;  // their face indices
     int *flag;
 };
@@ -769,8 +771,9 @@ __global__ __launch_bounds__(MAXNT) void k_face_open(P<TS> p)
     const double dP[3] = {fx.dP[0], fx.dP[1], fx.dP[2]};
     const CONST_AS WallC &W = cptr(p.wall)[fx.bidx];
     const bool sym = fx.kind == KIND_SYM;
-    const double Ui[3] = {W.mode == 0.0 ? cptr(p.cU)[3 * c] : W.U[0], W.mode == 0.0 ? cptr(p.cU)[3 * c + 1] : W.U[1],
-                          W.mode == 0.0 ? cptr(p.cU)[3 * c + 2] : W.U[2]};
+    const bool fixedU = W.mode == 1.0;
+    const double Ui[3] = {fixedU ? W.U[0] : cptr(p.cU)[3 * c], fixedU ? W.U[1] : cptr(p.cU)[3 * c + 1],
+                          fixedU ? W.U[2] : cptr(p.cU)[3 * c + 2]};
     const double RT = p.R * W.T;
     const TS *__restrict__ rowC = p.gb_in + (size_t)c * p.nvp;
     const TS *__restrict__ hrowC = H ? p.hb_in + (size_t)c * p.nvp : nullptr;
@@ -1147,6 +1150,33 @@ __global__ __launch_bounds__(256) void k_cell_fin(P<TS> p)
 }
 
 // ---------------------------------------------------------------------------------------------
+// k_pressure_bc: S10, updatePressureInOutBC (fvDVM.C:730-806), one thread per open face.  Pressure
+// inlet: rho_b = p / (R T_b).  Pressure outlet, from the boundary cell's new state (rho_i, T_i),
+// a_i = sqrt(R T_i (K + 5) / (K + 3)): rho_b = rho_i + (p - rho_i R T_i) / a_i^2, T_b = p / (R rho_i).
+// The next step's incoming Maxwellian uses them (k_face_open).
+// ---------------------------------------------------------------------------------------------
+template <int D, typename TS>
+__global__ __launch_bounds__(256) void k_pressure_bc(P<TS> p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.nopen) return;
+    const FaceX &fx = p.fx[p.open[i]];
+    WallC &W = p.wall[fx.bidx];
+    const double pr = p.bpress[fx.bidx];
+    if (W.mode == 2.0) {
+        W.rho_b = pr / p.R / W.T;
+    } else if (W.mode == 3.0) {
+        const int c = fx.P;
+        const double Ti = p.cT[c], rhoi = p.crho[c];
+        const double ai = sqrt(p.R * Ti * (p.K + 5) / (p.K + 3));
+        W.rho_b = rhoi + (pr - rhoi * p.R * Ti) / ai / ai;
+        W.T = pr / (p.R * rhoi);
+        const double tp = 2.0 * PI * (p.R * W.T);
+        W.pre = D == 3 ? 1.0 / (tp * sqrt(tp)) : 1.0 / tp;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // k_cell_s1: S1 of the next step (discreteVelocity.C:393-406): gbar = (1 - r) g~ + r geq with the
 // new cell macros, in place in the output buffer.  One workgroup per cell over all velocities
 // (a pure stream of the row; the exp table is built once per cell).
@@ -1397,6 +1427,8 @@ struct dugks_engine {
     uint32_t *vidx = nullptr, *mir = nullptr;
     int has_sym = 0;     // symmetry, far-field or fixed patches present (they need face-value mode)
     int nopen = 0;       // number of such faces
+    int npress = 0;      // pressure-inlet / outlet faces among them
+    double *bpress = nullptr;
     int *open = nullptr; // their face indices
     double *axw = nullptr;
     double *axis = nullptr;
@@ -1506,6 +1538,7 @@ P<TS> params(dugks_engine *h)
     p.gBg_in = (const TS *)h->gB[h->cur]; p.gBh_in = (const TS *)h->hB[h->cur];
     p.gBg_out = (TS *)h->gB[1 - h->cur]; p.gBh_out = (TS *)h->hB[1 - h->cur];
     p.nopen = h->nopen;
+    p.bpress = h->bpress;
     p.open = h->open;
     p.fvg = h->face_values ? (TS *)h->fvg : nullptr;
     p.fvh = h->face_values ? (TS *)h->fvh : nullptr;
@@ -1547,6 +1580,8 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
     hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
     if (ev) (void)hipEventRecord(ev[4], h->stream);
     hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
+    if (h->npress)
+        hipLaunchKernelGGL((k_pressure_bc<D, TS>), dim3((h->nopen + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[5], h->stream);
     hipLaunchKernelGGL((k_cell_s1<D, H, TS>), dim3(h->nc), dim3(S1_NT), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[6], h->stream);
@@ -1788,7 +1823,7 @@ int build(dugks_engine *h, const dugks_case *cs)
 
     // ---- boundary bookkeeping ----
     std::vector<int> bkind(nb, 0), bpartner(nb, -1), bpatch(nb, -1), bwall(nb, -1), bsym(nb, -1);
-    std::vector<double> Ub(3 * (size_t)nb, 0.0), Tb(nb, 0.0), rhob(nb, 0.0);
+    std::vector<double> Ub(3 * (size_t)nb, 0.0), Tb(nb, 0.0), rhob(nb, 0.0), pressb(nb, 0.0);
     h->patch_type.assign(cs->n_patches, 0);
     h->patch_start.assign(cs->n_patches, 0);
     h->patch_size.assign(cs->n_patches, 0);
@@ -1799,12 +1834,15 @@ int build(dugks_engine *h, const dugks_case *cs)
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf)
             return fail(h, DUGKS_ERR_INVALID, "patch " + std::to_string(pi) + " face range outside the boundary");
         if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_SYMMETRY &&
-            pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_FIXED)
+            pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_FIXED &&
+            pp.type != DUGKS_PATCH_PRESSURE_IN && pp.type != DUGKS_PATCH_PRESSURE_OUT)
             return fail(h, DUGKS_ERR_UNSUPPORTED, "patch " + std::to_string(pi) + ": unsupported type");
-        const bool far = pp.type == DUGKS_PATCH_FARFIELD || pp.type == DUGKS_PATCH_FIXED;
+        const bool press = pp.type == DUGKS_PATCH_PRESSURE_IN || pp.type == DUGKS_PATCH_PRESSURE_OUT;
+        const bool far = pp.type == DUGKS_PATCH_FARFIELD || pp.type == DUGKS_PATCH_FIXED || press;
         if (far) {
-            if (!pp.rho || !pp.T || (pp.type == DUGKS_PATCH_FIXED && !pp.U))
-                return fail(h, DUGKS_ERR_INVALID, "far-field / fixed patch without rho, T (and U)");
+            if (!pp.T || (!press && !pp.rho) || (pp.type == DUGKS_PATCH_FIXED && !pp.U))
+                return fail(h, DUGKS_ERR_INVALID, "open patch without rho, T (and U)");
+            if (press && !(pp.pressure > 0)) return fail(h, DUGKS_ERR_INVALID, "pressure patch pressure must be > 0");
             h->has_sym = 1;
         }
         h->patch_type[pi] = pp.type; h->patch_start[pi] = pp.start; h->patch_size[pi] = pp.size;
@@ -1851,7 +1889,9 @@ int build(dugks_engine *h, const dugks_case *cs)
                 if (pp.type == DUGKS_PATCH_FIXED)
                     for (int d = 0; d < 3; ++d) Ub[3 * b + d] = pp.U[3 * i + d];
                 Tb[b] = pp.T[i];
-                rhob[b] = pp.rho[i];
+                // pressureIn / pressureOut rho patch values start at one (pIn.C:75)
+                rhob[b] = press ? 1.0 : pp.rho[i];
+                pressb[b] = press ? pp.pressure : 0.0;
                 if (!(Tb[b] > 0) || !(rhob[b] > 0)) return fail(h, DUGKS_ERR_INVALID, "boundary rho and T must be > 0");
                 bwall[b] = nw++;
             } else {
@@ -2184,6 +2224,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     // ---- wall constants (U, T, pre); the sums come from k_wall_init ----
     std::vector<WallC> wc(nw);
     std::vector<int> wfslot(nw);
+    std::vector<double> wpress(nw, 0.0);
     for (int b = 0; b < nb; ++b) {
         if (bwall[b] < 0) continue;
         WallC W;
@@ -2193,7 +2234,10 @@ int build(dugks_engine *h, const dugks_case *cs)
         double tp = 2.0 * PI * (cs->R * Tb[b]);
         W.pre = D == 3 ? 1.0 / (tp * std::sqrt(tp)) : 1.0 / tp;
         W.rho_b = rhob[b];
-        W.mode = bkind[b] == DUGKS_PATCH_FIXED ? 1.0 : 0.0;
+        W.mode = bkind[b] == DUGKS_PATCH_FIXED ? 1.0
+               : bkind[b] == DUGKS_PATCH_PRESSURE_IN ? 2.0 : bkind[b] == DUGKS_PATCH_PRESSURE_OUT ? 3.0 : 0.0;
+        wpress[bwall[b]] = pressb[b];
+        if (bkind[b] == DUGKS_PATCH_PRESSURE_IN || bkind[b] == DUGKS_PATCH_PRESSURE_OUT) ++h->npress;
         wc[bwall[b]] = W;
     }
     for (int i = 0; i < nw; ++i) {
@@ -2241,6 +2285,7 @@ int build(dugks_engine *h, const dugks_case *cs)
         (r = upload(h, h->corder, corder.data(), nc)) || (r = upload(h, h->forder, forder.data(), nf)) || (r = upload(h, h->wfslot, wfslot.data(), nw)))
         return r;
     if (nw && (r = upload(h, h->wall, wc.data(), nw))) return r;
+    if (nw && ((r = dalloc(h, &h->bpress, nw)) || (r = upload(h, h->bpress, wpress.data(), nw)))) return r;
     h->nopen = (int)open_faces.size();
     if (h->nopen && ((r = dalloc(h, &h->open, open_faces.size())) || (r = upload(h, h->open, open_faces.data(), open_faces.size()))))
         return r;
@@ -2269,7 +2314,7 @@ void release(dugks_engine *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->vidx); dfree(h->mir); dfree(h->open); dfree(h->axw); dfree(h->moff); dfree(h->axis);
+    dfree(h->vidx); dfree(h->mir); dfree(h->open); dfree(h->bpress); dfree(h->axw); dfree(h->moff); dfree(h->axis);
     dfree(h->cs); dfree(h->ccase); dfree(h->flag); dfree(h->wfslot); dfree(h->slots); dfree(h->cV);
     dfree(h->dtc); dfree(h->dtp); dfree(h->muc); dfree(h->crho); dfree(h->cU); dfree(h->cT); dfree(h->cq);
     dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->feq);
@@ -2545,7 +2590,7 @@ int dugks_init_equilibrium(dugks_engine *h, const double *rho, const double *U, 
                 const int f = s.face;
                 if (s.kind == KIND_WALL) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];
-                } else if (s.kind == KIND_SYM || (s.kind == KIND_FAR && wc[s.bidx].mode == 0.0)) {
+                } else if (s.kind == KIND_SYM || (s.kind == KIND_FAR && wc[s.bidx].mode != 1.0)) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = U[3 * c + d];   // U zeroGradient there
                 } else if (s.kind == KIND_FAR) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];
@@ -2726,6 +2771,7 @@ int dugks_state_size(dugks_engine *h, int64_t *bytes)
     n += nh * (int64_t)h->nc * h->nvp * es;
     n += nh * (int64_t)(h->nw ? h->nw : 1) * h->nvp * es;
     n += 9 * (int64_t)h->nc * 8 + 3 * (int64_t)h->nf * 8 + (int64_t)h->n_cases * 8;
+    n += (int64_t)h->nw * (int64_t)sizeof(WallC);   // pressure patches evolve their boundary state
     *bytes = n;
     return 0;
 }
@@ -2758,6 +2804,8 @@ int dugks_save_state(dugks_engine *h, void *buf, int64_t bytes)
     HIPCHK(h, hipMemcpy(p, h->ctau, 8 * nc, hipMemcpyDeviceToHost)); p += 8 * nc;
     HIPCHK(h, hipMemcpy(p, h->fU, 24 * (size_t)h->nf, hipMemcpyDeviceToHost)); p += 24 * (size_t)h->nf;
     std::memcpy(p, h->dt_prep.data(), 8 * h->n_cases);
+    p += 8 * h->n_cases;
+    if (h->nw) HIPCHK(h, hipMemcpy(p, h->wall, sizeof(WallC) * (size_t)h->nw, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -2791,6 +2839,8 @@ int dugks_load_state(dugks_engine *h, const void *buf, int64_t bytes)
     HIPCHK(h, hipMemcpy(h->ctau, p, 8 * nc, hipMemcpyHostToDevice)); p += 8 * nc;
     HIPCHK(h, hipMemcpy(h->fU, p, 24 * (size_t)h->nf, hipMemcpyHostToDevice)); p += 24 * (size_t)h->nf;
     std::memcpy(h->dt_prep.data(), p, 8 * h->n_cases);
+    p += 8 * h->n_cases;
+    if (h->nw) HIPCHK(h, hipMemcpy(h->wall, p, sizeof(WallC) * (size_t)h->nw, hipMemcpyHostToDevice));
     h->raw = (int)hdr[7];
     std::memcpy(&h->off_rho, &hdr[10], sizeof(double));
     std::memcpy(&h->off_T, &hdr[11], sizeof(double));

@@ -166,7 +166,7 @@ def read_field(path: str, n_cells: int, patch_sizes: Dict[str, int], vector: boo
         val = None
         if vm is not None and name in patch_sizes:
             val = _field_value(vm.group(1), patch_sizes[name], vector)
-        bf[name] = dict(type=typ, value=val)
+        bf[name] = dict(type=typ, value=val, entries=b)
     return internal, bf
 
 
@@ -310,6 +310,22 @@ def read_case(path: str, dt: float = None) -> cases.Case:
                 kw["U"] = U_b[b["name"]]["value"]
             kind = cases.PATCH_FARFIELD if rtype == "farField" else cases.PATCH_FIXED
             plist.append(cases.Patch(b["name"], kind, owner[sl], Sf_all[sl], Cf_all[sl], **kw))
+        elif rtype in ("pressureIn", "pressureOut"):
+            # farField DF patches whose rho (and, at the outlet, T) follow the patch pressure
+            # after every step (fvDVM.C:730-806); the pressure is the patch's own keyword
+            pk = _keyword(rho_b[b["name"]]["entries"], rtype)
+            if pk is None:
+                raise NotImplementedError(f"patch {b['name']}: {rtype} without its {rtype} keyword")
+            tb = T_b[b["name"]]
+            if tb["type"] == "fixedValue":
+                Tp = tb["value"]
+            elif rtype == "pressureOut" and tb["type"] == "zeroGradient":
+                Tp = T_i[owner[sl]]
+            else:
+                raise NotImplementedError(f"patch {b['name']}: {rtype} needs a fixedValue T")
+            kind = cases.PATCH_PRESSURE_IN if rtype == "pressureIn" else cases.PATCH_PRESSURE_OUT
+            plist.append(cases.Patch(b["name"], kind, owner[sl], Sf_all[sl], Cf_all[sl], T=Tp,
+                                     pressure=_number(pk)))
         elif rtype == "symmetryMod":
             # DVMsymmetry DF patch (bcMap, discreteVelocity.C:279): mirror plane
             plist.append(cases.Patch(b["name"], cases.PATCH_SYMMETRY, owner[sl], Sf_all[sl], Cf_all[sl]))

@@ -55,9 +55,15 @@ enum dugks_patch_type {
     DUGKS_PATCH_FARFIELD = 4,  /* rho farField, T fixedValue (gSurf farField): incoming
                                   values rho_b M(U of the owner cell, T_b)
                                   (discreteVelocity.C:574-604)                               */
-    DUGKS_PATCH_FIXED = 5      /* rho fixedValue (gSurf mixed): incoming values frozen at the
+    DUGKS_PATCH_FIXED = 5,     /* rho fixedValue (gSurf mixed): incoming values frozen at the
                                   boundary Maxwellian rho_b M(U_b, T_b)
                                   (discreteVelocity.C:312-344, 556-573, 1046-1060)           */
+    DUGKS_PATCH_PRESSURE_IN = 6,  /* rho pressureIn: a far field whose rho_b = p / (R T_b) is
+                                  reset after every step (fvDVM.C:743-770); rho_b starts at 1
+                                  as the patch field's constructor sets it                    */
+    DUGKS_PATCH_PRESSURE_OUT = 7  /* rho pressureOut: a far field whose rho_b and T_b follow the
+                                  boundary cells' characteristics after every step
+                                  (fvDVM.C:772-804); rho_b starts at 1                        */
 };
 
 /* Storage precision of the resident distributions. */
@@ -74,6 +80,7 @@ typedef struct dugks_patch {
     const double* U;           /* [size][3] boundary U (wall, fixed), else may be NULL          */
     const double* T;           /* [size]    boundary T (wall, far field, fixed), else NULL      */
     const double* rho;         /* [size]    boundary rho (far field, fixed), else NULL          */
+    double pressure;           /* pressureIn / pressureOut: the patch pressure (pIn.C:72)       */
 } dugks_patch;
 
 typedef struct dugks_case {

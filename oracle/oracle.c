@@ -19,6 +19,8 @@
  *      velocity's outgoing face value; gBarPvol fixedGradient as on walls)
  *   far-field (rho farField) and fixed (rho fixedValue, gSurf mixed) patches:
  *      incoming values rho_b M(U, T_b), discreteVelocity.C:556-604
+ *   S10 updatePressureInOutBC   fvDVM.C:730-806 (pressureIn / pressureOut patches,
+ *      far fields whose rho_b, T_b are reset after each step)
  *   S6 updateMacroSurf          fvDVM.C:456-582
  *   S7 updateGHsurf             discreteVelocity.C:819-932
  *   S8 updateGHtildeVol         discreteVelocity.C:934-978
@@ -80,6 +82,7 @@ typedef struct oracle {
     int *mirror;     /* [3][nv] velocity mirrored across axis a (fvDVM.C:152-166, 183-195) */
     double *Ub, *Tb; /* [nb*3], [nb] wall values             */
     double *rhob;    /* [nb] far-field / fixed boundary rho  */
+    double *pressb;  /* [nb] pressureIn / pressureOut: p     */
     double *nhat;    /* [nb*3] Sf/|Sf|                       */
     double *dcoef;   /* [nb] 1/|delta|                       */
     int n_patches; int *pstart, *psize, *ptype;
@@ -272,6 +275,7 @@ oracle *oracle_create(const dugks_case *cs)
     o->bpartner = (int *)xmalloc(sizeof(int)*(nb + 1));
     o->Ub = (double *)xmalloc(sizeof(double)*3*(nb + 1)); o->Tb = (double *)xmalloc(sizeof(double)*(nb + 1));
     o->rhob = (double *)xmalloc(sizeof(double)*(nb + 1));
+    o->pressb = (double *)xmalloc(sizeof(double)*(nb + 1));
     o->nhat = (double *)xmalloc(sizeof(double)*3*(nb + 1)); o->dcoef = (double *)xmalloc(sizeof(double)*(nb + 1));
     o->n_patches = cs->n_patches;
     o->pstart = (int *)xmalloc(sizeof(int)*(cs->n_patches + 1));
@@ -292,6 +296,10 @@ oracle *oracle_create(const dugks_case *cs)
                 for (int d = 0; d < 3; ++d) o->Ub[3*b + d] = pp->U ? pp->U[3*i + d] : 0.0;
                 o->Tb[b] = pp->T ? pp->T[i] : cs->T_ref;
                 o->rhob[b] = pp->rho ? pp->rho[i] : 0.0;
+                o->pressb[b] = pp->pressure;
+                /* pressureIn / pressureOut rho patch values start at one
+                 * (pressureInFvPatchField.C:75, pressureOutFvPatchField.C) */
+                if (pp->type == DUGKS_PATCH_PRESSURE_IN || pp->type == DUGKS_PATCH_PRESSURE_OUT) o->rhob[b] = 1.0;
             }
         }
     }
@@ -399,7 +407,7 @@ oracle *oracle_create(const dugks_case *cs)
 void oracle_destroy(oracle *o)
 {
     if (!o) return;
-    void *ptrs[] = {o->rhob, o->gt0, o->ht0, o->gradb0_g, o->gradb0_h, o->bmirror, o->mirror, o->owner, o->nei, o->C, o->V, o->Sf, o->Cf, o->w, o->delta, o->magSf, o->bkind, o->bpatch,
+    void *ptrs[] = {o->pressb, o->rhob, o->gt0, o->ht0, o->gradb0_g, o->gradb0_h, o->bmirror, o->mirror, o->owner, o->nei, o->C, o->V, o->Sf, o->Cf, o->w, o->delta, o->magSf, o->bkind, o->bpatch,
                     o->bpartner, o->Ub, o->Tb, o->nhat, o->dcoef, o->pstart, o->psize, o->ptype, o->pvec, o->nvec,
                     o->cstart, o->cface, o->xi, o->wv, o->cell_case, o->mu_case, o->dt_case, o->gt, o->ht,
                     o->gradb_g, o->gradb_h, o->rho, o->U, o->T, o->q, o->tau, o->incoming, o->rho_w, o->rho_f,
@@ -465,7 +473,7 @@ int oracle_init_equilibrium(oracle *o, const double *rho, const double *U, const
             if (o->bkind[b] == DUGKS_PATCH_CYCLIC) {
                 int N = o->owner[o->nfi + o->bpartner[b]];
                 o->U_f[3*f + d] = o->w[f]*o->U[3*o->owner[f] + d] + (1 - o->w[f])*o->U[3*N + d];
-            } else if (o->bkind[b] == DUGKS_PATCH_SYMMETRY || o->bkind[b] == DUGKS_PATCH_FARFIELD) {
+            } else if (o->bkind[b] != DUGKS_PATCH_WALL && o->bkind[b] != DUGKS_PATCH_FIXED) {
                 o->U_f[3*f + d] = o->U[3*o->owner[f] + d];   /* U zeroGradient on the patch */
             } else {
                 o->U_f[3*f + d] = o->Ub[3*b + d];
@@ -657,7 +665,8 @@ static void compute_k(oracle *o, int k, int want_faces)
                 } else {
                     o->gs[f] = 0.0; o->hs[f] = 0.0; /* set by S4 */
                 }
-            } else if (o->bkind[b] == DUGKS_PATCH_FARFIELD || o->bkind[b] == DUGKS_PATCH_FIXED) {
+            } else if (o->bkind[b] == DUGKS_PATCH_FARFIELD || o->bkind[b] == DUGKS_PATCH_FIXED ||
+                       o->bkind[b] == DUGKS_PATCH_PRESSURE_IN || o->bkind[b] == DUGKS_PATCH_PRESSURE_OUT) {
                 if (s > 0) {
                     o->gs[f] = o->gbar[own] + dot3(o->grad_g + 3*own, dO);
                     o->hs[f] = o->hbar[own] + dot3(o->grad_h + 3*own, dO);
@@ -665,7 +674,7 @@ static void compute_k(oracle *o, int k, int want_faces)
                     /* incoming: rho_b M(U, T_b), U the owner cell's velocity (farField,
                      * discreteVelocity.C:588-602) or the frozen boundary velocity (mixed,
                      * initBoundaryField, discreteVelocity.C:312-344) */
-                    const double *Uin = o->bkind[b] == DUGKS_PATCH_FARFIELD ? o->U + 3*own : o->Ub + 3*b;
+                    const double *Uin = o->bkind[b] == DUGKS_PATCH_FIXED ? o->Ub + 3*b : o->U + 3*own;
                     o->gs[f] = o->rhob[b]*maxwell_by_rho(o, Uin, o->Tb[b], xi);
                     o->hs[f] = o->gs[f]*(o->R*o->Tb[b])*(o->K + 3 - o->dim);
                 }
@@ -929,6 +938,19 @@ static int do_step(oracle *o)
             double dt = dt_of_cell(o, c);
             double fac = 2.0*o->tau[c]/(2.0*o->tau[c] + dt*o->Pr);
             for (int d = 0; d < 3; ++d) o->q[3*c + d] = fac*(double)o->qacc_c[3*c + d];
+        }
+    }
+    /* S10 updatePressureInOutBC (fvDVM.C:730-806): boundary rho (and T) of pressure patches from
+     * the boundary cells' new state, used by the next step's incoming Maxwellian */
+    for (int b = 0; b < nb; ++b) {
+        if (o->bkind[b] == DUGKS_PATCH_PRESSURE_IN) {
+            o->rhob[b] = o->pressb[b]/o->R/o->Tb[b];
+        } else if (o->bkind[b] == DUGKS_PATCH_PRESSURE_OUT) {
+            int i = o->owner[nfi + b];
+            double Ti = o->T[i], rhoi = o->rho[i];
+            double ai = sqrt(o->R*Ti*(K + 5)/(K + 3));
+            o->rhob[b] = rhoi + (o->pressb[b] - rhoi*o->R*Ti)/ai/ai;
+            o->Tb[b] = o->pressb[b]/(o->R*rhoi);
         }
     }
     (void)D;

@@ -10,7 +10,8 @@ import math
 
 import numpy as np
 
-from dugksfoam_amd.cases import PATCH_CYCLIC, PATCH_FARFIELD, PATCH_FIXED, PATCH_SYMMETRY, PATCH_WALL
+from dugksfoam_amd.cases import (PATCH_CYCLIC, PATCH_FARFIELD, PATCH_FIXED, PATCH_PRESSURE_IN, PATCH_PRESSURE_OUT,
+                                 PATCH_SYMMETRY, PATCH_WALL)
 
 
 class NumpyDUGKS:
@@ -30,12 +31,17 @@ class NumpyDUGKS:
         self.Tb = np.ones(self.nb)
         self.axis = np.zeros(self.nb, np.int64)
         self.rhob = np.zeros(self.nb)
+        self.press = np.zeros(self.nb)
         for p in c.patches:
             b = np.arange(p["start"], p["start"] + p["size"]) - nfi
             self.kind[b] = p["type"]
             if p["type"] == PATCH_WALL:
                 self.Ub[b] = p["U"]
                 self.Tb[b] = p["T"]
+            elif p["type"] in (PATCH_PRESSURE_IN, PATCH_PRESSURE_OUT):
+                self.Tb[b] = p["T"]
+                self.rhob[b] = 1.0            # the pressure patch fields start at one
+                self.press[b] = p["pressure"]
             elif p["type"] in (PATCH_FARFIELD, PATCH_FIXED):
                 self.Tb[b] = p["T"]
                 self.rhob[b] = p["rho"]
@@ -55,7 +61,7 @@ class NumpyDUGKS:
         self.mirror = np.stack([(iz * n + iy) * n + (n - 1 - ix), (iz * n + (n - 1 - iy)) * n + ix,
                                 ((nz - 1 - iz) * n + iy) * n + ix])
         self.sym = np.nonzero(self.kind == PATCH_SYMMETRY)[0]
-        self.far = np.nonzero((self.kind == PATCH_FARFIELD) | (self.kind == PATCH_FIXED))[0]
+        self.far = np.nonzero(np.isin(self.kind, (PATCH_FARFIELD, PATCH_FIXED, PATCH_PRESSURE_IN, PATCH_PRESSURE_OUT)))[0]
         self.wall = np.nonzero(self.kind == PATCH_WALL)[0]
         self.cyc = np.nonzero(self.kind == PATCH_CYCLIC)[0]
         self.bP = self.own[nfi:]
@@ -194,7 +200,7 @@ class NumpyDUGKS:
         if self.far.size:
             # far field: rho_b M(U of the outlet cell, T_b); fixed: rho_b M(U_b, T_b)
             fb = self.far
-            Uin = np.where((self.kind[fb] == PATCH_FARFIELD)[:, None], self.U[self.bP[fb]], self.Ub[fb])
+            Uin = np.where((self.kind[fb] == PATCH_FIXED)[:, None], self.Ub[fb], self.U[self.bP[fb]])
             Mf = self.rhob[fb][None, :] * self._maxwell_by_rho(Uin, self.Tb[fb])
             gb[:, fb] = np.where(sb[:, fb] > 0, gP[:, fb], Mf)
             hb[:, fb] = np.where(sb[:, fb] > 0, hP[:, fb], Mf * (R * self.Tb[fb])[None, :] * (K + 3 - D))
@@ -258,6 +264,15 @@ class NumpyDUGKS:
         q = 0.5 * np.einsum("k,kcd,kc->cd", w, cc, np.sum(cc ** 2, axis=2) * self.gt + self.ht)
         q = (2 * tau / (2 * tau + dt * Pr))[:, None] * q
         self.rho, self.U, self.T, self.q, self.tau = rho, U, T, q, tau
+        # S10 updatePressureInOutBC (fvDVM.C:730-806)
+        pin = self.kind == PATCH_PRESSURE_IN
+        self.rhob[pin] = self.press[pin] / R / self.Tb[pin]
+        pout = np.nonzero(self.kind == PATCH_PRESSURE_OUT)[0]
+        if pout.size:
+            Ti, rhoi = T[self.bP[pout]], rho[self.bP[pout]]
+            ai = np.sqrt(R * Ti * (K + 5) / (K + 3))
+            self.rhob[pout] = rhoi + (self.press[pout] - rhoi * R * Ti) / ai / ai
+            self.Tb[pout] = self.press[pout] / (R * rhoi)
 
     def macro(self):
         return dict(rho=self.rho, U=self.U, T=self.T, q=self.q)

@@ -276,3 +276,28 @@ def test_open_boundaries_read_back(tmp_path):
     for p, q in zip(r.patches, c.patches):
         if q["rho"] is not None:
             np.testing.assert_array_equal(p["rho"], q["rho"])
+
+
+def test_pressure_patches_read_back(tmp_path):
+    """rho pressureIn / pressureOut patches (with their pressure keywords, pIn.C:72) read back as
+    the pressure channel's inlet and outlet; a zeroGradient outlet T takes the cells' T."""
+    c = cases.pressure_channel(8, 4, nv=8)
+    wall = foam_box.wall((0, 0, 0), c.T0[0])
+    inlet, outlet = c.patches[1], c.patches[2]
+    empty = "        type empty;\n"
+    fields = dict(
+        rho=(f"uniform {float(c.rho0[0])!r}", {
+            "walls": wall["rho"], "frontAndBack": empty,
+            "inlet": f"        type pressureIn;\n        pressureIn {inlet['pressure']!r};\n        value uniform 1;\n",
+            "outlet": f"        type pressureOut;\n        pressureOut {outlet['pressure']!r};\n        value uniform 1;\n"}),
+        U=("uniform (0 0 0)", {"walls": wall["U"], "frontAndBack": empty,
+                               "inlet": "        type zeroGradient;\n", "outlet": "        type zeroGradient;\n"}),
+        T=(f"uniform {float(c.T0[0])!r}", {"walls": wall["T"], "frontAndBack": empty,
+                                           "inlet": wall["T"], "outlet": "        type zeroGradient;\n"}))
+    patches = [("walls", "wall", ["y0", "y1"], ""), ("inlet", "patch", ["x0"], ""),
+               ("outlet", "patch", ["x1"], ""), ("frontAndBack", "empty", ["z0", "z1"], "")]
+    foam_box.write_box(str(tmp_path), 8, 4, 0, L=(2.0, 1.0, 1.0), patches=patches, fields=fields,
+                       dvm=foam_box.dvm_props(c), xi=c.xi, w=c.xi_weights, control=foam_box.control(c.dt))
+    r = foam.read_case(str(tmp_path))
+    _same_case(r, c)
+    assert [p["pressure"] for p in r.patches] == [p["pressure"] for p in c.patches]

@@ -43,6 +43,7 @@ def _small_cases():
         "symcavity_odd": (lambda: cases.symmetry_cavity(5, 10, nv=9), 8),
         "symcavity_left": (lambda: cases.symmetry_cavity(5, 10, nv=10, plane="x0"), 8),
         "openchannel_12x6": (lambda: cases.open_channel(12, 6, nv=12), 10),
+        "pchannel_12x6": (lambda: cases.pressure_channel(12, 6, nv=12), 10),
     }
 
 
@@ -154,8 +155,12 @@ def test_dt_change_and_steps_split():
         assert rel_l2(me[k], mo[k]) <= FP64_TOL, k
 
 
-def test_save_load_resume_bit_exact():
-    c = cases.cavity(10, 13)
+@pytest.mark.parametrize("mk", [lambda: cases.cavity(10, 13), lambda: cases.pressure_channel(10, 5, nv=10)],
+                         ids=["cavity", "pressure_channel"])
+def test_save_load_resume_bit_exact(mk):
+    """The state image holds everything a step reads, including the boundary state that
+    pressure patches carry from step to step."""
+    c = mk()
     e = Engine(c)
     e.init_equilibrium()
     e.step(nsteps=4)
@@ -332,7 +337,7 @@ def test_face_value_mode(which):
     switch on a running engine."""
     mk, n = _small_cases()[which]
     c = mk()
-    if any(p["type"] in (cases.PATCH_SYMMETRY, cases.PATCH_FARFIELD, cases.PATCH_FIXED) for p in c.patches):
+    if any(p["type"] not in (cases.PATCH_WALL, cases.PATCH_CYCLIC) for p in c.patches):
         # symmetry, far-field and fixed patches are handled in face-value mode only
         e = Engine(c)
         assert e.face_values()

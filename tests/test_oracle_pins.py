@@ -246,7 +246,7 @@ def test_independent_numpy_restatement(oracle_lib):
     from ref_numpy import NumpyDUGKS
     for c in (cases.cavity(5, 9), cases.cavity(3, 8, dim=3, kn=0.1), cases.channel(6, 4, 1.0, 0.6, nv=8),
               cases.symmetry_cavity(4, 6, nv=9), cases.symmetry_cavity(3, 6, nv=8, plane="x0"),
-              cases.open_channel(6, 4, nv=8)):
+              cases.open_channel(6, 4, nv=8), cases.pressure_channel(6, 4, nv=8)):
         o = _oracle(c)
         o.init_equilibrium()
         r = NumpyDUGKS(c)
