@@ -10,7 +10,8 @@ read_case(path) builds a Case (dugksfoam_amd.cases.Case) from the files a dugksF
                                   open boundaries with an incoming Maxwellian; empty patches are
                                   omitted (fvPatch size 0); cyclic patches are coupled pairs
   constant/DVMProperties          fvDVMparas {nDV, xiMax, xiMin}, gasProperties
-                                  {R, omega, Tref, muRef, Pr, KInner = 0} (fvDVM.C:913-933)
+                                  {R, omega, Tref, muRef, Pr, KInner = 0} (fvDVM.C:913-933),
+                                  DFwriteCellList (fvDVM.C:1067)
   constant/Xis, constant/weights  the 1-D discrete-velocity set (fvDVM.C:56-117)
   system/controlDict              deltaT, adjustTimeStep, maxCo
 
@@ -367,7 +368,9 @@ def read_case(path: str, dt: float = None) -> cases.Case:
     case.rho0 = np.ascontiguousarray(rho_i, dtype=np.float64)
     case.U0 = np.ascontiguousarray(U_i, dtype=np.float64)
     case.T0 = np.ascontiguousarray(T_i, dtype=np.float64)
-    case.meta.update(source=os.path.abspath(path), controls=ctl,
+    m = re.search(r"DFwriteCellList\s*(\d*)\s*\(([^)]*)\)", props)
+    df_cells = [int(x) for x in m.group(2).split()] if m else []
+    case.meta.update(source=os.path.abspath(path), controls=ctl, df_cells=df_cells,
                      wall_patches=[q.name for q in plist if q.type == cases.PATCH_WALL],
                      boundary=bnd)
     return case

@@ -119,7 +119,8 @@ class Solver:
         return self.macro
 
     def write(self):
-        """<case>/<time>/{rho,U,T,q}; wall patches carry rho_w, the wall U and T, and qWall."""
+        """<case>/<time>/{rho,U,T,q}; wall patches carry rho_w, the wall U and T, and qWall; the
+        other non-coupled patches carry their boundary cells' values (calculated)."""
         m = self.fields()
         tn = time_name(self.t, self.ctl["timePrecision"])
         d = os.path.join(self.case_dir, tn)
@@ -137,6 +138,11 @@ class Solver:
                 for k in bf:
                     bf[k][name] = ("cyclic", None)
                 continue
+            if p["type"] != cases.PATCH_WALL:
+                cells = self.case.owner[p["start"]:p["start"] + p["size"]]
+                for k in bf:
+                    bf[k][name] = ("calculated", m[k][cells])
+                continue
             w = self.b.wall(live[name])
             bf["rho"][name] = ("calculatedMaxwell", w["rho_w"])
             bf["U"][name] = ("fixedValue", p["U"])
@@ -144,7 +150,22 @@ class Solver:
             bf["q"][name] = ("calculated", w["q_wall"])
         for k, vec in (("rho", False), ("U", True), ("T", False), ("q", True)):
             write_field(os.path.join(d, k), k, m[k], bf[k], vec, location=tn)
+        self.write_df()
         return d
+
+    def write_df(self):
+        """writeDFonCells (fvDVM.C:877-882, 818-875): for entry i of DFwriteCellList the
+        reference writes g~ of cell i -- the list position, not the listed cell id -- for every
+        velocity (reference order) to <case>/0/DF<i>, overwriting it at each output time."""
+        n = len(self.case.meta.get("df_cells", []))
+        if not n:
+            return
+        g, _ = self.b.df(np.arange(n))
+        for i in range(n):
+            body = f"{g.shape[1]}\n(\n" + "\n".join(repr(float(v)) for v in g[i]) + "\n)\n"
+            with open(os.path.join(self.case_dir, "0", f"DF{i}"), "w") as f:
+                f.write("FoamFile\n{\n    version     2.0;\n    format      ascii;\n"
+                        f"    class       scalarList;\n    location    \"0\";\n    object      DF{i};\n}}\n\n" + body)
 
     def run(self, max_steps: Optional[int] = None, write: bool = True) -> int:
         every = int(self.ctl["writeInterval"]) if self.ctl["writeControl"] == "timeStep" else 0

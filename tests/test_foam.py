@@ -301,3 +301,37 @@ def test_pressure_patches_read_back(tmp_path):
     r = foam.read_case(str(tmp_path))
     _same_case(r, c)
     assert [p["pressure"] for p in r.patches] == [p["pressure"] for p in c.patches]
+
+
+def test_solver_writes_open_patches_and_df(tmp_path, oracle_lib):
+    """Time directories of a case with symmetry and far-field patches, and the DF dump of
+    DFwriteCellList (writeDFonCells: entry i writes cell i, fvDVM.C:877-882)."""
+    c = cases.symmetry_cavity(4, 8, nv=8)
+    lid, fixed = foam_box.wall(c.patches[0]["U"][0], c.T0[0]), foam_box.wall((0, 0, 0), c.T0[0])
+    sym = {"rho": "        type symmetryMod;\n        value uniform 1;\n", "U": "        type zeroGradient;\n",
+           "T": "        type zeroGradient;\n"}
+    fields = {}
+    for k in ("rho", "U", "T"):
+        internal = {"rho": f"uniform {float(c.rho0[0])!r}", "U": "uniform (0 0 0)", "T": f"uniform {float(c.T0[0])!r}"}[k]
+        fields[k] = (internal, {"movingWall": lid[k], "fixedWalls": fixed[k], "symmetryWall": sym[k],
+                                "frontAndBack": "        type empty;\n"})
+    patches = [("movingWall", "wall", ["y1"], ""), ("fixedWalls", "wall", ["x0", "y0"], ""),
+               ("symmetryWall", "wall", ["x1"], ""), ("frontAndBack", "empty", ["z0", "z1"], "")]
+    dvm = foam_box.dvm_props(c) + "\nDFwriteCellList (5 7);\n"
+    foam_box.write_box(str(tmp_path), 4, 8, 0, L=(0.5, 1.0, 1.0), patches=patches, fields=fields, dvm=dvm,
+                       xi=c.xi, w=c.xi_weights, control=foam_box.control(c.dt, writeInterval=2))
+    case = foam.read_case(str(tmp_path))
+    assert case.meta["df_cells"] == [5, 7]
+    s = solver.Solver(str(tmp_path), backend=_OracleBackend(case, oracle_lib), log=None)
+    s.b.df = s.b.o.df
+    s.run(max_steps=4)
+    g, _ = s.b.o.df([0, 1])
+    for i in range(2):
+        d = foam.read_labels_float(str(tmp_path / "0" / f"DF{i}"))
+        np.testing.assert_array_equal(d, g[i])
+    sizes = {p["name"]: p["nFaces"] for p in case.meta["boundary"]}
+    tn = solver.time_name(s.t, 9)
+    rho, rb = foam.read_field(str(tmp_path / tn / "rho"), case.n_cells, sizes, vector=False)
+    assert rb["symmetryWall"]["type"] == "calculated"
+    p = case.patches[2]
+    np.testing.assert_array_equal(rb["symmetryWall"]["value"], rho[case.owner[p["start"]:p["start"] + p["size"]]])
