@@ -1,0 +1,96 @@
+"""Mass-flow objective of the periodic micro-channel and its gradient with respect to the
+lower-wall temperature coefficients.
+
+The workload: cases.channel (200 x 40 cells, 32^2 velocities, cyclic in x, lower wall at
+T(x) = T0 (1 + sum_k a_k sin(2 pi k x / L)), upper wall at T0), run for N steps from rest; the
+objective J(a) is the mass flow through the plane x = L/2 averaged over the last M steps,
+
+    J = (1/M) sum_{n > N - M} sum_{f on the plane} rho_f (U_f . S_f),
+
+with rho_f, U_f the face macros of S6 (fvDVM.C:473-495, dugks_get_face_macro).
+
+The gradient dJ/da_k is formed by central differences of the FP64 engine,
+(J(a + h e_k) - J(a - h e_k)) / 2h, i.e. 2 n_modes forward runs.  A hand-written reverse-mode
+(discrete adjoint) sweep is not part of this module.  MassFlow wraps J in a
+torch.autograd.Function whose backward returns that gradient.
+"""
+from __future__ import annotations
+
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from . import cases
+
+
+def plane_faces(case: cases.Case, x: float, axis: int = 0) -> np.ndarray:
+    """Internal faces lying on the plane x_axis = x with their normal along that axis."""
+    nfi = case.n_internal_faces
+    cf = case.Cf[:nfi, axis]
+    sf = case.Sf[:nfi]
+    on = np.abs(cf - x) <= 1e-9 * max(1.0, abs(x))
+    along = np.abs(sf[:, axis]) > 0.5 * np.linalg.norm(sf, axis=1)
+    return np.nonzero(on & along)[0]
+
+
+def mass_flow(case: cases.Case, steps: int, avg_last: int, backend: Optional[Callable] = None) -> float:
+    """J for one case: steps - avg_last steps, then the plane mass flow after each of the last
+    avg_last steps, averaged.  backend(case) returns an object with the Engine methods
+    init_equilibrium / step / face_macro / close (default: the HIP engine, FP64)."""
+    if backend is None:
+        from .engine import Engine
+        backend = Engine
+    faces = plane_faces(case, 0.5 * case.meta["Lx"])
+    if faces.size == 0:
+        raise ValueError("no faces on the mid plane")
+    S = case.Sf[faces]
+    e = backend(case)
+    try:
+        e.init_equilibrium()
+        if steps > avg_last:
+            e.step(nsteps=steps - avg_last)
+        acc = 0.0
+        for _ in range(avg_last):
+            e.step(nsteps=1)
+            fm = e.face_macro()
+            acc += float(np.sum(fm["rho"][faces] * np.einsum("ij,ij->i", fm["U"][faces], S)))
+        return acc / avg_last
+    finally:
+        close = getattr(e, "close", None)
+        if close:
+            close()
+
+
+def gradient(a: Sequence[float], steps: int = 2000, avg_last: int = 500, h: float = 1e-6,
+             backend: Optional[Callable] = None, **channel_kw):
+    """(J(a), dJ/da) for the channel with coefficients a (central differences, step h)."""
+    a = np.asarray(a, dtype=np.float64)
+    J = mass_flow(cases.channel(a=a, n_modes=a.size, **channel_kw), steps, avg_last, backend)
+    g = np.zeros_like(a)
+    for k in range(a.size):
+        ap, am = a.copy(), a.copy()
+        ap[k] += h
+        am[k] -= h
+        Jp = mass_flow(cases.channel(a=ap, n_modes=a.size, **channel_kw), steps, avg_last, backend)
+        Jm = mass_flow(cases.channel(a=am, n_modes=a.size, **channel_kw), steps, avg_last, backend)
+        g[k] = (Jp - Jm) / (2.0 * h)
+    return J, g
+
+
+def mass_flow_function(steps: int = 2000, avg_last: int = 500, h: float = 1e-6, **channel_kw):
+    """A torch.autograd.Function computing J(a) (forward) and dJ/da (backward)."""
+    import torch
+
+    class MassFlow(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, a):
+            J, g = gradient(a.detach().cpu().numpy(), steps, avg_last, h, **channel_kw)
+            ctx.save_for_backward(torch.as_tensor(g, dtype=a.dtype))
+            return torch.as_tensor(J, dtype=a.dtype)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            (g,) = ctx.saved_tensors
+            return grad_out * g
+
+    return MassFlow
