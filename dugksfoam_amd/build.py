@@ -2,7 +2,8 @@
 
     python -m dugksfoam_amd.build
 
-produces dugksfoam_amd/_build/libdugks.so from dugksfoam_amd/csrc/*.hip.
+produces dugksfoam_amd/_build/libdugks.so from dugksfoam_amd/csrc/*.hip (the step engine and the
+discrete adjoint).
 """
 from __future__ import annotations
 
@@ -13,7 +14,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "dugks_engine.hip")
+SRCS = [os.path.join(HERE, "csrc", n) for n in ("dugks_engine.hip", "dugks_adjoint.hip")]
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libdugks.so")
 ARCH = os.environ.get("DUGKS_OFFLOAD_ARCH", "gfx950")
@@ -41,7 +42,7 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *SRCS, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -494,7 +494,7 @@ def channel(nx: int = 200, ny: int = 40, Lx: float = 5.0, Ly: float = 1.0, nv: i
     mu = mu_from_kn(kn, R, T0, rho0)
     dt = courant_dt(max(nx / Lx, ny / Ly), d, xi_max, co)
     return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
-                   meta=dict(kn=kn, a=a, nx=nx, ny=ny, Lx=Lx))
+                   meta=dict(kn=kn, a=a, nx=nx, ny=ny, Lx=Lx, T0=T0))
 
 
 def hard_core_points(n: int, r: float, fixed: np.ndarray, rng, margin: float) -> np.ndarray:

@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "../../include/dugks.h"
+#include "lsq.h"
 
 namespace {
 
@@ -1794,20 +1795,7 @@ void shape_candidates(dugks_engine *h)
     h->nbc_cap = h->nb_cap = std::max(h->nb_cap, h->nbc_cap);
 }
 
-double mag3(const double *a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
-
-// inverse of a symmetric tensor (xx, xy, xz, yy, yz, zz) by cofactors
-void symm_inverse(const double *s, double *r)
-{
-    double det = s[0] * s[3] * s[5] + s[1] * s[4] * s[2] + s[2] * s[1] * s[4] - s[0] * s[4] * s[4] -
-                 s[1] * s[1] * s[5] - s[2] * s[3] * s[2];
-    r[0] = (s[3] * s[5] - s[4] * s[4]) / det;
-    r[1] = (s[2] * s[4] - s[1] * s[5]) / det;
-    r[2] = (s[1] * s[4] - s[2] * s[3]) / det;
-    r[3] = (s[0] * s[5] - s[2] * s[2]) / det;
-    r[4] = (s[1] * s[2] - s[0] * s[4]) / det;
-    r[5] = (s[0] * s[3] - s[1] * s[1]) / det;
-}
+using dugks_lsq::mag3;
 
 int build(dugks_engine *h, const dugks_case *cs)
 {
@@ -1916,83 +1904,10 @@ int build(dugks_engine *h, const dugks_case *cs)
         if (!(mag3(cs->Sf + 3 * f) > 0)) return fail(h, DUGKS_ERR_INVALID, "zero-area face");
 
     // ---- least-squares vectors (leastSquaresVectors) ----
-    std::vector<double> dd(6 * (size_t)nc, 0.0), pvec(3 * (size_t)nf), nvec(3 * (size_t)nfi);
-    std::vector<double> magSf(nf);
+    std::vector<double> pvec, nvec, magSf(nf);
     for (int f = 0; f < nf; ++f) magSf[f] = mag3(cs->Sf + 3 * f);
-    // wdd = (|Sf|/|d|^2) d d; dd_own += (1 - w) wdd; dd_nei += w wdd (leastSquaresVectors order)
-    auto outer_add = [&](int cell, double m, const double *d) {
-        double *t = &dd[6 * (size_t)cell];
-        t[0] += m * (d[0] * d[0]); t[1] += m * (d[0] * d[1]); t[2] += m * (d[0] * d[2]);
-        t[3] += m * (d[1] * d[1]); t[4] += m * (d[1] * d[2]); t[5] += m * (d[2] * d[2]);
-    };
+    if (const char *why = dugks_lsq::vectors(cs, bkind, pvec, nvec)) return fail(h, DUGKS_ERR_INVALID, why);
     const double *Cc = cs->C, *w = cs->weights;
-    for (int f = 0; f < nfi; ++f) {
-        int o = cs->owner[f], n = cs->neighbour[f];
-        double d[3] = {Cc[3 * n] - Cc[3 * o], Cc[3 * n + 1] - Cc[3 * o + 1], Cc[3 * n + 2] - Cc[3 * o + 2]};
-        double m = magSf[f] / (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        const double wdd[6] = {m * (d[0] * d[0]), m * (d[0] * d[1]), m * (d[0] * d[2]),
-                               m * (d[1] * d[1]), m * (d[1] * d[2]), m * (d[2] * d[2])};
-        double *to = &dd[6 * (size_t)o], *tn = &dd[6 * (size_t)n];
-        for (int i = 0; i < 6; ++i) {
-            to[i] += (1 - w[f]) * wdd[i];
-            tn[i] += w[f] * wdd[i];
-        }
-    }
-    for (int b = 0; b < nb; ++b) {
-        int f = nfi + b;
-        const double *d = cs->delta + 3 * b;
-        double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (!(d2 > 0)) return fail(h, DUGKS_ERR_INVALID, "zero boundary delta");
-        double m = bkind[b] == DUGKS_PATCH_CYCLIC ? (1 - w[f]) * magSf[f] / d2 : magSf[f] / d2;
-        outer_add(cs->owner[f], m, d);
-    }
-    std::vector<double> inv(6 * (size_t)nc);
-    {
-        const double *s0 = &dd[0];
-        double scale = s0[0] * s0[0] + 2 * s0[1] * s0[1] + 2 * s0[2] * s0[2] + s0[3] * s0[3] + 2 * s0[4] * s0[4] +
-                       s0[5] * s0[5];
-        bool rm[3] = {s0[0] * s0[0] / scale < 1e-15, s0[3] * s0[3] / scale < 1e-15, s0[5] * s0[5] / scale < 1e-15};
-        for (int c = 0; c < nc; ++c) {
-            double s[6];
-            std::memcpy(s, &dd[6 * (size_t)c], sizeof s);
-            if (rm[0]) s[0] += 1;
-            if (rm[1]) s[3] += 1;
-            if (rm[2]) s[5] += 1;
-            double *r = &inv[6 * (size_t)c];
-            symm_inverse(s, r);
-            if (rm[0]) r[0] -= 1;
-            if (rm[1]) r[3] -= 1;
-            if (rm[2]) r[5] -= 1;
-            for (int i = 0; i < 6; ++i)
-                if (!std::isfinite(r[i])) return fail(h, DUGKS_ERR_INVALID, "singular least-squares tensor");
-        }
-    }
-    auto sdot = [](const double *s, const double *d, double *r) {
-        r[0] = s[0] * d[0] + s[1] * d[1] + s[2] * d[2];
-        r[1] = s[1] * d[0] + s[3] * d[1] + s[4] * d[2];
-        r[2] = s[2] * d[0] + s[4] * d[1] + s[5] * d[2];
-    };
-    for (int f = 0; f < nfi; ++f) {
-        int o = cs->owner[f], n = cs->neighbour[f];
-        double d[3] = {Cc[3 * n] - Cc[3 * o], Cc[3 * n + 1] - Cc[3 * o + 1], Cc[3 * n + 2] - Cc[3 * o + 2]};
-        double m = magSf[f] / (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        double a[3], bb[3];
-        sdot(&inv[6 * (size_t)o], d, a);
-        sdot(&inv[6 * (size_t)n], d, bb);
-        for (int i = 0; i < 3; ++i) {
-            pvec[3 * f + i] = (1 - w[f]) * m * a[i];
-            nvec[3 * f + i] = -w[f] * m * bb[i];
-        }
-    }
-    for (int b = 0; b < nb; ++b) {
-        int f = nfi + b;
-        const double *d = cs->delta + 3 * b;
-        double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        double a[3];
-        sdot(&inv[6 * (size_t)cs->owner[f]], d, a);
-        double m = bkind[b] == DUGKS_PATCH_CYCLIC ? ((1 - w[f]) * magSf[f] / d2) : magSf[f] * (1.0 / d2);
-        for (int i = 0; i < 3; ++i) pvec[3 * f + i] = m * a[i];
-    }
 
     // ---- cell slots (faces ascending) ----
     std::vector<int> cnt(nc + 1, 0);

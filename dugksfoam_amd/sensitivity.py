@@ -9,10 +9,13 @@ objective J(a) is the mass flow through the plane x = L/2 averaged over the last
 
 with rho_f, U_f the face macros of S6 (fvDVM.C:473-495, dugks_get_face_macro).
 
-The gradient dJ/da_k is formed by central differences of the FP64 engine,
-(J(a + h e_k) - J(a - h e_k)) / 2h, i.e. 2 n_modes forward runs.  A hand-written reverse-mode
-(discrete adjoint) sweep is not part of this module.  MassFlow wraps J in a
-torch.autograd.Function whose backward returns that gradient.
+Two ways to the gradient dJ/da_k:
+  adjoint_gradient  the discrete adjoint (dugksfoam_amd.adjoint): one forward and one reverse
+                    sweep give dJ/dT_w on every wall face, chained through
+                    dT_w/da_k = T0 sin(2 pi k x_f / L) on the lower wall;
+  gradient          central differences of the FP64 engine, (J(a + h e_k) - J(a - h e_k)) / 2h,
+                    i.e. 2 n_modes forward runs.
+mass_flow_function wraps J in a torch.autograd.Function whose backward returns either.
 """
 from __future__ import annotations
 
@@ -77,14 +80,66 @@ def gradient(a: Sequence[float], steps: int = 2000, avg_last: int = 500, h: floa
     return J, g
 
 
-def mass_flow_function(steps: int = 2000, avg_last: int = 500, h: float = 1e-6, **channel_kw):
-    """A torch.autograd.Function computing J(a) (forward) and dJ/da (backward)."""
+def objective_coef(case: cases.Case, avg_last: int) -> np.ndarray:
+    """Per-face weights c_f of the mid-plane mass flow averaged over avg_last steps."""
+    faces = plane_faces(case, 0.5 * case.meta["Lx"])
+    if faces.size == 0:
+        raise ValueError("no faces on the mid plane")
+    coef = np.zeros(case.n_faces)
+    coef[faces] = 1.0 / avg_last
+    return coef
+
+
+def dTw_da(case: cases.Case) -> tuple:
+    """(boundary-face indices b of the lower wall, dT_w(b)/da_k [n_b, n_modes])."""
+    nfi = case.n_internal_faces
+    low = next(p for p in case.patches if p["name"] == "lowerWall")
+    b = np.arange(low["start"], low["start"] + low["size"]) - nfi
+    xb = case.Cf[nfi + b, 0]
+    k = np.arange(1, np.asarray(case.meta["a"]).size + 1)
+    return b, case.meta["T0"] * np.sin(2 * np.pi * k[None, :] * xb[:, None] / case.meta["Lx"])
+
+
+def adjoint_gradient(a: Sequence[float], steps: int = 2000, avg_last: int = 500, backend: Optional[Callable] = None,
+                     checkpoint_every: int = 100, **channel_kw):
+    """(J(a), dJ/da) from one forward and one reverse sweep of the discrete adjoint.
+    backend(case) returns an object with the dugksfoam_amd.adjoint.Adjoint run method."""
+    if backend is None:
+        from .adjoint import Adjoint
+        backend = Adjoint
+    a = np.asarray(a, dtype=np.float64)
+    c = cases.channel(a=a, n_modes=a.size, **channel_kw)
+    if not 1 <= avg_last <= steps:
+        raise ValueError("need 1 <= avg_last <= steps")
+    adj = backend(c)
+    try:
+        J, g = adj.run(steps, steps - avg_last + 1, objective_coef(c, avg_last),
+                       checkpoint_every=min(checkpoint_every, steps))
+    finally:
+        close = getattr(adj, "close", None)
+        if close:
+            close()
+    b, dT = dTw_da(c)
+    return J, dT.T @ g[b]
+
+
+def mass_flow_function(steps: int = 2000, avg_last: int = 500, h: float = 1e-6, method: str = "adjoint",
+                       **channel_kw):
+    """A torch.autograd.Function computing J(a) (forward) and dJ/da (backward), the gradient
+    from the discrete adjoint (method "adjoint") or central differences (method "fd")."""
     import torch
+
+    if method not in ("adjoint", "fd"):
+        raise ValueError("method must be 'adjoint' or 'fd'")
 
     class MassFlow(torch.autograd.Function):
         @staticmethod
         def forward(ctx, a):
-            J, g = gradient(a.detach().cpu().numpy(), steps, avg_last, h, **channel_kw)
+            an = a.detach().cpu().numpy()
+            if method == "adjoint":
+                J, g = adjoint_gradient(an, steps, avg_last, **channel_kw)
+            else:
+                J, g = gradient(an, steps, avg_last, h, **channel_kw)
             ctx.save_for_backward(torch.as_tensor(g, dtype=a.dtype))
             return torch.as_tensor(J, dtype=a.dtype)
 

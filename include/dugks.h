@@ -215,6 +215,36 @@ void dugks_destroy(dugks_engine* h);
 const char* dugks_last_error(dugks_engine* h);
 int dugks_abi_version(void);
 
+/* ---- Discrete adjoint (no reference counterpart) ----------------------------------------
+ * Gradient of a time-averaged mass-flux objective with respect to the wall temperatures:
+ *
+ *   J = sum over steps n = first_step .. n_steps of  sum_f c_f (rho U)_f^n . S_f,
+ *
+ * (rho U)_f^n = sum_k w_k xi_k g_f,k the face momentum of step n (the S6 face moments of
+ * fvDVM.C:473-483, i.e. dugks_get_face_macro's rho * U after that step), c_f = face_coef[f].
+ * dJ_dTw[b] = dJ/dT_w of boundary face n_internal_faces + b (0 on cyclic faces).
+ * The run starts from the Shakhov equilibrium of rho, U, T with q = 0 (as
+ * dugks_init_equilibrium) and advances n_steps FP64 steps of size dt, keeping a checkpoint every
+ * checkpoint_every steps (0: ceil(sqrt(n_steps))); the reverse sweep re-runs each segment.
+ * Cases: one case, wall and cyclic patches, FP64 (the case's storage field is ignored); other
+ * patch kinds give DUGKS_ERR_UNSUPPORTED.  Device memory: about
+ * (ceil(n_steps / C) + C + 1) x (2 n_cells + 2 n_boundary_faces) x nv x 8 bytes for the trajectory
+ * plus about (10 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;
+ * DUGKS_ERR_MEMORY when that does not fit. */
+typedef struct dugks_adjoint dugks_adjoint;
+/* *out is set also when the call fails, so that dugks_adjoint_last_error can explain it;
+ * release it with dugks_adjoint_destroy either way. */
+int dugks_adjoint_create(const dugks_case* cs, dugks_adjoint** out);
+int dugks_adjoint_run(dugks_adjoint* a, const double* rho, const double* U, const double* T, double dt,
+                      int32_t n_steps, int32_t first_step, int32_t checkpoint_every,
+                      const double* face_coef, double* J, double* dJ_dTw);
+/* Cell fields rho, U[3], T, q[3] after the last run's final step.  NULL skips. */
+int dugks_adjoint_get_macro(dugks_adjoint* a, double* rho, double* U, double* T, double* q);
+/* Device time of the last run's forward sweep and of its reverse sweep (with re-runs), ms. */
+int dugks_adjoint_last_ms(dugks_adjoint* a, double* forward_ms, double* reverse_ms);
+void dugks_adjoint_destroy(dugks_adjoint* a);
+const char* dugks_adjoint_last_error(dugks_adjoint* a);
+
 #ifdef __cplusplus
 }
 #endif

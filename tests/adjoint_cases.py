@@ -1,0 +1,69 @@
+"""Shared helpers of the adjoint tests: the objective evaluated on the oracle, wall-temperature
+perturbations and the small cases that cover the adjoint's code paths."""
+from __future__ import annotations
+
+import copy
+
+import numpy as np
+
+from dugksfoam_amd import cases
+
+
+def wall_mask(c) -> np.ndarray:
+    nfi, nb = c.n_internal_faces, c.n_faces - c.n_internal_faces
+    m = np.zeros(nb, bool)
+    for p in c.patches:
+        if p["type"] == cases.PATCH_WALL:
+            m[p["start"] - nfi: p["start"] - nfi + p["size"]] = True
+    return m
+
+
+def with_wall_T(c, dT):
+    """The case with T_w of boundary face b raised by dT[b]."""
+    c2 = copy.deepcopy(c)
+    nfi = c.n_internal_faces
+    for p in c2.patches:
+        if p["type"] == cases.PATCH_WALL:
+            p["T"] = np.asarray(p["T"], float) + dT[p["start"] - nfi: p["start"] - nfi + p["size"]]
+    return c2
+
+
+def objective(backend_cls, c, steps, first, coef) -> float:
+    """J = sum_{n >= first} sum_f coef_f rho_f (U_f . S_f) from a backend's face macros."""
+    e = backend_cls(c)
+    e.init_equilibrium()
+    J = 0.0
+    for n in range(1, steps + 1):
+        e.step(nsteps=1)
+        if n >= first:
+            fm = e.face_macro()
+            J += float(np.sum(coef * fm["rho"] * np.einsum("ij,ij->i", fm["U"], c.Sf)))
+    close = getattr(e, "close", None)
+    if close:
+        close()
+    return J
+
+
+def directional_fd(backend_cls, c, steps, first, coef, v, h=1e-3) -> float:
+    """Central difference of J along the wall-temperature direction v (Kelvin per unit step)."""
+    Jp = objective(backend_cls, with_wall_T(c, h * v), steps, first, coef)
+    Jm = objective(backend_cls, with_wall_T(c, -h * v), steps, first, coef)
+    return (Jp - Jm) / (2 * h)
+
+
+# (name, case builder, steps, first step of the objective, checkpoint spacing)
+CASES = [
+    ("cavity2d_nv9", lambda: cases.cavity(5, 9), 6, 3, 0),            # xi.S == 0 on axis faces: averaging branches
+    ("cavity2d_nv8_every4", lambda: cases.cavity(5, 8), 7, 1, 4),
+    ("cavity2d_K2_every1", lambda: cases.cavity(4, 7, K=2), 5, 2, 1),  # h carries K internal degrees of freedom
+    ("cavity3d", lambda: cases.cavity(3, 5, dim=3), 4, 2, 0),
+    ("channel", lambda: cases.channel(nx=8, ny=4, Lx=1.0, nv=7, kn=0.5, n_modes=3), 6, 1, 2),
+    ("channel_hot", lambda: cases.channel(nx=8, ny=4, Lx=1.0, nv=8, kn=0.2, n_modes=3, amp=0.2), 8, 5, 3),
+]
+
+
+def random_problem(c, seed=0):
+    rng = np.random.default_rng(seed)
+    coef = rng.standard_normal(c.n_faces)
+    v = rng.standard_normal(c.n_faces - c.n_internal_faces) * wall_mask(c)
+    return coef, v

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT
-from dugksfoam_amd import abi, cases, engine
+from dugksfoam_amd import abi, adjoint, cases, engine
 
 HEADER = os.path.join(ROOT, "include", "dugks.h")
 
@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol(lib):
     exported = set(re.findall(r" T (dugks_\w+)", out))
     missing = [n for n in names if n not in exported]
     assert not missing, missing
-    assert set(engine.SYMBOLS) == set(names)
+    assert set(engine.SYMBOLS) | set(adjoint.SYMBOLS) == set(names)
     for n in names:
         assert hasattr(lib, n)
 
@@ -99,6 +99,16 @@ def test_no_cpu_fallback_without_gpu(lib):
     with pytest.raises(engine.DugksError) as e:
         engine.Engine(cases.cavity(3, 5))
     assert e.value.code == -2
+
+
+def test_adjoint_no_cpu_fallback_without_gpu(lib):
+    """The adjoint, too, refuses to run without a GPU, and explains why."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(engine.DugksError) as e:
+        adjoint.Adjoint(cases.cavity(3, 5))
+    assert e.value.code in (-1, -2) and "HIP device" in str(e.value)
 
 
 def test_missing_library_fails_loudly(tmp_path, monkeypatch):
