@@ -188,6 +188,50 @@ def sweep():
         print(json.dumps(r), flush=True)
 
 
+def adjoint_bench(steps: int, avg_last: int, every: int):
+    """The discrete adjoint on the micro-channel (200 x 40 cells, 32^2 velocities, 16 wall-temperature
+    modes): one forward + reverse sweep giving dJ/da of the mid-plane mass flow averaged over the
+    last avg_last of `steps` steps, timed; beside it the FP64 engine's time for one forward run of
+    the same steps, from which central differences would need 2 n_modes + 1 runs."""
+    from dugksfoam_amd import cases, sensitivity
+    from dugksfoam_amd.adjoint import Adjoint
+    from dugksfoam_amd.engine import Engine
+    c = cases.config("channel")
+    n_modes = int(np.asarray(c.meta["a"]).size)
+    log(f"[adjoint] channel cells={c.n_cells} nv={c.nv} steps={steps} avg_last={avg_last} every={every}")
+    e = Engine(c)
+    e.init_equilibrium()
+    e.autotune(True)
+    e.step(nsteps=e.autotune_steps_left())
+    e.init_equilibrium()
+    t0 = time.perf_counter()
+    e.step(nsteps=steps)
+    engine_s = time.perf_counter() - t0
+    e.close()
+    log(f"[adjoint] engine forward {engine_s:.3f} s")
+    A = Adjoint(c)
+    t0 = time.perf_counter()
+    J, g = A.run(steps, steps - avg_last + 1, sensitivity.objective_coef(c, avg_last), checkpoint_every=every)
+    wall = time.perf_counter() - t0
+    fwd, rev = A.last_ms()
+    A.close()
+    b, dT = sensitivity.dTw_da(c)
+    ga = dT.T @ g[b]
+    log(f"[adjoint] J {J:.6e} forward {fwd:.0f} ms reverse {rev:.0f} ms wall {wall:.2f} s")
+    fd_s = (2 * n_modes + 1) * engine_s
+    print(json.dumps({
+        "metric": "adjoint gradient wall time (dJ/da, 16 modes)", "value": round(wall, 3), "unit": "s",
+        "higher_is_better": False, "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "channel: 200x40 quads x 32^2 velocities, Kn 0.5, cyclic in x, lower wall "
+                               "T(x) = 273 (1 + sum a_k sin(2 pi k x / 5)), a_k ~ U[-0.02, 0.02] (seed 2)",
+                   "steps": steps, "objective_steps": avg_last, "checkpoint_every": every,
+                   "cells": c.n_cells, "velocities": c.nv},
+        "forward_sweep_ms": round(fwd, 1), "reverse_sweep_ms": round(rev, 1),
+        "engine_forward_run_s": round(engine_s, 3),
+        "central_differences_s": round(fd_s, 2), "speedup_vs_central_differences": round(fd_s / wall, 2),
+        "J": J, "dJ_da": [float(f"{x:.9e}") for x in ga]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,7 +244,15 @@ def main():
                     help="only run every workload in FP64 and FP32 storage (throughput, FP32 error)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device copy of GIB GiB (rocprof counter calibration)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only time the discrete adjoint's gradient on the micro-channel")
+    ap.add_argument("--adj-steps", type=int, default=2000)
+    ap.add_argument("--adj-avg", type=int, default=500)
+    ap.add_argument("--adj-every", type=int, default=100)
     a = ap.parse_args()
+    if a.adjoint:
+        adjoint_bench(a.adj_steps, min(a.adj_avg, a.adj_steps), min(a.adj_every, a.adj_steps))
+        return
     if a.copy_calibration > 0:
         import torch
         n = int(a.copy_calibration * 2 ** 30) // 8

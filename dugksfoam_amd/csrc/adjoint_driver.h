@@ -3,7 +3,8 @@
 //   B::alloc<T>(n)            n zeroed elements of device memory (nullptr and B::fail on error)
 //   B::upload(d, h, bytes)    host -> device;  B::download(h, d, bytes) device -> host (waits)
 //   B::copy(d, s, bytes)      device -> device;  B::zero(d, bytes)
-//   B::launch(n, fn)          fn(x, i) for points i < n, x the backend's executor (adjoint_core.h)
+//   B::launch<Tag>(n, fn)     fn(x, i) for points i < n, x the backend's executor (adjoint_core.h);
+//                             Tag names the phase
 //   B::free_bytes()           device memory available for the trajectory
 //   B::tic(slot) / B::ms(a, b) device-time marks
 //   B::sync(), B::bad, B::msg error state
@@ -32,6 +33,18 @@
 #endif
 
 namespace adj {
+
+// Launch tags: they name the kernel of each phase in profiles.
+struct k_wall_in {};
+struct k_init {};
+struct k_s1 {};
+struct k_grad {};
+struct k_face {};
+struct k_cell {};
+struct k_face_adj {};
+struct k_recon_adj {};
+struct k_grad_adj {};
+struct k_s1_adj {};
 
 struct Mesh {
     int D = 0, K = 0, nc = 0, nfi = 0, nf = 0, nv = 0;
@@ -265,10 +278,10 @@ struct Engine {
         const double *cf = coef;
         double *ja = j ? jacc : nullptr;
         double *Gng = fwd ? b.Gg : nullptr, *Gnh = fwd ? b.Gh : nullptr;
-        be.launch(q.nc, ADJ_FN { s1_cell(x, q, a, i, gp_, hp_); });
-        be.launch(q.nc, ADJ_FN { grad_cell(x, q, a, i, gp_, hp_, gg_, gh_, Gng, Gnh); });
-        be.launch(q.nf, ADJ_FN { face_fwd(x, q, i, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
-        if (fwd) be.launch(q.nc, ADJ_FN { cell_fwd(x, q, a, i, gp_, hp_, gr, hr, b); });
+        be.template launch<k_s1>(q.nc, ADJ_FN { s1_cell(x, q, a, i, gp_, hp_); });
+        be.template launch<k_grad>(q.nc, ADJ_FN { grad_cell(x, q, a, i, gp_, hp_, gg_, gh_, Gng, Gnh); });
+        be.template launch<k_face>(q.nf, ADJ_FN { face_fwd(x, q, i, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
+        if (fwd) be.template launch<k_cell>(q.nc, ADJ_FN { cell_fwd(x, q, a, i, gp_, hp_, gr, hr, b); });
     }
 
     // reverse of step n (state a = state n); cur: index of the adjoint of state n + 1
@@ -281,11 +294,11 @@ struct Engine {
         const double *ag1 = ag[cur], *ah1 = ah[cur], *AG1g = AGg[cur], *AG1h = AGh[cur];
         double *ag0 = ag[cur ^ 1], *ah0 = ah[cur ^ 1], *AG0g = AGg[cur ^ 1], *AG0h = AGh[cur ^ 1];
         double *abf_ = abf, *abh_ = abh, *dTw_ = dTw, *apg_ = apg, *aph_ = aph, *agg_ = agg, *agh_ = agh;
-        be.launch(q.nf, ADJ_FN { face_adj(x, q, i, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_); });
+        be.template launch<k_face_adj>(q.nf, ADJ_FN { face_adj(x, q, i, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_); });
         if (n == 0) return;   // the initial state's adjoint is not needed
-        be.launch(q.nc, ADJ_FN { recon_adj(x, q, i, abf_, abh_, ag1, ah1, AG1g, AG1h, apg_, aph_, agg_, agh_); });
-        be.launch(q.nc, ADJ_FN { grad_adj(x, q, i, agg_, agh_, apg_, aph_, AG0g, AG0h); });
-        be.launch(q.nc, ADJ_FN { s1_adj(x, q, a, i, ag1, ah1, apg_, aph_, ag0, ah0); });
+        be.template launch<k_recon_adj>(q.nc, ADJ_FN { recon_adj(x, q, i, abf_, abh_, ag1, ah1, AG1g, AG1h, apg_, aph_, agg_, agh_); });
+        be.template launch<k_grad_adj>(q.nc, ADJ_FN { grad_adj(x, q, i, agg_, agh_, apg_, aph_, AG0g, AG0h); });
+        be.template launch<k_s1_adj>(q.nc, ADJ_FN { s1_adj(x, q, a, i, ag1, ah1, apg_, aph_, ag0, ah0); });
         cur ^= 1;
     }
 
@@ -325,11 +338,11 @@ struct Engine {
         double *win_ = win, *dwin_ = dwin;
         const double *r0 = rho0, *u0 = U0, *t0 = T0;
         const int nfi = m.nfi;
-        be.launch(nb, ADJ_FN {
+        be.template launch<k_wall_in>(nb, ADJ_FN {
             if (q.fkind[nfi + i] == FK_WALL) wall_in(x, q, nfi + i, win_, dwin_);
         });
         St cur = state(ncp + C - 1), nxt = state(ncp + C);
-        be.launch(q.nc, ADJ_FN { init_cell(x, q, cur, i, r0, u0, t0); });
+        be.template launch<k_init>(q.nc, ADJ_FN { init_cell(x, q, cur, i, r0, u0, t0); });
         be.zero(cur.Gg, 2 * nbv * sizeof(double));
         be.tic(0);
         for (int n = 0; n < nsteps; ++n) {

@@ -82,7 +82,7 @@ struct GX {
     }
 };
 
-template <class Fn>
+template <class Tag, class Fn>
 __global__ __launch_bounds__(NT) void k_points(Fn fn)
 {
     __shared__ double lds[(NT / 64) * SUM_MAX];
@@ -170,11 +170,11 @@ struct HipBackend {
         if (bytes) check(hipMemsetAsync(d, 0, bytes, stream), "memset");
     }
 
-    template <class Fn>
+    template <class Tag, class Fn>
     void launch(int n, Fn fn)
     {
         if (n <= 0 || bad) return;
-        hipLaunchKernelGGL(k_points<Fn>, dim3(n), dim3(NT), 0, stream, fn);
+        hipLaunchKernelGGL((k_points<Tag, Fn>), dim3(n), dim3(NT), 0, stream, fn);
         check(hipGetLastError(), "kernel launch");
     }
 

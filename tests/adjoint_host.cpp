@@ -62,7 +62,7 @@ struct HostBackend {
     void download(void *h, const void *d, size_t b) { std::memcpy(h, d, b); }
     void copy(void *d, const void *s, size_t b) { std::memmove(d, s, b); }
     void zero(void *d, size_t b) { std::memset(d, 0, b); }
-    template <class Fn>
+    template <class Tag, class Fn>
     void launch(int n, Fn fn)
     {
         for (int i = 0; i < n; ++i) {
