@@ -26,7 +26,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = None, face_values: bool = False):
+def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = None, face_values: bool = False,
+                fused_cell: bool = False):
     """Algorithmic bytes per launch of each step kernel: every array the kernel must
     touch, read once and written once.
 
@@ -37,7 +38,9 @@ def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = No
     partial sums.  k_cell reads gbar (and the wall rows, or in face-value mode each face value
     once) and writes g~ and its partial sums;
     k_cell_s1 reads g~ and writes the next gbar; the two _fin kernels touch O(faces) and
-    O(cells) data.  h is carried only when K + 3 - D != 0.  The whole-step figure is the
+    O(cells) data.  With fused_cell, k_cell also forms the macros and the next gbar in place
+    (k_cell_fin and k_cell_s1 do not run; the re-read of its own row adds nothing to the floor).
+    h is carried only when K + 3 - D != 0.  The whole-step figure is the
     floor of the scheme: gbar read twice and written once (3 s per update), plus the face
     values written and read once in face-value mode."""
     s = storage_bytes
@@ -62,6 +65,9 @@ def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = No
         "k_cell_fin": cpart + nc * 8 * 9,
         "k_cell_s1": nd * s * 2 * nc * nv,
     })
+    if fused_cell:
+        kb["k_cell"] += nc * 8 * 9 - cpart
+        del kb["k_cell_fin"], kb["k_cell_s1"]
     step_min = nd * s * (3 * nc * nv + 3 * nw * nv) + 2 * fvb
     return kb, step_min
 
@@ -98,13 +104,14 @@ def run_gpu(case, storage: str, steps: int, warmup: int, tune: bool = True):
     n_blocks, n_blocks_cell = e.blocks()[2], e.cell_blocks()[2]
     shapes = {"k_face": e.blocks()[:3], "k_cell": e.cell_blocks()}
     fv = e.face_values()
+    fused = e.fused_cell()
     m = e.macro()
     ok = bool(np.all(np.isfinite(m["rho"])) and np.all(m["rho"] > 0))
     e.close()
     wall = t1 - t0
     upd = case.n_cells * case.nv * steps
     sb = 8 if storage == "fp64" else 4
-    kb, step_min = bytes_model(case, sb, n_blocks, n_blocks_cell, fv)
+    kb, step_min = bytes_model(case, sb, n_blocks, n_blocks_cell, fv, fused)
     kern = {k: (kms[k] / n, kb[k]) for k in kb}
     dom = max(kern, key=lambda k: kern[k][0])
     avg_ms, bytes_ = kern[dom]
@@ -121,6 +128,7 @@ def run_gpu(case, storage: str, steps: int, warmup: int, tune: bool = True):
     roof["launch_shapes"] = {k: {"threads": v[0], "iters": v[1], "velocity_blocks": v[2]} for k, v in shapes.items()}
     roof["launch_shapes"]["tuning_steps"] = tune_steps
     roof["face_values"] = fv
+    roof["fused_cell"] = fused
     return dict(value=upd / wall / 1e9, ms_per_step=wall / steps * 1e3, kernel_ms=step_ms,
                 roofline=roof, finite=ok)
 

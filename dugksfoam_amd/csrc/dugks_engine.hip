@@ -309,6 +309,7 @@ template <int D, bool H>
 DEVI void eq_eval(const EqC &e, const double *tab, int nxi, const Vel &v, double kc, int K, double &geq,
                   double &heq)
 {
+#pragma clang fp contract(on)
     double cx = e.U[0] - v.x, cy = e.U[1] - v.y, cz = e.U[2] - v.z;
     double c2 = (cx * cx + cy * cy + cz * cz) * e.invRT;
     double pc = -(cx * e.qc[0] + cy * e.qc[1] + cz * e.qc[2]);
@@ -380,6 +381,30 @@ DEVI double div_rn(double a, double b, double y)
     return fma(r, y, q);
 }
 
+// Shared scalar formulas.  "contract(on)" fuses multiply-adds only within one expression, so
+// every kernel that uses them rounds identically (the default contraction is decided after
+// inlining and may differ from kernel to kernel).
+// S1 relaxation factor r = 1.5 dt / (2 tau + dt) (discreteVelocity.C:393)
+DEVI double s1_r(double dt, double tau)
+{
+#pragma clang fp contract(on)
+    return 1.5 * dt / (2.0 * tau + dt);
+}
+
+// heat-flux factor 2 tau / (2 tau + c) (fvDVM.C:522, 727)
+DEVI double q_fac(double tau, double c)
+{
+#pragma clang fp contract(on)
+    return 2.0 * tau / (2.0 * tau + c);
+}
+
+// S1 value (1 - r) g~ + r geq for a stored row value gv with row offset og (FP32 deviations)
+DEVI double s1_value(double gv, double og, double r, double geq)
+{
+#pragma clang fp contract(on)
+    return (1.0 - r) * (gv + og) + r * geq - og;
+}
+
 template <int D>
 DEVI double dotg(const double *G, const double *dC, const double *hx)
 {
@@ -437,6 +462,7 @@ template <int D>
 DEVI void finish_moments(const double *a, const double *Uh, double R, int K, double mu, double omega,
                          double Tref, double &rho, double *U, double &T, double &tau, double *q)
 {
+#pragma clang fp contract(on)
     rho = a[0];
     double u2 = 0.0;
 #pragma unroll
@@ -952,10 +978,11 @@ __global__ __launch_bounds__(256) void k_face_fin(P<TS> p)
 // k_face built it, relaxed towards the face equilibrium (incoming wall values: rho_w times the
 // wall Maxwellian, left unrelaxed, discreteVelocity.C:693-731) and its flux applied.  g~ of this
 // step is recovered from the resident gbar as (gbar - r geq) / (1 - r); the new g~ goes to the
-// output buffer, where k_cell_s1 turns it into the next gbar.
+// output buffer, where k_cell_s1 turns it into the next gbar.  With F1 (face-value mode, one
+// workgroup per cell) the kernel also does the work of k_cell_fin and k_cell_s1 for its cell.
 // Dynamic LDS: [velocity table (3 nax + 2) | tabC | tabF x MAXS | tabW x MAXS], nax doubles each.
 // ---------------------------------------------------------------------------------------------
-template <int D, bool H, int NS, typename TS, bool FV>
+template <int D, bool H, int NS, typename TS, bool FV, bool F1 = false>
 __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL_WAVES_FV : CELL_WAVES, 8))) void k_cell(P<TS> p)
 {
     constexpr int NC = NMom<D>::v;
@@ -987,7 +1014,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
     const double qcv[3] = {cptr(p.cq)[3 * c], cptr(p.cq)[3 * c + 1], cptr(p.cq)[3 * c + 2]};
     EqC ec;
     eq_consts(cptr(p.crho)[c], Uc, cptr(p.cT)[c], qcv, p.R, p.Pr, D, ec);
-    const double rc = 1.5 * dt / (2.0 * cptr(p.ctau)[c] + dt);
+    const double rc = s1_r(dt, cptr(p.ctau)[c]);
     const double inv1mr = 1.0 / (1.0 - rc);
     for (int t = threadIdx.x; t < nax; t += blockDim.x) {
         const int ax = t / p.nxi;
@@ -1115,7 +1142,52 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
         acc_moments<D, H>(acc, ag, ah, v, Uc);
     }
     block_sum<NC>(acc, red, tot);
-    if ((int)threadIdx.x < NC) p.cpart[((size_t)c * p.nbc + b) * NC + threadIdx.x] = tot[threadIdx.x];
+    if (!F1) {
+        if ((int)threadIdx.x < NC) p.cpart[((size_t)c * p.nbc + b) * NC + threadIdx.x] = tot[threadIdx.x];
+        return;
+    }
+    // F1 (one workgroup holds all velocities of the cell, nbc = 1): the work of k_cell_fin and
+    // k_cell_s1 for this cell, with their arithmetic, on the row this workgroup has just written.
+    // Each lane re-reads only the velocities it stored itself, from L2.
+    double a[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) a[m] = 0.0 + tot[m];
+    double rho, U[3], T, tau, q[3];
+    finish_moments<D>(a, Uc, p.R, p.K, p.muc[cas], p.omega, p.Tref, rho, U, T, tau, q);
+    const double fqc = q_fac(tau, dt * p.Pr);
+    for (int d = 0; d < 3; ++d) q[d] = fqc * q[d];
+    if (threadIdx.x == 0) {
+        p.crho[c] = rho;
+        p.cT[c] = T;
+        p.ctau[c] = tau;
+        for (int d = 0; d < 3; ++d) {
+            p.cU[3 * c + d] = U[d];
+            p.cq[3 * c + d] = q[d];
+        }
+        if (!(rho > 0.0) || !(T > 0.0) || !isfinite(rho) || !isfinite(T) || !isfinite(tau)) p.flag[0] = 1;
+    }
+    EqC en;
+    eq_consts(rho, U, T, q, p.R, p.Pr, D, en);
+    // every lane is past the velocity loop (block_sum ends in a barrier): tabC can be rebuilt
+    for (int t = threadIdx.x; t < nax; t += blockDim.x) {
+        const int ax = t / p.nxi;
+        const double u = U[ax] - p.axis[t - ax * p.nxi];
+        tabC[t] = exp(-(u * u) * (0.5 * en.invRT));
+    }
+    __syncthreads();
+    const double rn = s1_r(dt, tau);
+    TS *__restrict__ go = p.gb_out + (size_t)c * p.nvp;
+    TS *__restrict__ ho = H ? p.hb_out + (size_t)c * p.nvp : nullptr;
+    for (int it = 0; it < p.nibc; ++it) {
+        const unsigned k = k0 + it * p.ntc;
+        const Vel v = load_vel<D>(p, k, vt);
+        const double gv = ld(go, k), hv = H ? ld(ho, k) : 0.0;
+        double geq, heq;
+        eq_eval<D, H>(en, tabC, p.nxi, v, p.kc, p.K, geq, heq);
+        const double og = row_off<D>(p, v, 0, vt), oh = H ? row_off<D>(p, v, 1, vt) : 0.0;
+        go[k] = (TS)s1_value(gv, og, rn, geq);
+        if (H) ho[k] = (TS)s1_value(hv, oh, rn, heq);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1138,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_cell_fin(P<TS> p)
     const double Uc[3] = {p.cU[3 * c], p.cU[3 * c + 1], p.cU[3 * c + 2]};
     double rho, U[3], T, tau, q[3];
     finish_moments<D>(a, Uc, p.R, p.K, p.muc[cas], p.omega, p.Tref, rho, U, T, tau, q);
-    const double fqc = 2.0 * tau / (2.0 * tau + dt * p.Pr);
+    const double fqc = q_fac(tau, dt * p.Pr);
     for (int d = 0; d < 3; ++d) q[d] = fqc * q[d];
     p.crho[c] = rho;
     p.cT[c] = T;
@@ -1201,7 +1273,7 @@ __global__ __launch_bounds__(MAXNT) void k_cell_s1(P<TS> p)
         tabN[t] = exp(-(u * u) * (0.5 * en.invRT));
     }
     __syncthreads();
-    const double rn = 1.5 * dt / (2.0 * cptr(p.ctau)[c] + dt);
+    const double rn = s1_r(dt, cptr(p.ctau)[c]);
     TS *__restrict__ go = p.gb_out + (size_t)c * p.nvp;
     TS *__restrict__ ho = H ? p.hb_out + (size_t)c * p.nvp : nullptr;
     // S1_UNR velocities per thread in flight: all loads of a group are issued before its stores
@@ -1226,8 +1298,8 @@ __global__ __launch_bounds__(MAXNT) void k_cell_s1(P<TS> p)
                 double geq, heq;
                 eq_eval<D, H>(en, tabN, p.nxi, vv[u], p.kc, p.K, geq, heq);
                 const double og = row_off<D>(p, vv[u], 0, vt), oh = H ? row_off<D>(p, vv[u], 1, vt) : 0.0;
-                go[k] = (TS)((1.0 - rn) * (gv[u] + og) + rn * geq - og);
-                if (H) ho[k] = (TS)((1.0 - rn) * (hv[u] + oh) + rn * heq - oh);
+                go[k] = (TS)s1_value(gv[u], og, rn, geq);
+                if (H) ho[k] = (TS)s1_value(hv[u], oh, rn, heq);
             }
         }
     }
@@ -1288,8 +1360,8 @@ __global__ __launch_bounds__(MAXNT) void k_convert(P<TS> p, const double *dt_old
     __syncthreads();
     const double tau = p.ctau[c];
     double ro = 0.0, rn = 0.0;
-    if (mode != 0) ro = 1.5 * dt_old[cas] / (2.0 * tau + dt_old[cas]);
-    if (mode != 2) rn = 1.5 * dt_new[cas] / (2.0 * tau + dt_new[cas]);
+    if (mode != 0) ro = s1_r(dt_old[cas], tau);
+    if (mode != 2) rn = s1_r(dt_new[cas], tau);
     const double inv1mr = 1.0 / (1.0 - ro);
     for (int it = 0; it < p.niter; ++it) {
         const int k = threadIdx.x + it * p.nt;
@@ -1374,7 +1446,7 @@ __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, co
         tab[t] = exp(-(u * u) * (0.5 * e.invRT));
     }
     __syncthreads();
-    const double ro = raw ? 0.0 : 1.5 * dt_old[cas] / (2.0 * p.ctau[c] + dt_old[cas]);
+    const double ro = raw ? 0.0 : s1_r(dt_old[cas], p.ctau[c]);
     const double inv1mr = 1.0 / (1.0 - ro);
     for (int it = 0; it < p.niter; ++it) {
         const int k = threadIdx.x + it * p.nt;
@@ -1449,6 +1521,7 @@ struct dugks_engine {
     void *gB[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
     int face_values = 0;  // face-value mode: the face values are stored, k_cell reads them
     int face_pass_legacy = 0;  // face-value mode with the single face pass (k_face storing)
+    int fuse_s1 = 1;           // one workgroup per cell: k_cell also forms the macros and S1
     void *fvg = nullptr, *fvh = nullptr;
     int cur = 0;         // buffer holding the current state
     int raw = 1;         // state holds g~ (1) or gbar prepared with dt_prep (0)
@@ -1578,6 +1651,20 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
     if (ev) (void)hipEventRecord(ev[2], h->stream);
     hipLaunchKernelGGL((k_face_fin<D, TS>), dim3((h->nf + 255) / 256), dim3(256), 0, h->stream, p);
     if (ev) (void)hipEventRecord(ev[3], h->stream);
+    if constexpr (FV) {
+        if (h->fuse_s1 && h->nvbc == 1) {
+            // k_cell with the cell macros and the next step's S1 in its tail (k_cell_fin and
+            // k_cell_s1 fused; their profile slots stay empty)
+            hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV, true>), block_grid(h->nc, 1), dim3(h->ntc), cell_lds(h),
+                               h->stream, p);
+            if (ev) (void)hipEventRecord(ev[4], h->stream);
+            if (h->npress)
+                hipLaunchKernelGGL((k_pressure_bc<D, TS>), dim3((h->nopen + 255) / 256), dim3(256), 0, h->stream, p);
+            if (ev) (void)hipEventRecord(ev[5], h->stream);
+            if (ev) (void)hipEventRecord(ev[6], h->stream);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
     if (ev) (void)hipEventRecord(ev[4], h->stream);
     hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
@@ -2343,7 +2430,8 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
         HIPCHK(h, hipEventSynchronize(h->tev[NKERN]));
         float tf = 0.f, tc = 0.f;
         HIPCHK(h, hipEventElapsedTime(&tf, h->tev[0], h->tev[2]));
-        HIPCHK(h, hipEventElapsedTime(&tc, h->tev[3], h->tev[4]));
+        // the cell side: k_cell, k_cell_fin and k_cell_s1 (one kernel when fused)
+        HIPCHK(h, hipEventElapsedTime(&tc, h->tev[3], h->tev[6]));
         if (i >= 0 && i < nfc && (h->tune_best_face == 0.0 || tf < h->tune_best_face)) {
             h->tune_best_face = tf;
             h->tune_face = i;
@@ -2455,6 +2543,9 @@ int dugks_create(const dugks_case *cs, dugks_engine **out)
         if (h->has_sym || (fv ? std::atoi(fv) != 0 : fits)) r = dugks_set_face_values(h, 1);
         const char *fl = std::getenv("DUGKS_FACE_PASS_LEGACY");
         h->face_pass_legacy = fl && std::atoi(fl) && !h->has_sym ? 1 : 0;
+        // DUGKS_FUSE_S1=0 keeps k_cell_fin and k_cell_s1 as kernels of their own
+        const char *f1 = std::getenv("DUGKS_FUSE_S1");
+        h->fuse_s1 = f1 && !std::atoi(f1) ? 0 : 1;
     }
     if (r) {
         // the handle is returned without device resources so that dugks_last_error can

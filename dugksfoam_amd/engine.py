@@ -79,6 +79,13 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def _atoi(v: str) -> int:
+    """C atoi: optional sign and leading digits, 0 when there are none."""
+    import re
+    m = re.match(r"\s*([+-]?\d+)", v)
+    return int(m.group(1)) if m else 0
+
+
 class Engine:
     """One DUGKS engine on one GPU for one Case (or a disjoint batch of cases)."""
 
@@ -97,6 +104,9 @@ class Engine:
             self._h = None
             raise DugksError(r, msg)
         self.storage = storage
+        # dugks_create reads DUGKS_FUSE_S1 once (C atoi: unset or nonzero keeps the fusion on)
+        v = os.environ.get("DUGKS_FUSE_S1")
+        self._fuse_s1 = v is None or _atoi(v) != 0
         nv, nc, nf = C.c_int32(), C.c_int32(), C.c_int32()
         L.dugks_get_sizes(h, C.byref(nv), C.byref(nc), C.byref(nf))
         self.nv, self.n_cells, self.n_faces = nv.value, nc.value, nf.value
@@ -193,7 +203,16 @@ class Engine:
         n = C.c_int64()
         self._chk(load_library().dugks_get_profile(self._h, dptr(ms), C.byref(n)))
         names = self.KERNELS_FV if self.face_values() else self.KERNELS
-        return {k: v for k, v in zip(names, ms.tolist()) if k}, n.value
+        out = {k: v for k, v in zip(names, ms.tolist()) if k}
+        if self.fused_cell():
+            out.pop("k_cell_fin", None)
+            out.pop("k_cell_s1", None)
+        return out, n.value
+
+    def fused_cell(self) -> bool:
+        """True when k_cell also forms the cell macros and the next step's S1 (face-value mode,
+        one workgroup per cell, DUGKS_FUSE_S1 not 0): k_cell_fin and k_cell_s1 do not run."""
+        return self._fuse_s1 and self.face_values() and self.cell_blocks()[2] == 1
 
     def set_graphs(self, on: bool = True):
         """Replay captured two-step HIP graphs for multi-step calls (default on)."""

@@ -403,6 +403,35 @@ def test_face_value_mode(which):
         b.close()
 
 
+@pytest.mark.parametrize("which", list(_small_cases()))
+def test_fused_cell_tail_is_bit_identical(which, monkeypatch):
+    """With one workgroup per cell, k_cell forms the cell macros and the next gbar itself
+    (k_cell_fin and k_cell_s1 fused into its tail): results bit-identical to the separate
+    kernels, FP64 and FP32 storage, over graph-replayed steps."""
+    mk, n = _small_cases()[which]
+    c = mk()
+    for st in ("fp64", "fp32"):
+        monkeypatch.setenv("DUGKS_FUSE_S1", "1")
+        a = Engine(c, storage=st)
+        monkeypatch.setenv("DUGKS_FUSE_S1", "0")
+        b = Engine(c, storage=st)
+        assert a.fused_cell() and not b.fused_cell(), (a.face_values(), a.cell_blocks())
+        for e in (a, b):
+            e.init_equilibrium()
+            e.step(nsteps=n)
+        for get in ("macro", "face_macro"):
+            ma, mb = getattr(a, get)(), getattr(b, get)()
+            for k in ma:
+                assert np.array_equal(ma[k], mb[k]), (which, st, get, k, rel_l2(ma[k], mb[k]))
+        cells = np.arange(0, c.n_cells, max(1, c.n_cells // 7), dtype=np.int32)
+        ga, gb = a.df(cells), b.df(cells)
+        assert np.array_equal(ga[0], gb[0]) and np.array_equal(ga[1], gb[1]), (which, st, "df")
+        _say(f"  {which} {st} fused cell tail: bit-identical")
+        a.close()
+        b.close()
+    monkeypatch.delenv("DUGKS_FUSE_S1")
+
+
 @pytest.mark.parametrize("which", ["cavity3d_6x12", "cavity2d_12x21", "channel_20x8"])
 def test_autotune_shapes_keep_parity(which):
     """The online tuner runs every candidate work decomposition of both kernels on real
