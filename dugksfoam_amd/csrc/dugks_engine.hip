@@ -204,6 +204,9 @@ struct P {
     int nopen;          // symmetry, far-field, fixed and pressure faces
     const double *__restrict__ bpress;  // [wall record] pressureIn / pressureOut pressure
     const int *__restrict__ open;  // their face indices
+    const int *__restrict__ fsel;  // k_face_moments over a face list (the wall faces), else null
+    int nfsel;
+    const int *__restrict__ mxf;   // k_face_moments_mx: non-wall faces in launch order, 16 per tile (-1 pads)
     int *flag;
 };
 
@@ -841,10 +844,10 @@ __global__ __launch_bounds__(MAXNT) void k_face_moments(P<TS> p)
     __shared__ double tot[NM];
     __shared__ double vt[VT_SIZE];
     int pos, b;
-    if (!xcd_block(p.nf, p.nb, pos, b)) return;
+    if (!xcd_block(p.fsel ? p.nfsel : p.nf, p.nb, pos, b)) return;
     stage_vt(p, vt);
     __syncthreads();
-    const int f = cptr(p.forder)[pos];
+    const int f = p.fsel ? cptr(p.fsel)[pos] : cptr(p.forder)[pos];
     const CONST_AS FaceX &fx = cptr(p.fx)[f];
     const bool wall = fx.kind == KIND_WALL;
     const double S[3] = {fx.S[0], fx.S[1], fx.S[2]};
@@ -887,6 +890,133 @@ __global__ __launch_bounds__(MAXNT) void k_face_moments(P<TS> p)
     }
     block_sum<NM>(acc, red, tot);
     if ((int)threadIdx.x < NM) p.fpart[((size_t)f * p.nb + b) * NM + threadIdx.x] = tot[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------------------------
+// k_face_moments_mx: the S6 moment sums of 16 non-wall faces per workgroup as FP64 matrix products
+// (FP64 storage, face-value mode).  The sums are a contraction over velocity space,
+//   M[face][m] = sum_k fv[face][k] W[k][m],  W[k][m] = w_k phi_m(xi_k - Uh),
+// with phi_m the moment weights of acc_moments (1, xi, |xi|^2 / 2, ch |ch|^2 / 2, ch ch^T / 2;
+// h: 1 / 2, ch / 2) about one reference velocity Uh per tile: the previous velocity of the
+// tile's first face (faces of a tile are consecutive in the Morton launch order, so their
+// velocities are close), which is written to fU of each face of the tile for k_face_fin's exact
+// shift to the new face velocity.
+// Each wave sweeps 64-velocity chunks (wave w: chunks w, w + 4, ...): every lane computes the
+// weights of one velocity into LDS and loads 16 consecutive face values of one face (a face's 64
+// values per chunk are one coalesced 512 B segment), then 16 v_mfma_f64_16x16x4 accumulate
+// D[face][m] += sum over 4 velocities of A[face][k] B[k][m] with A from registers and B from LDS.
+// Operand layout of v_mfma_f64_16x16x4f64 (checked on gfx950): A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i mod 4), element i / 4.  The four wave
+// partials are added in wave order.
+// ---------------------------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int MX_NT = 256;
+constexpr int MX_WS = 17;   // LDS row stride of the weight chunk (doubles), odd against bank conflicts
+
+template <int D, bool H>
+__global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<double> p)
+{
+    constexpr int NMV = NMom<D>::v;
+    constexpr int NM = NMV + 1;
+    static_assert(NMV <= 16, "moments per MFMA column block");
+    __shared__ double vt[VT_SIZE];
+    __shared__ double Wl[MX_NT / 64][64 * MX_WS];
+    __shared__ double Wh[H ? MX_NT / 64 : 1][H ? 64 * 4 : 1];
+    __shared__ double Cr[MX_NT / 64][16 * 16];
+    const int tile = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+    stage_vt(p, vt);
+    const int f0 = p.mxf[16 * tile];
+    const int fi = p.mxf[16 * tile + i];
+    const double Uh[3] = {p.fU[3 * f0], p.fU[3 * f0 + 1], p.fU[3 * f0 + 2]};
+    __syncthreads();
+    const double *__restrict__ rowg = p.fvg + (size_t)(fi >= 0 ? fi : f0) * p.nvp;
+    const double *__restrict__ rowh = H ? p.fvh + (size_t)(fi >= 0 ? fi : f0) * p.nvp : nullptr;
+    const double live = fi >= 0 ? 1.0 : 0.0;
+    double *W = Wl[wave];
+    v4d c = {0.0, 0.0, 0.0, 0.0};
+    const int nchunk = p.nvp >> 6;
+    for (int r = 0; 4 * r < nchunk; ++r) {
+        const int chunk = 4 * r + wave;
+        const bool act = chunk < nchunk;
+        const int k0 = chunk << 6;
+        double a[16], ah[16];
+        if (act) {
+            // weights of velocity k0 + lane
+            const Vel v = load_vel<D>(p, (unsigned)(k0 + lane), vt);
+            const double xi[3] = {v.x, v.y, v.z};
+            double ch[3] = {0.0, 0.0, 0.0}, cc = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                ch[d] = xi[d] - Uh[d];
+                cc = fma(ch[d], ch[d], cc);
+            }
+            const double hw = 0.5 * v.w;
+            double *wr = W + lane * MX_WS;
+            wr[0] = v.w;
+#pragma unroll
+            for (int d = 0; d < D; ++d) wr[1 + d] = v.w * xi[d];
+            wr[D + 1] = hw * v.xi2;
+#pragma unroll
+            for (int d = 0; d < D; ++d) wr[D + 2 + d] = hw * ch[d] * cc;
+            int m = 2 * D + 2;
+#pragma unroll
+            for (int a0 = 0; a0 < D; ++a0)
+#pragma unroll
+                for (int a1 = a0; a1 < D; ++a1) wr[m++] = hw * ch[a0] * ch[a1];
+#pragma unroll
+            for (int j = NMV; j < 16; ++j) wr[j] = 0.0;
+            if (H) {
+                double *wh = Wh[wave] + lane * 4;
+                wh[0] = hw;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) wh[1 + d] = d < D ? hw * ch[d] : 0.0;
+            }
+            // face values k0 + 16 kk + m of face i
+            const double *ga = rowg + k0 + 16 * kk;
+#pragma unroll
+            for (int m2 = 0; m2 < 16; ++m2) a[m2] = live * ga[m2];
+            if (H) {
+                const double *ha = rowh + k0 + 16 * kk;
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2) ah[m2] = live * ha[m2];
+            }
+        }
+        __syncthreads();
+        if (act) {
+            // B[kk][j] of product m2 = W[16 kk + m2][j], j = this lane's column i
+            const double *wb = W + (16 * kk) * MX_WS + i;
+#pragma unroll
+            for (int m2 = 0; m2 < 16; ++m2)
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m2], wb[m2 * MX_WS], c, 0, 0, 0);
+            if (H) {
+                // h enters the energy (column D + 1) and Q (columns D + 2 .. 2 D + 1)
+                const int sel = i == D + 1 ? 0 : (i >= D + 2 && i < 2 * D + 2 ? i - D - 1 : -1);
+                const double *hb = Wh[wave] + (16 * kk) * 4 + (sel < 0 ? 0 : sel);
+                const double on = sel < 0 ? 0.0 : 1.0;
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2)
+                    c = __builtin_amdgcn_mfma_f64_16x16x4f64(ah[m2], on * hb[m2 * 4], c, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Cr[wave][(kk + 4 * r) * 16 + i] = c[r];
+    __syncthreads();
+    {
+        const int row = threadIdx.x >> 4, j = threadIdx.x & 15;
+        const int f = p.mxf[16 * tile + row];
+        if (f >= 0) {
+            const double sum = ((Cr[0][row * 16 + j] + Cr[1][row * 16 + j]) + Cr[2][row * 16 + j]) + Cr[3][row * 16 + j];
+            double *fp = p.fpart + (size_t)f * p.nb * NM;
+            if (j < NM) fp[j] = j < NMV ? sum : 0.0;
+            for (int b = 1; b < p.nb; ++b)
+                if (j < NM) fp[b * NM + j] = 0.0;
+            if (j < 3) p.fU[3 * f + j] = Uh[j];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1516,6 +1646,9 @@ struct dugks_engine {
     CellSt *cst = nullptr;
     FaceX *fx = nullptr;
     int *corder = nullptr, *forder = nullptr;
+    int *mxf = nullptr, *wallf = nullptr;   // k_face_moments_mx tiles; wall faces
+    int nmx_tiles = 0, nwallf = 0;
+    int face_mfma = 1;   // FP64 face-value mode: face moments on the matrix cores
     double *fpart = nullptr, *cpart = nullptr;
     void *gb[2] = {nullptr, nullptr}, *hb[2] = {nullptr, nullptr};
     void *gB[2] = {nullptr, nullptr}, *hB[2] = {nullptr, nullptr};
@@ -1614,6 +1747,9 @@ P<TS> params(dugks_engine *h)
     p.nopen = h->nopen;
     p.bpress = h->bpress;
     p.open = h->open;
+    p.fsel = nullptr;
+    p.nfsel = 0;
+    p.mxf = h->mxf;
     p.fvg = h->face_values ? (TS *)h->fvg : nullptr;
     p.fvh = h->face_values ? (TS *)h->fvh : nullptr;
     p.flag = h->flag;
@@ -1643,7 +1779,23 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
         if (h->nopen)
             hipLaunchKernelGGL((k_face_open<D, H, NS, TS>), dim3(h->nopen, h->nvb), dim3(h->nt), 0, h->stream, p);
         if (ev) (void)hipEventRecord(ev[1], h->stream);
-        hipLaunchKernelGGL((k_face_moments<D, H, TS>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
+        bool done = false;
+        if constexpr (sizeof(TS) == 8) {
+            if (h->face_mfma) {
+                // non-wall faces on the matrix cores, wall faces (outgoing sums) on the vector units
+                if (h->nmx_tiles)
+                    hipLaunchKernelGGL((k_face_moments_mx<D, H>), dim3(h->nmx_tiles), dim3(MX_NT), 0, h->stream, p);
+                if (h->nwallf) {
+                    P<TS> pw = p;
+                    pw.fsel = h->wallf;
+                    pw.nfsel = h->nwallf;
+                    hipLaunchKernelGGL((k_face_moments<D, H, TS>), block_grid(h->nwallf, h->nvb), dim3(h->nt), 0, h->stream, pw);
+                }
+                done = true;
+            }
+        }
+        if (!done)
+            hipLaunchKernelGGL((k_face_moments<D, H, TS>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
     } else {
         hipLaunchKernelGGL((k_face<D, H, NS, TS, FV>), block_grid(h->nf, h->nvb), dim3(h->nt), 0, h->stream, p);
         if (ev) (void)hipEventRecord(ev[1], h->stream);
@@ -2291,6 +2443,19 @@ int build(dugks_engine *h, const dugks_case *cs)
     h->nopen = (int)open_faces.size();
     if (h->nopen && ((r = dalloc(h, &h->open, open_faces.size())) || (r = upload(h, h->open, open_faces.data(), open_faces.size()))))
         return r;
+    // face lists of the matrix-core face moments: non-wall faces in launch order, 16 per tile
+    // (-1 pads the last tile), and the wall faces, whose outgoing-only sums stay on k_face_moments
+    {
+        std::vector<int> mx, wl;
+        for (int i = 0; i < nf; ++i) (fxs[forder[i]].kind == KIND_WALL ? wl : mx).push_back(forder[i]);
+        h->nmx_tiles = (int)((mx.size() + 15) / 16);
+        mx.resize((size_t)h->nmx_tiles * 16, -1);
+        h->nwallf = (int)wl.size();
+        if (h->nmx_tiles && ((r = dalloc(h, &h->mxf, mx.size())) || (r = upload(h, h->mxf, mx.data(), mx.size()))))
+            return r;
+        if (h->nwallf && ((r = dalloc(h, &h->wallf, wl.size())) || (r = upload(h, h->wallf, wl.data(), wl.size()))))
+            return r;
+    }
     HIPCHK(h, hipMemset(h->flag, 0, sizeof(int)));
     h->dt_prep.assign(h->n_cases, 0.0);
     return 0;
@@ -2321,7 +2486,7 @@ void release(dugks_engine *h)
     dfree(h->dtc); dfree(h->dtp); dfree(h->muc); dfree(h->crho); dfree(h->cU); dfree(h->cT); dfree(h->cq);
     dfree(h->ctau); dfree(h->frho); dfree(h->fU); dfree(h->fT); dfree(h->fq); dfree(h->feq);
     dfree(h->wall); dfree(h->wexp); dfree(h->qwall); dfree(h->swall); dfree(h->fx); dfree(h->cst);
-    dfree(h->corder); dfree(h->forder); dfree(h->fpart); dfree(h->cpart);
+    dfree(h->corder); dfree(h->forder); dfree(h->fpart); dfree(h->cpart); dfree(h->mxf); dfree(h->wallf);
     for (int i = 0; i < 2; ++i) {
         dfree(h->gb[i]); dfree(h->hb[i]);
         h->gB[i] = h->hB[i] = nullptr;
@@ -2543,6 +2708,9 @@ int dugks_create(const dugks_case *cs, dugks_engine **out)
         if (h->has_sym || (fv ? std::atoi(fv) != 0 : fits)) r = dugks_set_face_values(h, 1);
         const char *fl = std::getenv("DUGKS_FACE_PASS_LEGACY");
         h->face_pass_legacy = fl && std::atoi(fl) && !h->has_sym ? 1 : 0;
+        // DUGKS_FACE_MFMA=0 keeps the FP64 face moments on the vector units
+        const char *fm = std::getenv("DUGKS_FACE_MFMA");
+        h->face_mfma = fm && !std::atoi(fm) ? 0 : 1;
         // DUGKS_FUSE_S1=0 keeps k_cell_fin and k_cell_s1 as kernels of their own
         const char *f1 = std::getenv("DUGKS_FUSE_S1");
         h->fuse_s1 = f1 && !std::atoi(f1) ? 0 : 1;

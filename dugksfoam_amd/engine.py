@@ -107,6 +107,8 @@ class Engine:
         # dugks_create reads DUGKS_FUSE_S1 once (C atoi: unset or nonzero keeps the fusion on)
         v = os.environ.get("DUGKS_FUSE_S1")
         self._fuse_s1 = v is None or _atoi(v) != 0
+        v = os.environ.get("DUGKS_FACE_MFMA")
+        self._face_mfma = v is None or _atoi(v) != 0
         nv, nc, nf = C.c_int32(), C.c_int32(), C.c_int32()
         L.dugks_get_sizes(h, C.byref(nv), C.byref(nc), C.byref(nf))
         self.nv, self.n_cells, self.n_faces = nv.value, nc.value, nf.value
@@ -208,6 +210,11 @@ class Engine:
             out.pop("k_cell_fin", None)
             out.pop("k_cell_s1", None)
         return out, n.value
+
+    def face_mfma(self) -> bool:
+        """True when the face moments of non-wall faces are FP64 matrix products
+        (k_face_moments_mx: FP64 storage, face-value mode, DUGKS_FACE_MFMA not 0)."""
+        return self._face_mfma and self.storage == "fp64" and self.face_values()
 
     def fused_cell(self) -> bool:
         """True when k_cell also forms the cell macros and the next step's S1 (face-value mode,

@@ -377,7 +377,8 @@ def test_face_value_mode(which):
         a.set_face_values(True)
         b.set_face_values(False)
         assert a.face_values() and not b.face_values()
-        if st == "fp64" and a.cell_blocks() != b.cell_blocks():
+        if st == "fp64" and (a.cell_blocks() != b.cell_blocks() or a.face_mfma()):
+            # other velocity-sum splits, or the face moments as matrix products
             tol = FP64_TOL
         a.init_equilibrium()
         b.init_equilibrium()
@@ -401,6 +402,39 @@ def test_face_value_mode(which):
                 assert rel_l2(ma[k], mb[k]) <= FP64_TOL, (which, "switched", k)
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("which", list(_small_cases()))
+def test_face_moments_on_matrix_cores(which, monkeypatch):
+    """FP64 face-value mode sums the face moments of non-wall faces with FP64 MFMA
+    (k_face_moments_mx) by default: against the vector-unit sums (DUGKS_FACE_MFMA=0) to
+    round-off, and against the oracle no worse than the vector-unit sums (or within 1e-12)."""
+    mk, n = _small_cases()[which]
+    c = mk()
+    monkeypatch.setenv("DUGKS_FACE_MFMA", "1")
+    a = Engine(c)
+    monkeypatch.setenv("DUGKS_FACE_MFMA", "0")
+    b = Engine(c)
+    monkeypatch.delenv("DUGKS_FACE_MFMA")
+    assert a.face_mfma() and not b.face_mfma()
+    o = _oracle(c)
+    for e in (a, b):
+        e.init_equilibrium()
+        e.step(nsteps=n)
+    o.step(nsteps=n)
+    mo = o.macro()
+    for get in ("macro", "face_macro"):
+        ma, mb = getattr(a, get)(), getattr(b, get)()
+        for k in ma:
+            r = rel_l2(ma[k], mb[k])
+            _say(f"  {which} {get} {k}: matrix vs vector face moments {r:.2e}")
+            # two FP64 summation orders: round-off, up to 1e-11 on fields that nearly cancel
+            assert r <= 1e-11, (which, get, k, r)
+    for k in mo:
+        ra = rel_l2(a.macro()[k], mo[k])
+        assert ra <= max(FP64_TOL, 2 * rel_l2(b.macro()[k], mo[k])), (which, k, ra)
+    a.close()
+    b.close()
 
 
 @pytest.mark.parametrize("which", list(_small_cases()))
