@@ -894,7 +894,7 @@ __global__ __launch_bounds__(MAXNT) void k_face_moments(P<TS> p)
 
 // ---------------------------------------------------------------------------------------------
 // k_face_moments_mx: the S6 moment sums of 16 non-wall faces per workgroup as FP64 matrix products
-// (FP64 storage, face-value mode).  The sums are a contraction over velocity space,
+// (face-value mode).  The sums are a contraction over velocity space,
 //   M[face][m] = sum_k fv[face][k] W[k][m],  W[k][m] = w_k phi_m(xi_k - Uh),
 // with phi_m the moment weights of acc_moments (1, xi, |xi|^2 / 2, ch |ch|^2 / 2, ch ch^T / 2;
 // h: 1 / 2, ch / 2) about one reference velocity Uh per tile: the previous velocity of the
@@ -907,22 +907,25 @@ __global__ __launch_bounds__(MAXNT) void k_face_moments(P<TS> p)
 // D[face][m] += sum over 4 velocities of A[face][k] B[k][m] with A from registers and B from LDS.
 // Operand layout of v_mfma_f64_16x16x4f64 (checked on gfx950): A[i][k] in lane i + 16 k,
 // B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i mod 4), element i / 4.  The four wave
-// partials are added in wave order.
+// partials are added in wave order.  FP32 storage keeps face values as deviations from the rest
+// Maxwellian (row_off): a tile then holds 15 faces and row 15 carries the offsets, whose moments
+// are added to every face's (sum (s + o) W = sum s W + sum o W).
 // ---------------------------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int MX_NT = 256;
 constexpr int MX_WS = 17;   // LDS row stride of the weight chunk (doubles), odd against bank conflicts
 
-template <int D, bool H>
-__global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<double> p)
+template <int D, bool H, typename TS>
+__global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<TS> p)
 {
+    constexpr bool OFF = sizeof(TS) == 4;   // row 15 holds the FP32 row offsets
     constexpr int NMV = NMom<D>::v;
     constexpr int NM = NMV + 1;
     static_assert(NMV <= 16, "moments per MFMA column block");
     __shared__ double vt[VT_SIZE];
     __shared__ double Wl[MX_NT / 64][64 * MX_WS];
     __shared__ double Wh[H ? MX_NT / 64 : 1][H ? 64 * 4 : 1];
-    __shared__ double Cr[MX_NT / 64][16 * 16];
+    static_assert(64 * MX_WS >= 16 * 16, "the wave partials reuse the weight chunks");
     const int tile = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, kk = lane >> 4;
@@ -931,9 +934,10 @@ __global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<double> p)
     const int fi = p.mxf[16 * tile + i];
     const double Uh[3] = {p.fU[3 * f0], p.fU[3 * f0 + 1], p.fU[3 * f0 + 2]};
     __syncthreads();
-    const double *__restrict__ rowg = p.fvg + (size_t)(fi >= 0 ? fi : f0) * p.nvp;
-    const double *__restrict__ rowh = H ? p.fvh + (size_t)(fi >= 0 ? fi : f0) * p.nvp : nullptr;
+    const TS *__restrict__ rowg = p.fvg + (size_t)(fi >= 0 ? fi : f0) * p.nvp;
+    const TS *__restrict__ rowh = H ? p.fvh + (size_t)(fi >= 0 ? fi : f0) * p.nvp : nullptr;
     const double live = fi >= 0 ? 1.0 : 0.0;
+    const bool offrow = OFF && i == 15;
     double *W = Wl[wave];
     v4d c = {0.0, 0.0, 0.0, 0.0};
     const int nchunk = p.nvp >> 6;
@@ -973,14 +977,23 @@ __global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<double> p)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) wh[1 + d] = d < D ? hw * ch[d] : 0.0;
             }
-            // face values k0 + 16 kk + m of face i
-            const double *ga = rowg + k0 + 16 * kk;
+            // face values k0 + 16 kk + m of face i (row 15 in FP32 storage: the row offsets)
+            if (offrow) {
 #pragma unroll
-            for (int m2 = 0; m2 < 16; ++m2) a[m2] = live * ga[m2];
-            if (H) {
-                const double *ha = rowh + k0 + 16 * kk;
+                for (int m2 = 0; m2 < 16; ++m2) {
+                    const Vel vo = load_vel<D>(p, (unsigned)(k0 + 16 * kk + m2), vt);
+                    a[m2] = row_off<D>(p, vo, 0, vt);
+                    ah[m2] = H ? row_off<D>(p, vo, 1, vt) : 0.0;
+                }
+            } else {
+                const TS *ga = rowg + k0 + 16 * kk;
 #pragma unroll
-                for (int m2 = 0; m2 < 16; ++m2) ah[m2] = live * ha[m2];
+                for (int m2 = 0; m2 < 16; ++m2) a[m2] = live * (double)ga[m2];
+                if (H) {
+                    const TS *ha = rowh + k0 + 16 * kk;
+#pragma unroll
+                    for (int m2 = 0; m2 < 16; ++m2) ah[m2] = live * (double)ha[m2];
+                }
             }
         }
         __syncthreads();
@@ -1002,6 +1015,8 @@ __global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<double> p)
         }
         __syncthreads();
     }
+    // the wave partials D[row][j] go to the weight chunks (free after the loop's last barrier)
+    double(*Cr)[64 * MX_WS] = Wl;
 #pragma unroll
     for (int r = 0; r < 4; ++r) Cr[wave][(kk + 4 * r) * 16 + i] = c[r];
     __syncthreads();
@@ -1009,7 +1024,8 @@ __global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<double> p)
         const int row = threadIdx.x >> 4, j = threadIdx.x & 15;
         const int f = p.mxf[16 * tile + row];
         if (f >= 0) {
-            const double sum = ((Cr[0][row * 16 + j] + Cr[1][row * 16 + j]) + Cr[2][row * 16 + j]) + Cr[3][row * 16 + j];
+            double sum = ((Cr[0][row * 16 + j] + Cr[1][row * 16 + j]) + Cr[2][row * 16 + j]) + Cr[3][row * 16 + j];
+            if (OFF) sum += ((Cr[0][15 * 16 + j] + Cr[1][15 * 16 + j]) + Cr[2][15 * 16 + j]) + Cr[3][15 * 16 + j];
             double *fp = p.fpart + (size_t)f * p.nb * NM;
             if (j < NM) fp[j] = j < NMV ? sum : 0.0;
             for (int b = 1; b < p.nb; ++b)
@@ -1780,11 +1796,11 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
             hipLaunchKernelGGL((k_face_open<D, H, NS, TS>), dim3(h->nopen, h->nvb), dim3(h->nt), 0, h->stream, p);
         if (ev) (void)hipEventRecord(ev[1], h->stream);
         bool done = false;
-        if constexpr (sizeof(TS) == 8) {
+        {
             if (h->face_mfma) {
                 // non-wall faces on the matrix cores, wall faces (outgoing sums) on the vector units
                 if (h->nmx_tiles)
-                    hipLaunchKernelGGL((k_face_moments_mx<D, H>), dim3(h->nmx_tiles), dim3(MX_NT), 0, h->stream, p);
+                    hipLaunchKernelGGL((k_face_moments_mx<D, H, TS>), dim3(h->nmx_tiles), dim3(MX_NT), 0, h->stream, p);
                 if (h->nwallf) {
                     P<TS> pw = p;
                     pw.fsel = h->wallf;
@@ -2444,12 +2460,15 @@ int build(dugks_engine *h, const dugks_case *cs)
     if (h->nopen && ((r = dalloc(h, &h->open, open_faces.size())) || (r = upload(h, h->open, open_faces.data(), open_faces.size()))))
         return r;
     // face lists of the matrix-core face moments: non-wall faces in launch order, 16 per tile
-    // (-1 pads the last tile), and the wall faces, whose outgoing-only sums stay on k_face_moments
+    // (15 and an offset row with FP32 storage; -1 pads), and the wall faces, whose
+    // outgoing-only sums stay on k_face_moments
     {
-        std::vector<int> mx, wl;
-        for (int i = 0; i < nf; ++i) (fxs[forder[i]].kind == KIND_WALL ? wl : mx).push_back(forder[i]);
-        h->nmx_tiles = (int)((mx.size() + 15) / 16);
-        mx.resize((size_t)h->nmx_tiles * 16, -1);
+        std::vector<int> nw_faces, wl, mx;
+        for (int i = 0; i < nf; ++i) (fxs[forder[i]].kind == KIND_WALL ? wl : nw_faces).push_back(forder[i]);
+        const int per = h->storage == DUGKS_STORAGE_FP32 ? 15 : 16;
+        h->nmx_tiles = (int)((nw_faces.size() + per - 1) / per);
+        mx.assign((size_t)h->nmx_tiles * 16, -1);
+        for (size_t t = 0; t < nw_faces.size(); ++t) mx[(t / per) * 16 + t % per] = nw_faces[t];
         h->nwallf = (int)wl.size();
         if (h->nmx_tiles && ((r = dalloc(h, &h->mxf, mx.size())) || (r = upload(h, h->mxf, mx.data(), mx.size()))))
             return r;

@@ -213,8 +213,8 @@ class Engine:
 
     def face_mfma(self) -> bool:
         """True when the face moments of non-wall faces are FP64 matrix products
-        (k_face_moments_mx: FP64 storage, face-value mode, DUGKS_FACE_MFMA not 0)."""
-        return self._face_mfma and self.storage == "fp64" and self.face_values()
+        (k_face_moments_mx: face-value mode, DUGKS_FACE_MFMA not 0)."""
+        return self._face_mfma and self.face_values()
 
     def fused_cell(self) -> bool:
         """True when k_cell also forms the cell macros and the next step's S1 (face-value mode,

@@ -435,6 +435,23 @@ def test_face_moments_on_matrix_cores(which, monkeypatch):
         assert ra <= max(FP64_TOL, 2 * rel_l2(b.macro()[k], mo[k])), (which, k, ra)
     a.close()
     b.close()
+    # FP32 storage: the offsets of the stored deviations ride in row 15 of each tile
+    monkeypatch.setenv("DUGKS_FACE_MFMA", "1")
+    a = Engine(c, storage="fp32")
+    monkeypatch.setenv("DUGKS_FACE_MFMA", "0")
+    b = Engine(c, storage="fp32")
+    monkeypatch.delenv("DUGKS_FACE_MFMA")
+    assert a.face_mfma() and not b.face_mfma()
+    for e in (a, b):
+        e.init_equilibrium()
+        e.step(nsteps=n)
+    ma, mb = a.macro(), b.macro()
+    for k in ma:
+        r = rel_l2(ma[k], mb[k])
+        _say(f"  {which} fp32 {k}: matrix vs vector face moments {r:.2e}")
+        assert r <= 1e-6, (which, "fp32", k, r)
+    a.close()
+    b.close()
 
 
 @pytest.mark.parametrize("which", list(_small_cases()))
