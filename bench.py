@@ -154,7 +154,8 @@ def cpu_baseline(target_s: float = 12.0):
             "sample": f"FP64 oracle, 48^3 cells x 12^3 velocities, {n} step(s), {t:.1f} s"}
 
 
-SWEEP = (("cavity2d", 2000), ("delaunay", 150), ("cavity3d", 200), ("batch", 1000), ("channel", 2000))
+SWEEP = (("cavity2d", 2000), ("delaunay", 150), ("cavity3d", 200), ("batch", 1000), ("channel", 2000),
+         ("cavity3d_share", 200))
 
 
 def sweep():

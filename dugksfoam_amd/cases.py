@@ -763,13 +763,17 @@ def union(parts: List[Case], name: str = "union", meta: dict = None) -> Case:
 
 def config(name: str, **kw) -> Case:
     """Named workloads: 'cavity2d' (60^2 x 41^2), 'delaunay' (~200k triangles x 64^2),
-    'cavity3d' (48^3 x 24^3), 'batch' (256 cavities x 28^2), 'channel' (200x40 x 32^2)."""
+    'cavity3d' (48^3 x 24^3), 'batch' (256 cavities x 28^2), 'channel' (200x40 x 32^2), and
+    'cavity3d_share' (48^3 x 12^3: the mesh of 'cavity3d' with 1728 = 13824 / 8 velocities, the
+    velocity count one GPU holds when 'cavity3d' is split over 8 GPUs in velocity space)."""
     if name == "cavity2d":
         return cavity(**{**dict(n=60, nv=41, kn=0.075, dim=2), **kw})
     if name == "delaunay":
         return delaunay_cavity(**kw)
     if name == "cavity3d":
         return cavity(**{**dict(n=48, nv=24, kn=0.1, dim=3), **kw})
+    if name == "cavity3d_share":
+        return cavity(**{**dict(n=48, nv=12, kn=0.1, dim=3), **kw})
     if name == "batch":
         return batch(**kw)
     if name == "channel":
