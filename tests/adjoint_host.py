@@ -1,4 +1,4 @@
-"""Builds and binds tests/adjoint_host.cpp: the adjoint's own core and driver run on the host.
+"""Builds and binds tests/adjoint_host.cpp: adjoint_core.h and adjoint_driver.h run on the host.
 
 Test infrastructure (see adjoint_host.cpp).  HostAdjoint mirrors dugksfoam_amd.adjoint.Adjoint.
 """

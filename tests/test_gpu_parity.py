@@ -86,7 +86,7 @@ def test_face_fields_fp64(pair):
 def test_wall_data_fp64(pair):
     """rho_w at 1e-12; qWall and stressWall (fvDVM.C:539-581, diagnostics outside the
     compared fields) are velocity sums of terms ~ rho c^3 that cancel to a small
-    wall heat flux, so their rounding is judged at 1e-10."""
+    wall heat flux, so their rounding is compared at 1e-10."""
     name, c, o, e, _, _ = pair
     tol = dict(rho_w=FP64_TOL, q_wall=1e-10, stress_wall=1e-10)
     for i, p in enumerate(c.patches):
