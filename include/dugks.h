@@ -229,7 +229,7 @@ int dugks_abi_version(void);
  * Cases: one case, wall and cyclic patches, FP64 (the case's storage field is ignored); other
  * patch kinds give DUGKS_ERR_UNSUPPORTED.  Device memory: about
  * (ceil(n_steps / C) + C + 1) x (2 n_cells + 2 n_boundary_faces) x nv x 8 bytes for the trajectory
- * plus about (10 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;
+ * plus about (8 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;
  * DUGKS_ERR_MEMORY when that does not fit. */
 typedef struct dugks_adjoint dugks_adjoint;
 /* *out is set also when the call fails, so that dugks_adjoint_last_error can explain it;
