@@ -32,6 +32,7 @@ PATCH_FARFIELD = 4
 PATCH_FIXED = 5
 PATCH_PRESSURE_IN = 6
 PATCH_PRESSURE_OUT = 7
+PATCH_ZEROGRADIENT = 8
 
 # Boltzmann constant and argon data of demo/cavity/para.py:13-20
 K_BOLTZMANN = 1.38066e-23
@@ -424,18 +425,23 @@ def symmetry_cavity(nx: int = 30, ny: int = 60, nv: int = 28, kn: float = 0.075,
 def open_channel(nx: int = 20, ny: int = 8, Lx: float = 2.0, nv: int = 16, kn: float = 0.2,
                  rho_in: float = 1.05, u_in: float = 30.0, R: float = 208.0, T0: float = 273.0,
                  rho0: float = 1.0, xi_sigma: float = 4.0, Pr: float = 2.0 / 3.0, omega: float = 0.5,
-                 K: int = 0, co: float = 0.5) -> Case:
+                 K: int = 0, co: float = 0.5, outlet: str = "farField") -> Case:
     """Open 2-D channel [0, Lx] x [0, 1]: diffuse walls at y = 0, 1 (T0); the inlet x = 0 a fixed
     boundary Maxwellian (rho fixedValue: rho_in, U = (u_in, 0, 0), T0); the outlet x = Lx a far
-    field (rho farField: rho0, T0, incoming velocity from the outlet cells)."""
+    field (rho farField: rho0, T0, incoming velocity from the outlet cells) or, with
+    outlet="zeroGradient", a zeroGradient patch (face values = the outlet cells' values)."""
     d, C, V, own, nei, Sf, Cf, sides = box_mesh(nx, ny, 0, Lx, 1.0)
     walls = _wall("walls", _concat_sides([sides["y0"], sides["y1"]]), (0, 0, 0), T0)
     n_in, n_out = sides["x0"]["owner"].size, sides["x1"]["owner"].size
     inlet = Patch("inlet", PATCH_FIXED, sides["x0"]["owner"], sides["x0"]["Sf"], sides["x0"]["Cf"],
                   U=np.tile([u_in, 0.0, 0.0], (n_in, 1)), T=np.full(n_in, T0), rho=np.full(n_in, rho_in))
-    outlet = Patch("outlet", PATCH_FARFIELD, sides["x1"]["owner"], sides["x1"]["Sf"], sides["x1"]["Cf"],
-                   T=np.full(n_out, T0), rho=np.full(n_out, rho0))
-    geo = _assemble(f"openchannel_{nx}x{ny}_{nv}", d, C, V, own, nei, Sf, Cf, [walls, inlet, outlet])
+    if outlet == "zeroGradient":
+        out = Patch("outlet", PATCH_ZEROGRADIENT, sides["x1"]["owner"], sides["x1"]["Sf"], sides["x1"]["Cf"])
+    else:
+        out = Patch("outlet", PATCH_FARFIELD, sides["x1"]["owner"], sides["x1"]["Sf"], sides["x1"]["Cf"],
+                    T=np.full(n_out, T0), rho=np.full(n_out, rho0))
+    tag = "zg" if outlet == "zeroGradient" else ""
+    geo = _assemble(f"openchannel{tag}_{nx}x{ny}_{nv}", d, C, V, own, nei, Sf, Cf, [walls, inlet, out])
     xi_max = xi_sigma * math.sqrt(R * T0)
     xi, w = dv_newton_cotes(xi_max, nv)
     mu = mu_from_kn(kn, R, T0, rho0)

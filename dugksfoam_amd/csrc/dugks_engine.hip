@@ -61,9 +61,13 @@ constexpr int KIND_WALL = 2;  // Maxwell wall (fixedGradient gBarPvol, maxwellWa
 constexpr int KIND_CYC = 3;   // cyclic (coupled) patch face
 constexpr int KIND_SYM = 4;   // symmetry plane (symmetryMod rho, DVMsymmetry gSurf, fixedGradient gBarPvol)
 constexpr int KIND_FAR = 5;   // incoming Maxwellian: far field (farField rho) or fixed (fixedValue rho)
+constexpr int KIND_ZG = 6;    // zeroGradient: every face value is the owner cell's gBarPvol
 
 // Boundary faces with a lagged fixedGradient gBarPvol row; open faces relax only outgoing values.
-__host__ __device__ constexpr inline bool grad_row(int kind) { return kind == KIND_WALL || kind == KIND_SYM || kind == KIND_FAR; }
+__host__ __device__ constexpr inline bool grad_row(int kind)
+{
+    return kind == KIND_WALL || kind == KIND_SYM || kind == KIND_FAR || kind == KIND_ZG;
+}
 constexpr int MAXS = 8;       // faces per cell handled by k_cell
 constexpr int NAXMAX = 192;   // per-axis exp table entries (3 axes x 64 points)
 constexpr int MAXNT = 512;     // workgroup size cap (256 VGPRs per lane available)
@@ -744,7 +748,11 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                     if (D == 3) nh += HC[2] * dCo[2];
                     p.gBh_out[bi] = (TS)nh;
                 }
-                if (s > 0) {
+                if (kind == KIND_ZG) {
+                    // zeroGradient: the cell's gBarPvol for every velocity (discreteVelocity.C:551-555)
+                    fg[k] = (TS)gC;
+                    if (H) fh[k] = (TS)hC;
+                } else if (s > 0) {
                     fg[k] = (TS)face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx);
                     if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx);
                 }
@@ -2052,6 +2060,17 @@ void shape_candidates(dugks_engine *h)
 
 using dugks_lsq::mag3;
 
+// engine face kind of a non-cyclic boundary patch type
+int kind_of(int patch_type)
+{
+    switch (patch_type) {
+    case DUGKS_PATCH_WALL: return KIND_WALL;
+    case DUGKS_PATCH_SYMMETRY: return KIND_SYM;
+    case DUGKS_PATCH_ZEROGRADIENT: return KIND_ZG;
+    default: return KIND_FAR;   // far field, fixed, pressure inlet / outlet
+    }
+}
+
 int build(dugks_engine *h, const dugks_case *cs)
 {
     const int nc = cs->n_cells, nfi = cs->n_internal_faces, nf = cs->n_faces, nb = nf - nfi;
@@ -2078,7 +2097,8 @@ int build(dugks_engine *h, const dugks_case *cs)
             return fail(h, DUGKS_ERR_INVALID, "patch " + std::to_string(pi) + " face range outside the boundary");
         if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_SYMMETRY &&
             pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_FIXED &&
-            pp.type != DUGKS_PATCH_PRESSURE_IN && pp.type != DUGKS_PATCH_PRESSURE_OUT)
+            pp.type != DUGKS_PATCH_PRESSURE_IN && pp.type != DUGKS_PATCH_PRESSURE_OUT &&
+            pp.type != DUGKS_PATCH_ZEROGRADIENT)
             return fail(h, DUGKS_ERR_UNSUPPORTED, "patch " + std::to_string(pi) + ": unsupported type");
         const bool press = pp.type == DUGKS_PATCH_PRESSURE_IN || pp.type == DUGKS_PATCH_PRESSURE_OUT;
         const bool far = pp.type == DUGKS_PATCH_FARFIELD || pp.type == DUGKS_PATCH_FIXED || press;
@@ -2106,6 +2126,8 @@ int build(dugks_engine *h, const dugks_case *cs)
                     return fail(h, DUGKS_ERR_UNSUPPORTED,
                                 "symmetry patch " + std::to_string(pi) + " must be normal to a solution axis");
             }
+        } else if (pp.type == DUGKS_PATCH_ZEROGRADIENT) {
+            h->has_sym = 1;   // its incoming half is a stored face value: face-value mode only
         } else if (pp.type == DUGKS_PATCH_CYCLIC) {
             if (pp.neighbour_patch < 0 || pp.neighbour_patch >= cs->n_patches ||
                 cs->patches[pp.neighbour_patch].size != pp.size ||
@@ -2126,6 +2148,10 @@ int build(dugks_engine *h, const dugks_case *cs)
                 // a fixedGradient row like a wall's; the wall constants are placeholders
                 Tb[b] = cs->T_ref;
                 bsym[b] = sym_axis;
+                bwall[b] = nw++;
+            } else if (pp.type == DUGKS_PATCH_ZEROGRADIENT) {
+                // a fixedGradient row (discreteVelocity.C:436-470); wall constants are placeholders
+                Tb[b] = cs->T_ref;
                 bwall[b] = nw++;
             } else if (far) {
                 // a fixedGradient row; the wall record carries rho_b, T_b (and U_b)
@@ -2211,7 +2237,7 @@ int build(dugks_engine *h, const dugks_case *cs)
                 const int b = f - nfi;
                 for (int d = 0; d < 3; ++d) s.L[d] = pvec[3 * f + d];
                 if (bkind[b] != DUGKS_PATCH_CYCLIC) {
-                    s.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : (bkind[b] == DUGKS_PATCH_SYMMETRY ? KIND_SYM : KIND_FAR);
+                    s.kind = kind_of(bkind[b]);
                     s.bidx = bwall[b];
                     if (s.kind == KIND_SYM) s.other = bsym[b];
                     for (int d = 0; d < 3; ++d) s.dCo[d] = cs->Sf[3 * f + d] / magSf[f];
@@ -2249,10 +2275,11 @@ int build(dugks_engine *h, const dugks_case *cs)
         } else {
             const int b = f - nfi;
             if (bkind[b] != DUGKS_PATCH_CYCLIC) {
-                x.kind = bkind[b] == DUGKS_PATCH_WALL ? KIND_WALL : (bkind[b] == DUGKS_PATCH_SYMMETRY ? KIND_SYM : KIND_FAR);
+                x.kind = kind_of(bkind[b]);
                 x.bidx = bwall[b];
                 if (x.kind == KIND_SYM) x.N = bsym[b];
-                if (x.kind != KIND_WALL) open_faces.push_back(f);
+                // incoming halves written by k_face_open
+                if (x.kind == KIND_SYM || x.kind == KIND_FAR) open_faces.push_back(f);
                 for (int d = 0; d < 3; ++d) x.dN[d] = cs->Sf[3 * f + d] / magSf[f];
             } else {
                 x.kind = KIND_CYC;
@@ -2783,7 +2810,7 @@ int dugks_init_equilibrium(dugks_engine *h, const double *rho, const double *U, 
                 const int f = s.face;
                 if (s.kind == KIND_WALL) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];
-                } else if (s.kind == KIND_SYM || (s.kind == KIND_FAR && wc[s.bidx].mode != 1.0)) {
+                } else if (s.kind == KIND_SYM || s.kind == KIND_ZG || (s.kind == KIND_FAR && wc[s.bidx].mode != 1.0)) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = U[3 * c + d];   // U zeroGradient there
                 } else if (s.kind == KIND_FAR) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];

@@ -327,6 +327,9 @@ def read_case(path: str, dt: float = None) -> cases.Case:
             kind = cases.PATCH_PRESSURE_IN if rtype == "pressureIn" else cases.PATCH_PRESSURE_OUT
             plist.append(cases.Patch(b["name"], kind, owner[sl], Sf_all[sl], Cf_all[sl], T=Tp,
                                      pressure=_number(pk)))
+        elif rtype == "zeroGradient":
+            # zeroGradient DF patch (bcMap, discreteVelocity.C:276): face values = the cells' values
+            plist.append(cases.Patch(b["name"], cases.PATCH_ZEROGRADIENT, owner[sl], Sf_all[sl], Cf_all[sl]))
         elif rtype == "symmetryMod":
             # DVMsymmetry DF patch (bcMap, discreteVelocity.C:279): mirror plane
             plist.append(cases.Patch(b["name"], cases.PATCH_SYMMETRY, owner[sl], Sf_all[sl], Cf_all[sl]))

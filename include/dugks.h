@@ -61,9 +61,13 @@ enum dugks_patch_type {
     DUGKS_PATCH_PRESSURE_IN = 6,  /* rho pressureIn: a far field whose rho_b = p / (R T_b) is
                                   reset after every step (fvDVM.C:743-770); rho_b starts at 1
                                   as the patch field's constructor sets it                    */
-    DUGKS_PATCH_PRESSURE_OUT = 7  /* rho pressureOut: a far field whose rho_b and T_b follow the
+    DUGKS_PATCH_PRESSURE_OUT = 7, /* rho pressureOut: a far field whose rho_b and T_b follow the
                                   boundary cells' characteristics after every step
                                   (fvDVM.C:772-804); rho_b starts at 1                        */
+    DUGKS_PATCH_ZEROGRADIENT = 8  /* rho zeroGradient (gSurf zeroGradient): every face value is
+                                  the owner cell's gBarPvol (discreteVelocity.C:551-555); only
+                                  the outgoing half is relaxed in S7 (its field operators are
+                                  no-ops, zeroGradientFvsPatchField.H:163-181)                */
 };
 
 /* Storage precision of the resident distributions. */

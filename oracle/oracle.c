@@ -678,6 +678,11 @@ static void compute_k(oracle *o, int k, int want_faces)
                     o->gs[f] = o->rhob[b]*maxwell_by_rho(o, Uin, o->Tb[b], xi);
                     o->hs[f] = o->gs[f]*(o->R*o->Tb[b])*(o->K + 3 - o->dim);
                 }
+            } else if (o->bkind[b] == DUGKS_PATCH_ZEROGRADIENT) {
+                /* every velocity: the owner cell's gBarPvol (patchInternalField,
+                 * discreteVelocity.C:551-555) */
+                o->gs[f] = o->gbar[own];
+                o->hs[f] = o->hbar[own];
             } else if (o->bkind[b] == DUGKS_PATCH_SYMMETRY) {
                 if (s > 0) {
                     /* outgoing, discreteVelocity.C:673-689 */

@@ -11,7 +11,7 @@ import math
 import numpy as np
 
 from dugksfoam_amd.cases import (PATCH_CYCLIC, PATCH_FARFIELD, PATCH_FIXED, PATCH_PRESSURE_IN, PATCH_PRESSURE_OUT,
-                                 PATCH_SYMMETRY, PATCH_WALL)
+                                 PATCH_SYMMETRY, PATCH_WALL, PATCH_ZEROGRADIENT)
 
 
 class NumpyDUGKS:
@@ -63,6 +63,7 @@ class NumpyDUGKS:
         self.sym = np.nonzero(self.kind == PATCH_SYMMETRY)[0]
         self.far = np.nonzero(np.isin(self.kind, (PATCH_FARFIELD, PATCH_FIXED, PATCH_PRESSURE_IN, PATCH_PRESSURE_OUT)))[0]
         self.wall = np.nonzero(self.kind == PATCH_WALL)[0]
+        self.zg = np.nonzero(self.kind == PATCH_ZEROGRADIENT)[0]
         self.cyc = np.nonzero(self.kind == PATCH_CYCLIC)[0]
         self.bP = self.own[nfi:]
         self.bN = np.where(self.kind == PATCH_CYCLIC, self.own[nfi + np.maximum(self.partner, 0)], -1)
@@ -157,7 +158,7 @@ class NumpyDUGKS:
         phib_g = np.empty((xi.shape[0], self.nb))
         phib_h = np.empty_like(phib_g)
         cy = self.cyc
-        wl = np.concatenate([self.wall, self.sym, self.far])   # fixedGradient gBarPvol patches
+        wl = np.concatenate([self.wall, self.sym, self.far, self.zg])   # fixedGradient gBarPvol patches
         phib_g[:, wl] = gbar[:, self.bP[wl]] + self.gradB_g[:, wl] / self.dcoef[wl]
         phib_h[:, wl] = hbar[:, self.bP[wl]] + self.gradB_h[:, wl] / self.dcoef[wl]
         lsq_b_g = phib_g.copy()
@@ -204,6 +205,10 @@ class NumpyDUGKS:
             Mf = self.rhob[fb][None, :] * self._maxwell_by_rho(Uin, self.Tb[fb])
             gb[:, fb] = np.where(sb[:, fb] > 0, gP[:, fb], Mf)
             hb[:, fb] = np.where(sb[:, fb] > 0, hP[:, fb], Mf * (R * self.Tb[fb])[None, :] * (K + 3 - D))
+        if self.zg.size:
+            # zeroGradient: both halves are the cell's gBarPvol (no reconstruction)
+            gb[:, self.zg] = gbar[:, self.bP[self.zg]]
+            hb[:, self.zg] = hbar[:, self.bP[self.zg]]
         for b in self.sym:
             # symmetry plane: incoming = the mirrored velocity's outgoing reconstruction
             m = self.mirror[self.axis[b]]

@@ -89,13 +89,20 @@ def test_time_controls_follow_set_delta_t():
     assert foam.next_delta_t(0.1, 0.9, ctl) == 1.0
 
 
-def test_unsupported_patches_rejected(tmp_path):
+def test_zero_gradient_and_unsupported_patches(tmp_path):
+    """A zeroGradient rho patch becomes a zeroGradient DF patch (bcMap, discreteVelocity.C:276);
+    a rho boundary type outside the bcMap is rejected."""
     c = cases.cavity(n=4, nv=6)
     foam_box.cavity_case(str(tmp_path), c)
     p = tmp_path / "0" / "rho"
-    p.write_text(p.read_text().replace("type calculatedMaxwell;\n        value uniform 1.0;",
-                                       "type zeroGradient;\n        value uniform 1.0;", 1))
-    with pytest.raises(NotImplementedError, match="zeroGradient"):
+    text = p.read_text()
+    p.write_text(text.replace("type calculatedMaxwell;\n        value uniform 1.0;",
+                              "type zeroGradient;\n        value uniform 1.0;", 1))
+    r = foam.read_case(str(tmp_path))
+    assert cases.PATCH_ZEROGRADIENT in [q["type"] for q in r.patches]
+    p.write_text(text.replace("type calculatedMaxwell;\n        value uniform 1.0;",
+                              "type inletOutlet;\n        value uniform 1.0;", 1))
+    with pytest.raises(NotImplementedError, match="inletOutlet"):
         foam.read_case(str(tmp_path))
 
 

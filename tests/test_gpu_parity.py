@@ -44,6 +44,7 @@ def _small_cases():
         "symcavity_left": (lambda: cases.symmetry_cavity(5, 10, nv=10, plane="x0"), 8),
         "openchannel_12x6": (lambda: cases.open_channel(12, 6, nv=12), 10),
         "pchannel_12x6": (lambda: cases.pressure_channel(12, 6, nv=12), 10),
+        "zgchannel_12x6": (lambda: cases.open_channel(12, 6, nv=12, outlet="zeroGradient"), 10),
     }
 
 
@@ -123,7 +124,7 @@ def test_courant(pair):
 
 
 @pytest.mark.parametrize("which", ["cavity2d_12x21", "cavity3d_6x12", "delaunay_300", "channel_20x8", "symcavity_6x12",
-                                   "openchannel_12x6"])
+                                   "openchannel_12x6", "zgchannel_12x6"])
 def test_fp32_storage(which):
     mk, n = _small_cases()[which]
     c = mk()

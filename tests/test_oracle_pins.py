@@ -246,7 +246,8 @@ def test_independent_numpy_restatement(oracle_lib):
     from ref_numpy import NumpyDUGKS
     for c in (cases.cavity(5, 9), cases.cavity(3, 8, dim=3, kn=0.1), cases.channel(6, 4, 1.0, 0.6, nv=8),
               cases.symmetry_cavity(4, 6, nv=9), cases.symmetry_cavity(3, 6, nv=8, plane="x0"),
-              cases.open_channel(6, 4, nv=8), cases.pressure_channel(6, 4, nv=8)):
+              cases.open_channel(6, 4, nv=8), cases.pressure_channel(6, 4, nv=8),
+              cases.open_channel(6, 4, nv=8, outlet="zeroGradient")):
         o = _oracle(c)
         o.init_equilibrium()
         r = NumpyDUGKS(c)
@@ -262,6 +263,38 @@ def test_independent_numpy_restatement(oracle_lib):
         fo = o.face_macro()
         for k in ("rho", "U", "T", "q"):
             assert rel_l2(r.face_macro()[k], fo[k]) < tol[k], (c.name, k)
+
+
+def test_zero_gradient_face_is_the_cell_gbar(oracle_lib):
+    """On a zeroGradient patch every face value, incoming half included, is the owner cell's
+    gBarPvol (discreteVelocity.C:551-555): the face density and momentum of a step equal the
+    moments of (1 - r) g~ + r geq of the owner cell, formed from the state before the step with
+    the oracle's Shakhov routine.  A reconstructed face value (with the gradient term) or one
+    that keeps only the outgoing half does not match."""
+    c = cases.open_channel(8, 4, nv=8, outlet="zeroGradient")
+    o = _oracle(c)
+    o.init_equilibrium()
+    o.step(nsteps=3)                       # a developing flow, so the cells differ
+    nfi = c.n_internal_faces
+    out = next(p for p in c.patches if p["name"] == "outlet")
+    faces = np.arange(out["start"], out["start"] + out["size"])
+    cells = c.owner[faces]
+    g, _ = o.df(cells)
+    m = o.macro()
+    xi, w = o.velocities()
+    dt = c.dt
+    o.step(nsteps=1)
+    fm = o.face_macro()
+    for i, (f, cl) in enumerate(zip(faces, cells)):
+        tau = o.tau(m["T"][cl], m["rho"][cl])
+        r = 1.5 * dt / (2 * tau + dt)
+        geq = np.array([o.shakhov(m["rho"][cl], m["U"][cl], m["T"][cl], m["q"][cl], x)[0] for x in xi])
+        gbar = (1 - r) * g[i] + r * geq
+        rho = w @ gbar
+        mom = (w * gbar) @ xi
+        assert abs(fm["rho"][f] - rho) <= 1e-13 * rho, (f, fm["rho"][f], rho)
+        assert np.allclose(fm["rho"][f] * fm["U"][f], mom, rtol=1e-11, atol=1e-12 * rho)
+    assert nfi > 0
 
 
 def test_symmetry_plane_zero_normal_velocity(oracle_lib):
