@@ -933,6 +933,8 @@ __global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<TS> p)
     __shared__ double vt[VT_SIZE];
     __shared__ double Wl[MX_NT / 64][64 * MX_WS];
     __shared__ double Wh[H ? MX_NT / 64 : 1][H ? 64 * 4 : 1];
+    // FP32 row offsets of the chunk: g in column 16 (the padding) of the weight rows, h here
+    __shared__ double Ol[OFF && H ? MX_NT / 64 : 1][OFF && H ? 64 : 1];
     static_assert(64 * MX_WS >= 16 * 16, "the wave partials reuse the weight chunks");
     const int tile = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -985,15 +987,13 @@ __global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<TS> p)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) wh[1 + d] = d < D ? hw * ch[d] : 0.0;
             }
-            // face values k0 + 16 kk + m of face i (row 15 in FP32 storage: the row offsets)
-            if (offrow) {
-#pragma unroll
-                for (int m2 = 0; m2 < 16; ++m2) {
-                    const Vel vo = load_vel<D>(p, (unsigned)(k0 + 16 * kk + m2), vt);
-                    a[m2] = row_off<D>(p, vo, 0, vt);
-                    ah[m2] = H ? row_off<D>(p, vo, 1, vt) : 0.0;
-                }
-            } else {
+            if (OFF) {
+                wr[16] = row_off<D>(p, v, 0, vt);
+                if (H) Ol[wave][lane] = row_off<D>(p, v, 1, vt);
+            }
+            // face values k0 + 16 kk + m of face i (row 15 in FP32 storage: the row offsets,
+            // read from LDS after the barrier)
+            if (!offrow) {
                 const TS *ga = rowg + k0 + 16 * kk;
 #pragma unroll
                 for (int m2 = 0; m2 < 16; ++m2) a[m2] = live * (double)ga[m2];
@@ -1006,6 +1006,13 @@ __global__ __launch_bounds__(MX_NT) void k_face_moments_mx(P<TS> p)
         }
         __syncthreads();
         if (act) {
+            if (offrow) {
+#pragma unroll
+                for (int m2 = 0; m2 < 16; ++m2) {
+                    a[m2] = W[(16 * kk + m2) * MX_WS + 16];
+                    ah[m2] = H ? Ol[wave][16 * kk + m2] : 0.0;
+                }
+            }
             // B[kk][j] of product m2 = W[16 kk + m2][j], j = this lane's column i
             const double *wb = W + (16 * kk) * MX_WS + i;
 #pragma unroll
