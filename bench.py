@@ -150,8 +150,22 @@ def cpu_baseline(target_s: float = 12.0):
         n += 1
     o.close()
     rate = c.n_cells * c.nv * n / t / 1e9
-    return {"value": rate, "unit": "G cell-velocity updates/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, 48^3 cells x 12^3 velocities, {n} step(s), {t:.1f} s"}
+    out = {"value": rate, "unit": "G cell-velocity updates/s", "cores": threads, "kind": "port",
+           "sample": f"FP64 oracle, 48^3 cells x 12^3 velocities, {n} step(s), {t:.1f} s"}
+    # one core, on the 2-D cavity (60^2 cells x 41^2 velocities), about 10 s
+    c1 = cases.config("cavity2d")
+    o1 = Oracle(c1, threads=1)
+    o1.init_equilibrium()
+    n1, t1 = 0, 0.0
+    while t1 < 0.5 * target_s and n1 < 20:
+        t0 = time.perf_counter()
+        o1.step(nsteps=1)
+        t1 += time.perf_counter() - t0
+        n1 += 1
+    o1.close()
+    out["single_core"] = {"value": c1.n_cells * c1.nv * n1 / t1 / 1e9, "cores": 1,
+                          "sample": f"FP64 oracle, 60^2 cells x 41^2 velocities, {n1} step(s), {t1:.1f} s"}
+    return out
 
 
 SWEEP = (("cavity2d", 2000), ("delaunay", 150), ("cavity3d", 200), ("batch", 1000), ("channel", 2000),
