@@ -16,7 +16,6 @@ same init_equilibrium / step / macro / courant / wall methods can stand in for i
 from __future__ import annotations
 
 import argparse
-import math
 import os
 import sys
 import time

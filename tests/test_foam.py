@@ -13,7 +13,6 @@ import os
 import numpy as np
 import pytest
 
-from conftest import rel_l2
 from dugksfoam_amd import cases, foam, solver
 import foam_box
 
