@@ -7,7 +7,7 @@
 //                             Tag names the phase
 //   B::free_bytes()           device memory available for the trajectory
 //   B::tic(slot) / B::ms(a, b) device-time marks
-//   B::sync(), B::bad, B::msg error state
+//   B::sync(), B::bad, B::msg error state; B::clear_error() forgets a failed optional allocation
 // dugks_adjoint.hip instantiates it with the HIP backend; the CPU check build instantiates it with
 // plain loops.
 //
@@ -193,6 +193,9 @@ struct Engine {
     double *rho0 = nullptr, *U0 = nullptr, *T0 = nullptr, *finmac = nullptr;
     double *states = nullptr;
     size_t states_n = 0;
+    double *ftape = nullptr;   // face values and records of a segment's re-run steps
+    size_t ftape_n = 0;
+    int ftape_slots = 0;
     bool have_run = false;
     double fwd_ms = 0.0, rev_ms = 0.0;
     std::string err;
@@ -269,11 +272,19 @@ struct Engine {
 
     void copy_state(St d, St s) { be.copy(d.g, s.g, state_doubles() * sizeof(double)); }
 
-    void recompute(St a, St b, bool fwd, bool j)
+    size_t fslot_doubles() const { return 2 * nfv + 32 * (size_t)mesh.nf; }
+    double *fslot_g(int s) const { return ftape + (size_t)s * fslot_doubles(); }
+    double *fslot_h(int s) const { return fslot_g(s) + nfv; }
+    FaceRec *fslot_r(int s) const { return reinterpret_cast<FaceRec *>(fslot_g(s) + 2 * nfv); }
+
+    // one forward step's phases from state a; with fwd, the new state into b.  The face values
+    // and records go to the work arrays, or to face-tape slot `slot` when it is >= 0.
+    void recompute(St a, St b, bool fwd, bool j, int slot = -1)
     {
         const AP q = p;
-        double *gp_ = gp, *hp_ = hp, *gg_ = gg, *gh_ = gh, *gf_ = gf, *hf_ = hf;
-        FaceRec *fr_ = fr;
+        double *gp_ = gp, *hp_ = hp, *gg_ = gg, *gh_ = gh;
+        double *gf_ = slot >= 0 ? fslot_g(slot) : gf, *hf_ = slot >= 0 ? fslot_h(slot) : hf;
+        FaceRec *fr_ = slot >= 0 ? fslot_r(slot) : fr;
         double *gr = fwd ? abf : nullptr, *hr = fwd ? abh : nullptr;
         const double *cf = coef;
         double *ja = j ? jacc : nullptr;
@@ -284,13 +295,15 @@ struct Engine {
         if (fwd) be.template launch<k_cell>(q.nc, ADJ_FN { cell_fwd(x, q, a, i, gp_, hp_, gr, hr, b); });
     }
 
-    // reverse of step n (state a = state n); cur: index of the adjoint of state n + 1
-    void reverse(St a, int n, bool seed, int &cur)
+    // reverse of step n (state a = state n); cur: index of the adjoint of state n + 1.  The
+    // step's face values and records come from face-tape slot `slot` (>= 0) or are recomputed.
+    void reverse(St a, int n, bool seed, int &cur, int slot = -1)
     {
-        recompute(a, a, false, false);
+        if (slot < 0) recompute(a, a, false, false);
         const AP q = p;
-        const FaceRec *fr_ = fr;
-        const double *gf_ = gf, *hf_ = hf, *cf = seed ? coef : nullptr;
+        const FaceRec *fr_ = slot >= 0 ? fslot_r(slot) : fr;
+        const double *gf_ = slot >= 0 ? fslot_g(slot) : gf, *hf_ = slot >= 0 ? fslot_h(slot) : hf;
+        const double *cf = seed ? coef : nullptr;
         const double *ag1 = ag[cur], *ah1 = ah[cur], *AG1g = AGg[cur], *AG1h = AGh[cur];
         double *ag0 = ag[cur ^ 1], *ah0 = ah[cur ^ 1], *AG0g = AGg[cur ^ 1], *AG0h = AGh[cur ^ 1];
         double *abf_ = abf, *abh_ = abh, *dTw_ = dTw, *apg_ = apg, *aph_ = aph, *agg_ = agg, *agh_ = agh;
@@ -326,6 +339,26 @@ struct Engine {
             states = be.template alloc<double>(need);
             if (!states) return fail(DUGKS_ERR_MEMORY, "device allocation failed: " + be.msg);
             states_n = need;
+        }
+        // face tape: the face values and records of a segment's re-run steps, so that their
+        // reverse steps skip the recomputation; taken only where it fits beside the trajectory
+        {
+            const size_t want = (size_t)(C - 1) * fslot_doubles();
+            if (want != ftape_n) {
+                if (ftape) be.release(ftape);
+                ftape = nullptr;
+                ftape_n = 0;
+                ftape_slots = 0;
+                if (want && (double)want * sizeof(double) < 0.8 * be.free_bytes()) {
+                    ftape = be.template alloc<double>(want);
+                    if (ftape) {
+                        ftape_n = want;
+                        ftape_slots = C - 1;
+                    } else {
+                        be.clear_error();   // an optional buffer: recompute instead
+                    }
+                }
+            }
         }
         p.dt = dt;
         be.upload(rho0, rho, m.nc * sizeof(double));
@@ -365,8 +398,9 @@ struct Engine {
         for (int seg = ncp - 1; seg >= 0; --seg) {
             const int n0 = seg * C, n1 = std::min(nsteps, n0 + C);
             auto tape = [&](int n) { return n == n0 ? state(seg) : state(ncp + (n - n0 - 1)); };
-            for (int n = n0; n + 1 < n1; ++n) recompute(tape(n), tape(n + 1), true, false);
-            for (int n = n1 - 1; n >= n0; --n) reverse(tape(n), n, n + 1 >= first, ca);
+            auto slot = [&](int n) { return n + 1 < n1 && n - n0 < ftape_slots ? n - n0 : -1; };
+            for (int n = n0; n + 1 < n1; ++n) recompute(tape(n), tape(n + 1), true, false, slot(n));
+            for (int n = n1 - 1; n >= n0; --n) reverse(tape(n), n, n + 1 >= first, ca, slot(n));
         }
         be.tic(3);
         std::vector<double> dh(m.nf);

@@ -193,6 +193,12 @@ struct HipBackend {
         return t;
     }
     void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+    void clear_error()
+    {
+        (void)hipGetLastError();
+        bad = false;
+        msg.clear();
+    }
 };
 
 }  // namespace

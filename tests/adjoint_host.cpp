@@ -74,6 +74,11 @@ struct HostBackend {
     void tic(int i) { t[i] = std::chrono::steady_clock::now(); }
     double ms(int a, int b) { return std::chrono::duration<double, std::milli>(t[b] - t[a]).count(); }
     void sync() {}
+    void clear_error()
+    {
+        bad = false;
+        msg.clear();
+    }
 };
 
 }  // namespace
