@@ -234,7 +234,9 @@ int dugks_abi_version(void);
  * patch kinds give DUGKS_ERR_UNSUPPORTED.  Device memory: about
  * (ceil(n_steps / C) + C + 1) x (2 n_cells + 2 n_boundary_faces) x nv x 8 bytes for the trajectory
  * plus about (8 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;
- * DUGKS_ERR_MEMORY when that does not fit. */
+ * DUGKS_ERR_MEMORY when that does not fit.  Where it also fits, a face tape of
+ * (C - 1) x (2 nv + 32) x n_faces x 8 bytes spares the reverse steps a recomputation (results are
+ * identical without it). */
 typedef struct dugks_adjoint dugks_adjoint;
 /* *out is set also when the call fails, so that dugks_adjoint_last_error can explain it;
  * release it with dugks_adjoint_destroy either way. */
