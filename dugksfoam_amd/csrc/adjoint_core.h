@@ -8,13 +8,14 @@
 //   X::sum<N>(v)       replace the worker's partial sums v[N] by the point's totals,
 //   X::leader()        true for the one worker that writes per-point scalars.
 //
-// The step is fvDVM::evolution() (fvDVM.C:1086-1108) restricted to internal faces, Maxwell walls
-// and cyclic pairs (S1-S9 of the oracle).  State n is (g~, h~ [cell][k], the lagged fixedGradient
+// The step is fvDVM::evolution() (fvDVM.C:1086-1108) restricted to internal faces, Maxwell walls,
+// cyclic pairs, fixed-Maxwellian and zeroGradient patches (S1-S9 of the oracle).  State n is (g~, h~ [cell][k], the lagged fixedGradient
 // rows Gg, Gh [boundary face][k], the cell macros rho, U, T, q).  Its macros are the S9 values of
 // step n-1 (for n = 0 the given initial fields with q = 0).  The objective is
 //   J = sum over steps n >= first of sum_f c_f (rho U)_f . S_f,  (rho U)_f = sum_k w_k xi_k gbar_f,k
 // with gbar_f the face values of S4 (fvDVM.C:473-483), i.e. a weighted mass flux through faces.
-// Its gradient is taken with respect to the wall temperature of every wall face.
+// Its gradient is taken with respect to the temperature of every wall face and of every fixed
+// (frozen boundary Maxwellian) face.
 //
 // Reverse of one step (state n -> n+1), given the adjoint of g~^{n+1}, h~^{n+1} (with the adjoint
 // of the macros of n+1 already folded in) and of G^{n+1}:
@@ -39,6 +40,11 @@ namespace adj {
 constexpr double VSMALL = 1.0e-300;   // OpenFOAM VSMALL
 constexpr double PI = 3.14159265358979323846;
 constexpr int FK_INTERNAL = 0, FK_WALL = 1, FK_CYCLIC = 2;
+constexpr int FK_FIXED = 3;   // frozen incoming Maxwellian rho_b M(U_b, T_b) (rho fixedValue, gSurf mixed)
+constexpr int FK_ZG = 4;      // zeroGradient: every face value is the owner cell's gBarPvol
+
+// Non-coupled boundary faces: a lagged fixedGradient row, only the outgoing half relaxed.
+ADJ_HD bool open_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_ZG; }
 
 // Mesh, velocity set and gas; every array is indexed by global face / cell numbers.
 struct AP {
@@ -52,7 +58,8 @@ struct AP {
     const double *pvec, *nvec;   // least-squares vectors [nf][3], [nfi][3]
     const double *nhat, *adel;   // walls: Sf/|Sf|, 1/deltaCoeffs
     const double *xi, *wv;       // [nv][3], [nv], reference order
-    const double *Tw, *Uw;       // walls: T_w [nf], U_w [nf][3]
+    const double *Tw, *Uw;       // walls, fixed faces: T [nf], U [nf][3]
+    const double *rb;            // fixed faces: rho_b [nf]
     const double *win, *dwin;    // walls: inComingByRho and its T_w derivative
 };
 
@@ -352,7 +359,7 @@ ADJ_HD void grad_cell(X &x, const AP &p, St s, int c, const double *gp, const do
         if (Gng)
             for (int e = p.cstart[c]; e < p.cstart[c + 1]; ++e) {
                 const int f = p.cface[e];
-                if (f < nfi || p.fkind[f] != FK_WALL) continue;
+                if (f < nfi || p.fkind[f] == FK_CYCLIC) continue;
                 Gng[ix(f - nfi, nv, k)] = dot3(G, p.nhat + 3 * f);
                 Gnh[ix(f - nfi, nv, k)] = dot3(H, p.nhat + 3 * f);
             }
@@ -371,8 +378,13 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
     load_grad(p, gh, P, k, HP);
     const double *CP = p.C + 3 * P;
     const double dO[3] = {Cf[0] - CP[0] - hx[0], Cf[1] - CP[1] - hx[1], Cf[2] - CP[2] - hx[2]};
+    if (fk == FK_ZG) {
+        g = gp[ix(P, nv, k)];   // patchInternalField (discreteVelocity.C:551-555)
+        h = hp[ix(P, nv, k)];
+        return;
+    }
     const double gP = gp[ix(P, nv, k)] + dot3(GP, dO), hP = hp[ix(P, nv, k)] + dot3(HP, dO);
-    if (fk == FK_WALL) {
+    if (fk == FK_WALL || fk == FK_FIXED) {
         g = gP;
         h = hP;
         return;
@@ -438,7 +450,7 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *gp, const double *h
     double out[1] = {0.0};
     x.for_k(nv, [&](int k) {
         const double s = sdot(p.xi + 3 * k, S);
-        if (fk == FK_WALL && !(s > 0)) return;
+        if ((fk == FK_WALL || fk == FK_FIXED) && !(s > 0)) return;
         double g, h;
         face_value(p, f, k, s, gp, hp, gg, gh, g, h);
         gf[ix(f, nv, k)] = g;
@@ -457,6 +469,18 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *gp, const double *h
             const double g = rho_w * maxwell(p, p.Uw + 3 * f, Tw, xi, dM);
             gf[ix(f, nv, k)] = g;
             hf[ix(f, nv, k)] = g * (p.R * Tw) * p.kc;
+        });
+    }
+    if (fk == FK_FIXED) {
+        // frozen incoming Maxwellian rho_b M(U_b, T_b) (discreteVelocity.C:312-344, 556-573)
+        const double Tb = p.Tw[f], rb = p.rb[f];
+        x.for_k(nv, [&](int k) {
+            const double *xi = p.xi + 3 * k;
+            if (sdot(xi, S) > 0) return;
+            double dM;
+            const double g = rb * maxwell(p, p.Uw + 3 * f, Tb, xi, dM);
+            gf[ix(f, nv, k)] = g;
+            hf[ix(f, nv, k)] = g * (p.R * Tb) * p.kc;
         });
     }
     double a[5] = {0, 0, 0, 0, 0};
@@ -506,8 +530,8 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *gp, const double *h
     x.for_k(nv, [&](int k) {
         const double *xi = p.xi + 3 * k;
         const size_t i = ix(f, nv, k);
-        if (fk == FK_WALL && !(sdot(xi, S) > 0)) {
-            gr[i] = gf[i];   // incoming wall values are not relaxed (maxwellWall tag, mwFvs.H:161-181)
+        if (open_kind(fk) && !(sdot(xi, S) > 0)) {
+            gr[i] = gf[i];   // incoming halves are not relaxed (their patch operators are no-ops, *Fvs.H:161-181)
             hr[i] = hf[i];
             return;
         }
@@ -602,7 +626,7 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const FaceRec *fr, const double *
             bf += (s * dt) * ah1[ix(N, nv, k)] / VN;
         }
         double ab = af, bb = bf;
-        if (fk != FK_WALL || s > 0) {
+        if (!open_kind(fk) || s > 0) {
             double geq, heq;
             shakhov(p, e, xi, geq, heq);
             acc[0] += af * (geq - gf[i]) + bf * (heq - hf[i]);
@@ -637,7 +661,8 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const FaceRec *fr, const double *
     }
     double al, be[3], ga, de[3];
     macro_adj(p, m, acc + 1, th, al, be, ga, de);
-    const double Tw = fk == FK_WALL ? p.Tw[f] : 0.0, hfac = p.R * Tw * p.kc;
+    const double Tw = fk == FK_WALL || fk == FK_FIXED ? p.Tw[f] : 0.0, hfac = p.R * Tw * p.kc;
+    const double rbf = fk == FK_FIXED ? p.rb[f] : 0.0;
     double wa[2] = {0.0, 0.0};
     x.for_k(nv, [&](int k) {
         const double *xi = p.xi + 3 * k;
@@ -652,10 +677,22 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const FaceRec *fr, const double *
             wa[1] += t * R.rho_w * dM + bb * R.rho_w * M * p.R * p.kc;
             ab = 0.0;
             bb = 0.0;
+        } else if (fk == FK_FIXED && !(sdot(xi, S) > 0)) {
+            // incoming rho_b M(U_b, T_b): no state dependence, only the T_b derivative
+            double dM;
+            const double M = maxwell(p, p.Uw + 3 * f, Tw, xi, dM);
+            wa[1] += (ab + bb * hfac) * rbf * dM + bb * rbf * M * p.R * p.kc;
+            ab = 0.0;
+            bb = 0.0;
         }
         abf[i] = ab;
         abh[i] = bb;
     });
+    if (fk == FK_FIXED) {
+        x.template sum<2>(wa);
+        if (x.leader()) dTw[f] += wa[1];
+        return;
+    }
     if (fk != FK_WALL) return;
     x.template sum<2>(wa);
     const double oh = wa[0] / p.win[f], ih = -wa[0] * R.rho_w / p.win[f];
@@ -698,7 +735,15 @@ ADJ_HD void recon_adj(X &x, const AP &p, int c, const double *abf, const double 
                     G[q] += om * vg * d[q];
                     H[q] += om * vh * d[q];
                 }
-            } else if (p.fkind[f] == FK_WALL) {
+            } else if (p.fkind[f] == FK_ZG) {
+                a += vg;   // the cell's own value on every velocity
+                ah += vh;
+                const double bg = AGg[ix(f - nfi, nv, k)], bh = AGh[ix(f - nfi, nv, k)];
+                for (int q = 0; q < 3; ++q) {
+                    G[q] += bg * p.nhat[3 * f + q];
+                    H[q] += bh * p.nhat[3 * f + q];
+                }
+            } else if (p.fkind[f] == FK_WALL || p.fkind[f] == FK_FIXED) {
                 if (s > 0) {
                     a += vg;
                     ah += vh;

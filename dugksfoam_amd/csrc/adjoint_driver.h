@@ -50,7 +50,7 @@ struct Mesh {
     int D = 0, K = 0, nc = 0, nfi = 0, nf = 0, nv = 0;
     double R = 0, omega = 0, Tref = 0, mu = 0, Pr = 0;
     std::vector<int> cstart, cface, owner, nei, fkind, partner, pcell;
-    std::vector<double> S, Cf, C, V, wf, pvec, nvec, nhat, adel, xi, wv, Tw, Uw;
+    std::vector<double> S, Cf, C, V, wf, pvec, nvec, nhat, adel, xi, wv, Tw, Uw, rb;
 };
 
 // Validates the case and derives the per-face tables.  Returns nullptr or the reason (code set).
@@ -76,15 +76,18 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     std::vector<int> bkind(nb, 0), bpartner(nb, -1);
     m.Tw.assign(nf, 0.0);
     m.Uw.assign(3 * (size_t)nf, 0.0);
+    m.rb.assign(nf, 0.0);
     for (int pi = 0; pi < cs->n_patches; ++pi) {
         const dugks_patch &pp = cs->patches[pi];
         code = DUGKS_ERR_INVALID;
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf) return "patch face range outside the boundary";
         code = DUGKS_ERR_UNSUPPORTED;
-        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC)
-            return "the adjoint supports wall and cyclic patches only";
+        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_FIXED &&
+            pp.type != DUGKS_PATCH_ZEROGRADIENT)
+            return "the adjoint supports wall, cyclic, fixed and zeroGradient patches only";
         code = DUGKS_ERR_INVALID;
         if (pp.type == DUGKS_PATCH_WALL && (!pp.U || !pp.T)) return "wall patch without U/T";
+        if (pp.type == DUGKS_PATCH_FIXED && (!pp.U || !pp.T || !pp.rho)) return "fixed patch without rho, U, T";
         if (pp.type == DUGKS_PATCH_CYCLIC &&
             (pp.neighbour_patch < 0 || pp.neighbour_patch >= cs->n_patches ||
              cs->patches[pp.neighbour_patch].size != pp.size ||
@@ -94,10 +97,16 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
             const int b = pp.start - nfi + i, f = pp.start + i;
             if (bkind[b]) return "boundary face in two patches";
             bkind[b] = pp.type;
-            if (pp.type == DUGKS_PATCH_WALL) {
+            if (pp.type == DUGKS_PATCH_WALL || pp.type == DUGKS_PATCH_FIXED) {
                 m.Tw[f] = pp.T[i];
-                if (!(m.Tw[f] > 0)) return "wall temperature must be > 0";
+                if (!(m.Tw[f] > 0)) return "boundary temperature must be > 0";
                 for (int d = 0; d < 3; ++d) m.Uw[3 * (size_t)f + d] = pp.U[3 * i + d];
+                if (pp.type == DUGKS_PATCH_FIXED) {
+                    m.rb[f] = pp.rho[i];
+                    if (!(m.rb[f] > 0)) return "boundary density must be > 0";
+                }
+            } else if (pp.type == DUGKS_PATCH_ZEROGRADIENT) {
+                // nothing beyond the lagged gradient row
             } else {
                 bpartner[b] = cs->patches[pp.neighbour_patch].start - nfi + i;
             }
@@ -133,8 +142,8 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     m.adel.assign(nf, 0.0);
     for (int b = 0; b < nb; ++b) {
         const int f = nfi + b;
-        if (bkind[b] == DUGKS_PATCH_WALL) {
-            m.fkind[f] = FK_WALL;
+        if (bkind[b] != DUGKS_PATCH_CYCLIC) {
+            m.fkind[f] = bkind[b] == DUGKS_PATCH_WALL ? FK_WALL : (bkind[b] == DUGKS_PATCH_FIXED ? FK_FIXED : FK_ZG);
             const double mS = dugks_lsq::mag3(cs->Sf + 3 * f);
             for (int d = 0; d < 3; ++d) m.nhat[3 * (size_t)f + d] = cs->Sf[3 * f + d] / mS;
             m.adel[f] = 1.0 / (1.0 / dugks_lsq::mag3(cs->delta + 3 * b));
@@ -230,7 +239,7 @@ struct Engine {
         p.fkind = up(m.fkind); p.partner = up(m.partner); p.pcell = up(m.pcell);
         p.S = up(m.S); p.Cf = up(m.Cf); p.C = up(m.C); p.V = up(m.V); p.wf = up(m.wf);
         p.pvec = up(m.pvec); p.nvec = up(m.nvec); p.nhat = up(m.nhat); p.adel = up(m.adel);
-        p.xi = up(m.xi); p.wv = up(m.wv); p.Tw = up(m.Tw); p.Uw = up(m.Uw);
+        p.xi = up(m.xi); p.wv = up(m.wv); p.Tw = up(m.Tw); p.Uw = up(m.Uw); p.rb = up(m.rb);
         win = be.template alloc<double>(m.nf);
         dwin = be.template alloc<double>(m.nf);
         p.win = win;

@@ -9,21 +9,25 @@ import numpy as np
 from dugksfoam_amd import cases
 
 
+T_PATCHES = (cases.PATCH_WALL, cases.PATCH_FIXED)   # boundary kinds whose temperature J depends on
+
+
 def wall_mask(c) -> np.ndarray:
+    """Boundary faces with a temperature the adjoint differentiates: walls and fixed faces."""
     nfi, nb = c.n_internal_faces, c.n_faces - c.n_internal_faces
     m = np.zeros(nb, bool)
     for p in c.patches:
-        if p["type"] == cases.PATCH_WALL:
+        if p["type"] in T_PATCHES:
             m[p["start"] - nfi: p["start"] - nfi + p["size"]] = True
     return m
 
 
 def with_wall_T(c, dT):
-    """The case with T_w of boundary face b raised by dT[b]."""
+    """The case with the temperature of boundary face b (walls, fixed faces) raised by dT[b]."""
     c2 = copy.deepcopy(c)
     nfi = c.n_internal_faces
     for p in c2.patches:
-        if p["type"] == cases.PATCH_WALL:
+        if p["type"] in T_PATCHES:
             p["T"] = np.asarray(p["T"], float) + dT[p["start"] - nfi: p["start"] - nfi + p["size"]]
     return c2
 
@@ -59,6 +63,8 @@ CASES = [
     ("cavity3d", lambda: cases.cavity(3, 5, dim=3), 4, 2, 0),
     ("channel", lambda: cases.channel(nx=8, ny=4, Lx=1.0, nv=7, kn=0.5, n_modes=3), 6, 1, 2),
     ("channel_hot", lambda: cases.channel(nx=8, ny=4, Lx=1.0, nv=8, kn=0.2, n_modes=3, amp=0.2), 8, 5, 3),
+    # fixed-Maxwellian inlet (dJ/dT_b) and zeroGradient outlet
+    ("open_zg", lambda: cases.open_channel(6, 4, nv=8, outlet="zeroGradient"), 6, 2, 2),
 ]
 
 

@@ -74,5 +74,5 @@ def test_planted_mistake_is_caught():
 
 def test_unsupported_patches_rejected():
     c = cases.symmetry_cavity(4, 6, 6)
-    with pytest.raises(RuntimeError, match="wall and cyclic"):
+    with pytest.raises(RuntimeError, match="wall, cyclic"):
         HostAdjoint(c)
