@@ -119,9 +119,16 @@ struct alignas(16) FaceEq {
     double RT;
     double r;       // S7 relaxation factor h / (2 tau + h)
     double rho_w;   // wall density (walls only)
-    double pad[5];
+    double pad[5];  // walls: [0] T_w, [1] 1 / (2 pi R T_w)^(D/2)
+    // the equilibrium polynomial in xi: c2 - D - 2 = |xi|^2 invRT + B.xi + C0 and
+    // pc = -(U - xi).qc = Q0 + qc.xi, with c2 = |U - xi|^2 / RT
+    double B[3];    // -2 U / RT
+    double C0;      // |U|^2 / RT - (D + 2)
+    double Q0;      // -U.qc
+    double omr;     // 1 - r
+    double pad2[2];
 };
-static_assert(sizeof(FaceEq) == 128, "faceeq size");
+static_assert(sizeof(FaceEq) == 192, "faceeq size");
 
 // Wall face constants: wall velocity/temperature, inComingByRho and the moments of the incoming
 // Maxwellian (per unit rho_w) about the wall velocity.
@@ -325,6 +332,23 @@ DEVI void eq_eval(const EqC &e, const double *tab, int nxi, const Vel &v, double
     geq = (1.0 + pc * ((c2 - D) - 2.0)) * gB;
     if (H) heq = (kc + pc * ((c2 - D) * kc - 2.0 * K)) * gB * e.RT;
     else heq = 0.0;
+}
+
+// S7 relaxation of one face value towards the face's Shakhov equilibrium: (1 - r) gf + r geq_f
+// (discreteVelocity.C:867-920, 1016-1044).  tf points at the face's entry of the face-interleaved
+// exp tables (entry a of the face at tf[a * 2 NS]); the x entries carry r pre, so the table product
+// is r times the Gaussian part.  The polynomial factors use the face's expanded constants.
+template <int D, bool H, typename FE>
+DEVI void face_relax(const FE &fe, const double *tf, int xo, int yo, int zo, const Vel &v, double kc, int K,
+                     double &gf, double &hf)
+{
+#pragma clang fp contract(on)
+    double rgB = tf[xo] * tf[yo];
+    if (D == 3) rgB *= tf[zo];
+    const double P = v.xi2 * fe.invRT + fe.C0 + fe.B[0] * v.x + fe.B[1] * v.y + fe.B[2] * v.z;
+    const double pc = fe.Q0 + fe.qc[0] * v.x + fe.qc[1] * v.y + fe.qc[2] * v.z;
+    gf = fe.omr * gf + (1.0 + pc * P) * rgB;
+    if (H) hf = fe.omr * hf + (kc + pc * ((P + 2.0) * kc - 2.0 * K)) * rgB * fe.RT;
 }
 
 // Mesh constants are read through the constant address space: with a wave-uniform index the
@@ -1106,6 +1130,16 @@ __global__ __launch_bounds__(256) void k_face_fin(P<TS> p)
     fe.r = hh / (2 * tau + hh);
     fe.rho_w = rho_w;
     for (int d = 0; d < 5; ++d) fe.pad[d] = 0.0;
+    double uu = 0.0, uq = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        fe.B[d] = -2.0 * e.U[d] * e.invRT;
+        uu += e.U[d] * e.U[d];
+        uq += e.U[d] * e.qc[d];
+    }
+    fe.C0 = uu * e.invRT - (D + 2);
+    fe.Q0 = -uq;
+    fe.omr = 1.0 - fe.r;
+    fe.pad2[0] = fe.pad2[1] = 0.0;
     if (wall) {
         const WallC &W = p.wall[fx.bidx];
         fe.pad[0] = W.T;
@@ -1141,7 +1175,8 @@ __global__ __launch_bounds__(256) void k_face_fin(P<TS> p)
 // step is recovered from the resident gbar as (gbar - r geq) / (1 - r); the new g~ goes to the
 // output buffer, where k_cell_s1 turns it into the next gbar.  With F1 (face-value mode, one
 // workgroup per cell) the kernel also does the work of k_cell_fin and k_cell_s1 for its cell.
-// Dynamic LDS: [velocity table (3 nax + 2) | tabC | tabF x MAXS | tabW x MAXS], nax doubles each.
+// Dynamic LDS: [face tables (2 NS nax) | velocity table (3 nax + 2) | tabC (nax)].  The face tables
+// come first, so their addresses are a lane offset plus compile-time constants.
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, int NS, typename TS, bool FV, bool F1 = false>
 __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL_WAVES_FV : CELL_WAVES, 8))) void k_cell(P<TS> p)
@@ -1155,10 +1190,12 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
     if (!xcd_block(p.nc, p.nbc, pos, b)) return;
     const int c = cptr(p.corder)[pos];
     const int nax = p.nax;
-    double *vt = tabs;
-    double *tabC = tabs + 3 * nax + 2;
-    double *tabF = tabC + nax;
-    double *tabW = tabF + MAXS * nax;
+    // face-interleaved tables: relaxation entry a of face j at tabF[a 2NS + j], wall entry at
+    // tabW[a 2NS + j]
+    double *tabF = tabs;
+    double *tabW = tabs + NS;
+    double *vt = tabs + 2 * NS * nax;
+    double *tabC = vt + 3 * nax + 2;
     stage_vt(p, vt);
     const int cas = cptr(p.ccase)[c];
     const double dt = cptr(p.dtc)[cas];
@@ -1182,15 +1219,24 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
         const double u = Uc[ax] - p.axis[t - ax * p.nxi];
         tabC[t] = exp(-(u * u) * (0.5 * ec.invRT));
     }
-    // face equilibrium tables of every face and the wall Maxwellian tables of the wall faces
+    // face equilibrium tables of every face and the wall Maxwellian tables of the wall faces,
+    // interleaved by face so that the velocity loop addresses
+    // every face's entry from one lane offset; the x entries carry the factors of the whole
+    // product (r pre for the relaxation, rho_w pre_w for the incoming wall Maxwellian)
     for (int t = threadIdx.x; t < ns * nax; t += blockDim.x) {
         const int j = t / nax, a = t - j * nax;
         const int ax = a / p.nxi;
         const Slot &sj = p.slots[s0 + j];
         const FaceEq &fe = p.feq[sj.face];
         const double u = fe.U[ax] - p.axis[a - ax * p.nxi];
-        tabF[j * nax + a] = exp(-(u * u) * (0.5 * fe.invRT));
-        if (sj.kind == KIND_WALL) tabW[j * nax + a] = p.wexp[(size_t)sj.bidx * nax + a];
+        double e = exp(-(u * u) * (0.5 * fe.invRT));
+        if (ax == 0) e *= fe.r * fe.pre;
+        tabF[a * 2 * NS + j] = e;
+        if (sj.kind == KIND_WALL) {
+            double w = p.wexp[(size_t)sj.bidx * nax + a];
+            if (ax == 0) w *= fe.rho_w * fe.pad[1];
+            tabW[a * 2 * NS + j] = w;
+        }
     }
     __syncthreads();
 
@@ -1201,6 +1247,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
     for (int it = 0; it < p.nibc; ++it) {
         const unsigned k = k0 + it * p.ntc;
         const Vel v = load_vel<D>(p, k, vt);
+        const int xo = v.ix * 2 * NS, yo = (p.nxi + v.iy) * 2 * NS, zo = (2 * p.nxi + v.iz) * 2 * NS;
         const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
         const double gC = ld(rowC, k);                 // stored value (FP32: deviation)
         const double hC = H ? ld(hrowC, k) : 0.0;
@@ -1254,11 +1301,11 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
             const CONST_AS FaceEq &fe = cptr(p.feq)[S.face];
             double gf, hf = 0.0;
             if (kind == KIND_WALL && !(s > 0)) {
-                // S4 incoming Maxwellian, left unrelaxed by S7 (maxwellWall operators are no-ops)
-                const double *tw = tabW + j * nax;
-                double m = fe.pad[1] * tw[v.ix] * tw[p.nxi + v.iy];
-                if (D == 3) m *= tw[2 * p.nxi + v.iz];
-                gf = fe.rho_w * m;
+                // S4 incoming Maxwellian rho_w M_w, left unrelaxed by S7 (maxwellWall operators
+                // are no-ops); the x entries of the table carry rho_w pre_w
+                const double *tw = tabW + j;
+                gf = tw[xo] * tw[yo];
+                if (D == 3) gf *= tw[zo];
                 if (H) hf = gf * (p.R * fe.pad[0]) * p.kc;
             } else {
                 if (PRE) {
@@ -1277,26 +1324,13 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
                 }
                 // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium; on symmetry,
                 // far-field and fixed patches only the outgoing values (their operators are no-ops)
-                if (kind == KIND_OWN || kind == KIND_NEI || kind == KIND_CYC || s > 0) {
-                    EqC e;
-                    e.U[0] = fe.U[0]; e.U[1] = fe.U[1]; e.U[2] = fe.U[2];
-                    e.invRT = fe.invRT; e.pre = fe.pre; e.RT = fe.RT;
-                    e.qc[0] = fe.qc[0]; e.qc[1] = fe.qc[1]; e.qc[2] = fe.qc[2];
-                    double gq, hq;
-                    eq_eval<D, H>(e, tabF + j * nax, p.nxi, v, p.kc, p.K, gq, hq);
-                    const double r = fe.r;
-                    gf = (1.0 - r) * gf + r * gq;
-                    if (H) hf = (1.0 - r) * hf + r * hq;
-                }
+                if (kind == KIND_OWN || kind == KIND_NEI || kind == KIND_CYC || s > 0)
+                    face_relax<D, H>(fe, tabF + j, xo, yo, zo, v, p.kc, p.K, gf, hf);
             }
-            const double tg = div_rn(s * gf * dt, Vc, invV);
-            if (nei) ag += tg;
-            else ag -= tg;
-            if (H) {
-                const double th = div_rn(s * hf * dt, Vc, invV);
-                if (nei) ah += th;
-                else ah -= th;
-            }
+            // ag -/+ tg: the sign is exact, so the scalar-signed FMA rounds as the subtraction
+            const double sgn = nei ? 1.0 : -1.0;
+            ag = fma(sgn, div_rn(s * gf * dt, Vc, invV), ag);
+            if (H) ah = fma(sgn, div_rn(s * hf * dt, Vc, invV), ah);
         }
         p.gb_out[(size_t)c * p.nvp + k] = (TS)(ag - og);
         if (H) p.hb_out[(size_t)c * p.nvp + k] = (TS)(ah - oh);
