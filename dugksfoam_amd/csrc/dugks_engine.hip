@@ -247,6 +247,29 @@ DEVI double ld(const TS *a, size_t i)
     return (double)a[i];
 }
 
+// Store of a value that no later work of the same kernel reads (face values, the next step's
+// gbar): a streaming store, so that it does not displace rows still being re-read from L2.
+#ifndef DUGKS_NT_STORE
+#define DUGKS_NT_STORE 1
+#endif
+// Load of a value that no other workgroup of the kernel reads.
+#ifndef DUGKS_NT_LOAD
+#define DUGKS_NT_LOAD 1
+#endif
+template <typename TS>
+DEVI double ld_stream(const TS *a, size_t i)
+{
+    if (DUGKS_NT_LOAD) return (double)__builtin_nontemporal_load(a + i);
+    return (double)a[i];
+}
+
+template <typename TS>
+DEVI void st_stream(TS *a, size_t i, double v)
+{
+    if (DUGKS_NT_STORE) __builtin_nontemporal_store((TS)v, a + i);
+    else a[i] = (TS)v;
+}
+
 struct Vel {
     double x, y, z, w, xi2;
     int ix, iy, iz;
@@ -774,11 +797,11 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                 }
                 if (kind == KIND_ZG) {
                     // zeroGradient: the cell's gBarPvol for every velocity (discreteVelocity.C:551-555)
-                    fg[k] = (TS)gC;
-                    if (H) fh[k] = (TS)hC;
+                    st_stream(fg, k, gC);
+                    if (H) st_stream(fh, k, hC);
                 } else if (s > 0) {
-                    fg[k] = (TS)face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx);
-                    if (H) fh[k] = (TS)face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx);
+                    st_stream(fg, k, face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx));
+                    if (H) st_stream(fh, k, face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx));
                 }
                 continue;
             }
@@ -786,8 +809,8 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                 // this cell is the face's neighbour N: upwind where xi.S < -VSMALL
                 if (s < -VSMALL) {
                     const double z3[3] = {0, 0, 0};
-                    fg[k] = (TS)face_value<D>(KIND_OWN, s, 0.0, z3, dCo, gC, GC, dCs, 0.0, hx);
-                    if (H) fh[k] = (TS)face_value<D>(KIND_OWN, s, 0.0, z3, dCo, hC, HC, dCs, 0.0, hx);
+                    st_stream(fg, k, face_value<D>(KIND_OWN, s, 0.0, z3, dCo, gC, GC, dCs, 0.0, hx));
+                    if (H) st_stream(fh, k, face_value<D>(KIND_OWN, s, 0.0, z3, dCo, hC, HC, dCs, 0.0, hx));
                 }
                 continue;
             }
@@ -803,8 +826,8 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
                 grad_at<D, H, NS>(p, o, k, gO, hO, GO, HO);
             }
             if (mine) {
-                fg[k] = (TS)face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx);
-                if (H) fh[k] = (TS)face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx);
+                st_stream(fg, k, face_value<D>(kind, s, gC, GC, dCs, gO, GO, dCo, S.aux, hx));
+                if (H) st_stream(fh, k, face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx));
             }
         }
     }
@@ -1249,8 +1272,9 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
         const Vel v = load_vel<D>(p, k, vt);
         const int xo = v.ix * 2 * NS, yo = (p.nxi + v.iy) * 2 * NS, zo = (2 * p.nxi + v.iz) * 2 * NS;
         const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
-        const double gC = ld(rowC, k);                 // stored value (FP32: deviation)
-        const double hC = H ? ld(hrowC, k) : 0.0;
+        // stored value (FP32: deviation); in face-value mode no other workgroup reads this row
+        const double gC = FV ? ld_stream(rowC, k) : ld(rowC, k);
+        const double hC = H ? (FV ? ld_stream(hrowC, k) : ld(hrowC, k)) : 0.0;
         const double og = row_off<D>(p, v, 0, vt), oh = H ? row_off<D>(p, v, 1, vt) : 0.0;
         const double gCv = gC + og, hCv = hC + oh;      // gbar itself
         double GC[3] = {0, 0, 0}, HC[3] = {0, 0, 0};
@@ -1380,8 +1404,8 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
         double geq, heq;
         eq_eval<D, H>(en, tabC, p.nxi, v, p.kc, p.K, geq, heq);
         const double og = row_off<D>(p, v, 0, vt), oh = H ? row_off<D>(p, v, 1, vt) : 0.0;
-        go[k] = (TS)s1_value(gv, og, rn, geq);
-        if (H) ho[k] = (TS)s1_value(hv, oh, rn, heq);
+        st_stream(go, k, s1_value(gv, og, rn, geq));
+        if (H) st_stream(ho, k, s1_value(hv, oh, rn, heq));
     }
 }
 
