@@ -51,8 +51,8 @@ def _f64(a):
 
 
 class Adjoint:
-    """Discrete adjoint of the DUGKS step for one case (wall, cyclic, fixed and zeroGradient
-    patches) on one GPU."""
+    """Discrete adjoint of the DUGKS step for one case (wall, cyclic, fixed, zeroGradient and
+    symmetry patches) on one GPU."""
 
     def __init__(self, case, device: int = 0):
         L = _lib()

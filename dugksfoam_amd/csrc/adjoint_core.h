@@ -42,9 +42,10 @@ constexpr double PI = 3.14159265358979323846;
 constexpr int FK_INTERNAL = 0, FK_WALL = 1, FK_CYCLIC = 2;
 constexpr int FK_FIXED = 3;   // frozen incoming Maxwellian rho_b M(U_b, T_b) (rho fixedValue, gSurf mixed)
 constexpr int FK_ZG = 4;      // zeroGradient: every face value is the owner cell's gBarPvol
+constexpr int FK_SYM = 5;     // symmetry plane: incoming value = outgoing value of the mirrored velocity
 
 // Non-coupled boundary faces: a lagged fixedGradient row, only the outgoing half relaxed.
-ADJ_HD bool open_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_ZG; }
+ADJ_HD bool open_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_ZG || fk == FK_SYM; }
 
 // Mesh, velocity set and gas; every array is indexed by global face / cell numbers.
 struct AP {
@@ -61,6 +62,8 @@ struct AP {
     const double *Tw, *Uw;       // walls, fixed faces: T [nf], U [nf][3]
     const double *rb;            // fixed faces: rho_b [nf]
     const double *win, *dwin;    // walls: inComingByRho and its T_w derivative
+    const int *sax;              // symmetry faces: the mirror axis [nf]
+    const int *mir;              // [3][nv]: the velocity mirrored across axis a
 };
 
 // One state: g, h [nc][nv]; Gg, Gh [nf - nfi][nv]; mac [nc][8] = rho, U[3], T, q[3].
@@ -383,8 +386,22 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
         h = hp[ix(P, nv, k)];
         return;
     }
+    if (fk == FK_SYM && !(s > 0)) {
+        // incoming: the outgoing reconstruction of the mirrored velocity, at its own index
+        // (discreteVelocity.C:746-790); xi.S == 0 mirrors onto itself
+        const int km = p.mir[p.sax[f] * nv + k];
+        const double *xm = p.xi + 3 * km;
+        const double hm[3] = {0.5 * xm[0] * p.dt, 0.5 * xm[1] * p.dt, 0.5 * xm[2] * p.dt};
+        double GM[3], HM[3];
+        load_grad(p, gg, P, km, GM);
+        load_grad(p, gh, P, km, HM);
+        const double dM[3] = {Cf[0] - CP[0] - hm[0], Cf[1] - CP[1] - hm[1], Cf[2] - CP[2] - hm[2]};
+        g = gp[ix(P, nv, km)] + dot3(GM, dM);
+        h = hp[ix(P, nv, km)] + dot3(HM, dM);
+        return;
+    }
     const double gP = gp[ix(P, nv, k)] + dot3(GP, dO), hP = hp[ix(P, nv, k)] + dot3(HP, dO);
-    if (fk == FK_WALL || fk == FK_FIXED) {
+    if (fk == FK_WALL || fk == FK_FIXED || fk == FK_SYM) {
         g = gP;
         h = hP;
         return;
@@ -743,14 +760,22 @@ ADJ_HD void recon_adj(X &x, const AP &p, int c, const double *abf, const double 
                     G[q] += bg * p.nhat[3 * f + q];
                     H[q] += bh * p.nhat[3 * f + q];
                 }
-            } else if (p.fkind[f] == FK_WALL || p.fkind[f] == FK_FIXED) {
-                if (s > 0) {
-                    a += vg;
-                    ah += vh;
-                    for (int q = 0; q < 3; ++q) {
-                        G[q] += vg * d[q];
-                        H[q] += vh * d[q];
+            } else if (p.fkind[f] == FK_WALL || p.fkind[f] == FK_FIXED || p.fkind[f] == FK_SYM) {
+                // outgoing value of this velocity; on a symmetry plane also the incoming value of
+                // its mirror, which is the same reconstruction
+                double wg = s > 0 ? vg : 0.0, wh = s > 0 ? vh : 0.0;
+                if (p.fkind[f] == FK_SYM) {
+                    const int km = p.mir[p.sax[f] * nv + k];
+                    if (!(sdot(p.xi + 3 * km, p.S + 3 * f) > 0)) {
+                        wg += abf[ix(f, nv, km)];
+                        wh += abh[ix(f, nv, km)];
                     }
+                }
+                a += wg;
+                ah += wh;
+                for (int q = 0; q < 3; ++q) {
+                    G[q] += wg * d[q];
+                    H[q] += wh * d[q];
                 }
                 const double bg = AGg[ix(f - nfi, nv, k)], bh = AGh[ix(f - nfi, nv, k)];
                 for (int q = 0; q < 3; ++q) {

@@ -51,6 +51,7 @@ struct Mesh {
     double R = 0, omega = 0, Tref = 0, mu = 0, Pr = 0;
     std::vector<int> cstart, cface, owner, nei, fkind, partner, pcell;
     std::vector<double> S, Cf, C, V, wf, pvec, nvec, nhat, adel, xi, wv, Tw, Uw, rb;
+    std::vector<int> sax, mir;
 };
 
 // Validates the case and derives the per-face tables.  Returns nullptr or the reason (code set).
@@ -77,14 +78,25 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     m.Tw.assign(nf, 0.0);
     m.Uw.assign(3 * (size_t)nf, 0.0);
     m.rb.assign(nf, 0.0);
+    m.sax.assign(nf, 0);
     for (int pi = 0; pi < cs->n_patches; ++pi) {
         const dugks_patch &pp = cs->patches[pi];
         code = DUGKS_ERR_INVALID;
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf) return "patch face range outside the boundary";
         code = DUGKS_ERR_UNSUPPORTED;
         if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_FIXED &&
-            pp.type != DUGKS_PATCH_ZEROGRADIENT)
-            return "the adjoint supports wall, cyclic, fixed and zeroGradient patches only";
+            pp.type != DUGKS_PATCH_ZEROGRADIENT && pp.type != DUGKS_PATCH_SYMMETRY)
+            return "the adjoint supports wall, cyclic, fixed, zeroGradient and symmetry patches only";
+        int axis = 0;
+        if (pp.type == DUGKS_PATCH_SYMMETRY && pp.size) {
+            // the mirror axis from the patch's first face normal (discreteVelocity.C:766-768)
+            const double *S0 = cs->Sf + 3 * (size_t)pp.start;
+            const double mS = dugks_lsq::mag3(S0);
+            for (int d = 1; d < 3; ++d)
+                if (std::fabs(S0[d]) > std::fabs(S0[axis])) axis = d;
+            if (!(mS > 0) || !(std::fabs(S0[axis]) / mS > 1.0 - 1e-9) || axis >= cs->dim)
+                return "symmetry patch must be normal to a solution axis";
+        }
         code = DUGKS_ERR_INVALID;
         if (pp.type == DUGKS_PATCH_WALL && (!pp.U || !pp.T)) return "wall patch without U/T";
         if (pp.type == DUGKS_PATCH_FIXED && (!pp.U || !pp.T || !pp.rho)) return "fixed patch without rho, U, T";
@@ -107,6 +119,8 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
                 }
             } else if (pp.type == DUGKS_PATCH_ZEROGRADIENT) {
                 // nothing beyond the lagged gradient row
+            } else if (pp.type == DUGKS_PATCH_SYMMETRY) {
+                m.sax[f] = axis;
             } else {
                 bpartner[b] = cs->patches[pp.neighbour_patch].start - nfi + i;
             }
@@ -143,7 +157,10 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     for (int b = 0; b < nb; ++b) {
         const int f = nfi + b;
         if (bkind[b] != DUGKS_PATCH_CYCLIC) {
-            m.fkind[f] = bkind[b] == DUGKS_PATCH_WALL ? FK_WALL : (bkind[b] == DUGKS_PATCH_FIXED ? FK_FIXED : FK_ZG);
+            m.fkind[f] = bkind[b] == DUGKS_PATCH_WALL    ? FK_WALL
+                         : bkind[b] == DUGKS_PATCH_FIXED   ? FK_FIXED
+                         : bkind[b] == DUGKS_PATCH_SYMMETRY ? FK_SYM
+                                                            : FK_ZG;
             const double mS = dugks_lsq::mag3(cs->Sf + 3 * f);
             for (int d = 0; d < 3; ++d) m.nhat[3 * (size_t)f + d] = cs->Sf[3 * f + d] / mS;
             m.adel[f] = 1.0 / (1.0 / dugks_lsq::mag3(cs->delta + 3 * b));
@@ -180,6 +197,18 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
                 m.xi[3 * k] = X[ixx];
                 m.xi[3 * k + 1] = X[iy];
                 m.xi[3 * k + 2] = m.D == 3 ? X[iz] : 0.0;
+            }
+    // mirrored velocities (symmXtgID / symmYtgID / symmZtgID, fvDVM.C:152-166, 183-195): the 1-D
+    // set is symmetric, so index i mirrors to n - 1 - i
+    m.mir.assign(3 * (size_t)m.nv, 0);
+    const int nz = m.D == 3 ? n : 1;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+            for (int ixx = 0; ixx < n; ++ixx) {
+                const int kk = (iz * n + iy) * n + ixx;
+                m.mir[kk] = (iz * n + iy) * n + (n - 1 - ixx);
+                m.mir[m.nv + kk] = (iz * n + (n - 1 - iy)) * n + ixx;
+                m.mir[2 * m.nv + kk] = ((nz - 1 - iz) * n + iy) * n + ixx;
             }
     code = DUGKS_OK;
     return nullptr;
@@ -240,6 +269,7 @@ struct Engine {
         p.S = up(m.S); p.Cf = up(m.Cf); p.C = up(m.C); p.V = up(m.V); p.wf = up(m.wf);
         p.pvec = up(m.pvec); p.nvec = up(m.nvec); p.nhat = up(m.nhat); p.adel = up(m.adel);
         p.xi = up(m.xi); p.wv = up(m.wv); p.Tw = up(m.Tw); p.Uw = up(m.Uw); p.rb = up(m.rb);
+        p.sax = up(m.sax); p.mir = up(m.mir);
         win = be.template alloc<double>(m.nf);
         dwin = be.template alloc<double>(m.nf);
         p.win = win;

@@ -231,8 +231,8 @@ int dugks_abi_version(void);
  * The run starts from the Shakhov equilibrium of rho, U, T with q = 0 (as
  * dugks_init_equilibrium) and advances n_steps FP64 steps of size dt, keeping a checkpoint every
  * checkpoint_every steps (0: ceil(sqrt(n_steps))); the reverse sweep re-runs each segment.
- * Cases: one case; wall, cyclic, fixed and zeroGradient patches; FP64 (the case's storage field
- * is ignored).  Symmetry, far-field and pressure patches give DUGKS_ERR_UNSUPPORTED.  Device memory: about
+ * Cases: one case; wall, cyclic, fixed, zeroGradient and symmetry patches; FP64 (the case's
+ * storage field is ignored).  Far-field and pressure patches give DUGKS_ERR_UNSUPPORTED.  Device memory: about
  * (ceil(n_steps / C) + C + 1) x (2 n_cells + 2 n_boundary_faces) x nv x 8 bytes for the trajectory
  * plus about (8 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;
  * DUGKS_ERR_MEMORY when that does not fit.  Where it also fits, a face tape of

@@ -65,6 +65,9 @@ CASES = [
     ("channel_hot", lambda: cases.channel(nx=8, ny=4, Lx=1.0, nv=8, kn=0.2, n_modes=3, amp=0.2), 8, 5, 3),
     # fixed-Maxwellian inlet (dJ/dT_b) and zeroGradient outlet
     ("open_zg", lambda: cases.open_channel(6, 4, nv=8, outlet="zeroGradient"), 6, 2, 2),
+    # symmetry plane: incoming values are the mirrored velocities' outgoing reconstructions
+    ("symmetry_nv6", lambda: cases.symmetry_cavity(4, 6, 6), 6, 2, 0),
+    ("symmetry_nv7_x0", lambda: cases.symmetry_cavity(3, 6, 7, plane="x0"), 5, 1, 2),  # xi.S == 0: self-mirror
 ]
 
 
