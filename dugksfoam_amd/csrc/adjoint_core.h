@@ -43,9 +43,12 @@ constexpr int FK_INTERNAL = 0, FK_WALL = 1, FK_CYCLIC = 2;
 constexpr int FK_FIXED = 3;   // frozen incoming Maxwellian rho_b M(U_b, T_b) (rho fixedValue, gSurf mixed)
 constexpr int FK_ZG = 4;      // zeroGradient: every face value is the owner cell's gBarPvol
 constexpr int FK_SYM = 5;     // symmetry plane: incoming value = outgoing value of the mirrored velocity
+constexpr int FK_FAR = 6;     // far field: incoming rho_b M(U of the owner cell, T_b)
 
 // Non-coupled boundary faces: a lagged fixedGradient row, only the outgoing half relaxed.
-ADJ_HD bool open_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_ZG || fk == FK_SYM; }
+ADJ_HD bool open_kind(int fk) { return fk != FK_INTERNAL && fk != FK_CYCLIC; }
+// Boundary faces whose incoming half is a Maxwellian at the boundary temperature (dJ/dT output).
+ADJ_HD bool maxwell_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_FAR; }
 
 // Mesh, velocity set and gas; every array is indexed by global face / cell numbers.
 struct AP {
@@ -59,8 +62,8 @@ struct AP {
     const double *pvec, *nvec;   // least-squares vectors [nf][3], [nfi][3]
     const double *nhat, *adel;   // walls: Sf/|Sf|, 1/deltaCoeffs
     const double *xi, *wv;       // [nv][3], [nv], reference order
-    const double *Tw, *Uw;       // walls, fixed faces: T [nf], U [nf][3]
-    const double *rb;            // fixed faces: rho_b [nf]
+    const double *Tw, *Uw;       // walls, fixed and far-field faces: T [nf]; walls, fixed: U [nf][3]
+    const double *rb;            // fixed and far-field faces: rho_b [nf]
     const double *win, *dwin;    // walls: inComingByRho and its T_w derivative
     const int *sax;              // symmetry faces: the mirror axis [nf]
     const int *mir;              // [3][nv]: the velocity mirrored across axis a
@@ -401,7 +404,7 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
         return;
     }
     const double gP = gp[ix(P, nv, k)] + dot3(GP, dO), hP = hp[ix(P, nv, k)] + dot3(HP, dO);
-    if (fk == FK_WALL || fk == FK_FIXED || fk == FK_SYM) {
+    if (fk == FK_WALL || fk == FK_FIXED || fk == FK_SYM || fk == FK_FAR) {
         g = gP;
         h = hP;
         return;
@@ -458,16 +461,16 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
 // macros (S6) and, when gr is given, the relaxed face values (S7).  jacc[f] += c_f (rho U)_f . S_f
 // when jacc is given.
 template <class X>
-ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *gp, const double *hp, const double *gg,
-                     const double *gh, double *gf, double *hf, FaceRec *fr, double *gr, double *hr,
-                     const double *coef, double *jacc)
+ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, const double *gp, const double *hp,
+                     const double *gg, const double *gh, double *gf, double *hf, FaceRec *fr, double *gr,
+                     double *hr, const double *coef, double *jacc)
 {
     const int nv = p.nv, fk = f < p.nfi ? FK_INTERNAL : p.fkind[f];
     const double *S = p.S + 3 * f;
     double out[1] = {0.0};
     x.for_k(nv, [&](int k) {
         const double s = sdot(p.xi + 3 * k, S);
-        if ((fk == FK_WALL || fk == FK_FIXED) && !(s > 0)) return;
+        if (maxwell_kind(fk) && !(s > 0)) return;
         double g, h;
         face_value(p, f, k, s, gp, hp, gg, gh, g, h);
         gf[ix(f, nv, k)] = g;
@@ -488,14 +491,16 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *gp, const double *h
             hf[ix(f, nv, k)] = g * (p.R * Tw) * p.kc;
         });
     }
-    if (fk == FK_FIXED) {
-        // frozen incoming Maxwellian rho_b M(U_b, T_b) (discreteVelocity.C:312-344, 556-573)
+    if (fk == FK_FIXED || fk == FK_FAR) {
+        // incoming rho_b M(U, T_b): fixed faces freeze U_b (discreteVelocity.C:312-344, 556-573),
+        // far-field faces take the owner cell's velocity (discreteVelocity.C:588-602)
         const double Tb = p.Tw[f], rb = p.rb[f];
+        const double *Ub = fk == FK_FAR ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
         x.for_k(nv, [&](int k) {
             const double *xi = p.xi + 3 * k;
             if (sdot(xi, S) > 0) return;
             double dM;
-            const double g = rb * maxwell(p, p.Uw + 3 * f, Tb, xi, dM);
+            const double g = rb * maxwell(p, Ub, Tb, xi, dM);
             gf[ix(f, nv, k)] = g;
             hf[ix(f, nv, k)] = g * (p.R * Tb) * p.kc;
         });
@@ -620,9 +625,9 @@ ADJ_HD void cell_fwd(X &x, const AP &p, St s, int c, const double *gp, const dou
 // Writes the adjoint of the reconstructed face values (zero for incoming wall values) and adds
 // this step's dJ/dT_w of a wall face.  coef null: the step is outside the objective's window.
 template <class X>
-ADJ_HD void face_adj(X &x, const AP &p, int f, const FaceRec *fr, const double *gf, const double *hf,
-                     const double *ag1, const double *ah1, const double *coef, double *abf, double *abh,
-                     double *dTw)
+ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, const FaceRec *fr, const double *gf,
+                     const double *hf, const double *ag1, const double *ah1, const double *coef, double *abf,
+                     double *abh, double *dTw, double *aUf)
 {
     const int nv = p.nv, fk = f < p.nfi ? FK_INTERNAL : p.fkind[f];
     const double *S = p.S + 3 * f;
@@ -678,9 +683,10 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const FaceRec *fr, const double *
     }
     double al, be[3], ga, de[3];
     macro_adj(p, m, acc + 1, th, al, be, ga, de);
-    const double Tw = fk == FK_WALL || fk == FK_FIXED ? p.Tw[f] : 0.0, hfac = p.R * Tw * p.kc;
-    const double rbf = fk == FK_FIXED ? p.rb[f] : 0.0;
-    double wa[2] = {0.0, 0.0};
+    const double Tw = maxwell_kind(fk) ? p.Tw[f] : 0.0, hfac = p.R * Tw * p.kc;
+    const double rbf = fk == FK_FIXED || fk == FK_FAR ? p.rb[f] : 0.0;
+    const double *Ub = fk == FK_FAR ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
+    double wa[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // walls: M sum, dT; fixed, far field: dT, dU (far)
     x.for_k(nv, [&](int k) {
         const double *xi = p.xi + 3 * k;
         const size_t i = ix(f, nv, k);
@@ -694,20 +700,28 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const FaceRec *fr, const double *
             wa[1] += t * R.rho_w * dM + bb * R.rho_w * M * p.R * p.kc;
             ab = 0.0;
             bb = 0.0;
-        } else if (fk == FK_FIXED && !(sdot(xi, S) > 0)) {
-            // incoming rho_b M(U_b, T_b): no state dependence, only the T_b derivative
+        } else if ((fk == FK_FIXED || fk == FK_FAR) && !(sdot(xi, S) > 0)) {
+            // incoming rho_b M(U, T_b): the T_b derivative, and on far-field faces the derivative
+            // in the owner cell's velocity, dM/dU = M (xi - U) / (R T_b)
             double dM;
-            const double M = maxwell(p, p.Uw + 3 * f, Tw, xi, dM);
-            wa[1] += (ab + bb * hfac) * rbf * dM + bb * rbf * M * p.R * p.kc;
+            const double M = maxwell(p, Ub, Tw, xi, dM);
+            const double t = ab + bb * hfac;
+            wa[1] += t * rbf * dM + bb * rbf * M * p.R * p.kc;
+            if (fk == FK_FAR)
+                for (int d = 0; d < 3; ++d) wa[2 + d] += t * rbf * M * (xi[d] - Ub[d]) / (p.R * Tw);
             ab = 0.0;
             bb = 0.0;
         }
         abf[i] = ab;
         abh[i] = bb;
     });
-    if (fk == FK_FIXED) {
-        x.template sum<2>(wa);
-        if (x.leader()) dTw[f] += wa[1];
+    if (fk == FK_FIXED || fk == FK_FAR) {
+        x.template sum<5>(wa);
+        if (x.leader()) {
+            dTw[f] += wa[1];
+            if (fk == FK_FAR)
+                for (int d = 0; d < 3; ++d) aUf[3 * (size_t)f + d] = wa[2 + d];
+        }
         return;
     }
     if (fk != FK_WALL) return;
@@ -760,7 +774,8 @@ ADJ_HD void recon_adj(X &x, const AP &p, int c, const double *abf, const double 
                     G[q] += bg * p.nhat[3 * f + q];
                     H[q] += bh * p.nhat[3 * f + q];
                 }
-            } else if (p.fkind[f] == FK_WALL || p.fkind[f] == FK_FIXED || p.fkind[f] == FK_SYM) {
+            } else if (p.fkind[f] == FK_WALL || p.fkind[f] == FK_FIXED || p.fkind[f] == FK_SYM ||
+                       p.fkind[f] == FK_FAR) {
                 // outgoing value of this velocity; on a symmetry plane also the incoming value of
                 // its mirror, which is the same reconstruction
                 double wg = s > 0 ? vg : 0.0, wh = s > 0 ? vh : 0.0;
@@ -861,7 +876,7 @@ ADJ_HD void grad_adj(X &x, const AP &p, int c, const double *agg, const double *
 // folded into the distributions: writes the adjoint of g~^n, h~^n.
 template <class X>
 ADJ_HD void s1_adj(X &x, const AP &p, St s, int c, const double *ag1, const double *ah1, const double *apg,
-                   const double *aph, double *ag0, double *ah0)
+                   const double *aph, const double *aUf, double *ag0, double *ah0)
 {
     const int nv = p.nv;
     double rho, U[3], T, q[3];
@@ -883,6 +898,12 @@ ADJ_HD void s1_adj(X &x, const AP &p, St s, int c, const double *ag1, const doub
         mom_acc(acc + 9, xi, p.wv[k], U, s.g[i], s.h[i]);
     });
     x.template sum<19>(acc);
+    // far-field faces of the cell: their incoming Maxwellians took this state's cell velocity
+    for (int e = p.cstart[c]; e < p.cstart[c + 1]; ++e) {
+        const int f = p.cface[e];
+        if (f >= p.nfi && p.fkind[f] == FK_FAR)
+            for (int d = 0; d < 3; ++d) acc[2 + d] += aUf[3 * (size_t)f + d];
+    }
     const double th = acc[0] * (-2.0 * r / (2.0 * tau + p.dt));
     Mom m;
     m.rho = rho;

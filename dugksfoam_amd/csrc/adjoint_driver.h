@@ -85,8 +85,9 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf) return "patch face range outside the boundary";
         code = DUGKS_ERR_UNSUPPORTED;
         if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_FIXED &&
-            pp.type != DUGKS_PATCH_ZEROGRADIENT && pp.type != DUGKS_PATCH_SYMMETRY)
-            return "the adjoint supports wall, cyclic, fixed, zeroGradient and symmetry patches only";
+            pp.type != DUGKS_PATCH_ZEROGRADIENT && pp.type != DUGKS_PATCH_SYMMETRY &&
+            pp.type != DUGKS_PATCH_FARFIELD)
+            return "the adjoint supports wall, cyclic, fixed, farField, zeroGradient and symmetry patches only";
         int axis = 0;
         if (pp.type == DUGKS_PATCH_SYMMETRY && pp.size) {
             // the mirror axis from the patch's first face normal (discreteVelocity.C:766-768)
@@ -100,6 +101,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
         code = DUGKS_ERR_INVALID;
         if (pp.type == DUGKS_PATCH_WALL && (!pp.U || !pp.T)) return "wall patch without U/T";
         if (pp.type == DUGKS_PATCH_FIXED && (!pp.U || !pp.T || !pp.rho)) return "fixed patch without rho, U, T";
+        if (pp.type == DUGKS_PATCH_FARFIELD && (!pp.T || !pp.rho)) return "far-field patch without rho, T";
         if (pp.type == DUGKS_PATCH_CYCLIC &&
             (pp.neighbour_patch < 0 || pp.neighbour_patch >= cs->n_patches ||
              cs->patches[pp.neighbour_patch].size != pp.size ||
@@ -117,6 +119,10 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
                     m.rb[f] = pp.rho[i];
                     if (!(m.rb[f] > 0)) return "boundary density must be > 0";
                 }
+            } else if (pp.type == DUGKS_PATCH_FARFIELD) {
+                m.Tw[f] = pp.T[i];
+                m.rb[f] = pp.rho[i];
+                if (!(m.Tw[f] > 0) || !(m.rb[f] > 0)) return "boundary density and temperature must be > 0";
             } else if (pp.type == DUGKS_PATCH_ZEROGRADIENT) {
                 // nothing beyond the lagged gradient row
             } else if (pp.type == DUGKS_PATCH_SYMMETRY) {
@@ -160,6 +166,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
             m.fkind[f] = bkind[b] == DUGKS_PATCH_WALL    ? FK_WALL
                          : bkind[b] == DUGKS_PATCH_FIXED   ? FK_FIXED
                          : bkind[b] == DUGKS_PATCH_SYMMETRY ? FK_SYM
+                         : bkind[b] == DUGKS_PATCH_FARFIELD ? FK_FAR
                                                             : FK_ZG;
             const double mS = dugks_lsq::mag3(cs->Sf + 3 * f);
             for (int d = 0; d < 3; ++d) m.nhat[3 * (size_t)f + d] = cs->Sf[3 * f + d] / mS;
@@ -228,6 +235,7 @@ struct Engine {
     double *apg = nullptr, *aph = nullptr, *agg = nullptr, *agh = nullptr;
     FaceRec *fr = nullptr;
     double *win = nullptr, *dwin = nullptr, *jacc = nullptr, *dTw = nullptr, *coef = nullptr;
+    double *aUf = nullptr;   // far-field faces: adjoint of the owner cell's velocity [nf][3]
     double *rho0 = nullptr, *U0 = nullptr, *T0 = nullptr, *finmac = nullptr;
     double *states = nullptr;
     size_t states_n = 0;
@@ -287,6 +295,7 @@ struct Engine {
         agg = be.template alloc<double>(ngv); agh = be.template alloc<double>(ngv);
         fr = be.template alloc<FaceRec>(m.nf);
         jacc = be.template alloc<double>(m.nf); dTw = be.template alloc<double>(m.nf);
+        aUf = be.template alloc<double>(3 * (size_t)m.nf);
         coef = be.template alloc<double>(m.nf);
         rho0 = be.template alloc<double>(m.nc); U0 = be.template alloc<double>(3 * (size_t)m.nc);
         T0 = be.template alloc<double>(m.nc); finmac = be.template alloc<double>(8 * (size_t)m.nc);
@@ -330,7 +339,8 @@ struct Engine {
         double *Gng = fwd ? b.Gg : nullptr, *Gnh = fwd ? b.Gh : nullptr;
         be.template launch<k_s1>(q.nc, ADJ_FN { s1_cell(x, q, a, i, gp_, hp_); });
         be.template launch<k_grad>(q.nc, ADJ_FN { grad_cell(x, q, a, i, gp_, hp_, gg_, gh_, Gng, Gnh); });
-        be.template launch<k_face>(q.nf, ADJ_FN { face_fwd(x, q, i, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
+        const double *mac = a.mac;
+        be.template launch<k_face>(q.nf, ADJ_FN { face_fwd(x, q, i, mac, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
         if (fwd) be.template launch<k_cell>(q.nc, ADJ_FN { cell_fwd(x, q, a, i, gp_, hp_, gr, hr, b); });
     }
 
@@ -346,11 +356,13 @@ struct Engine {
         const double *ag1 = ag[cur], *ah1 = ah[cur], *AG1g = AGg[cur], *AG1h = AGh[cur];
         double *ag0 = ag[cur ^ 1], *ah0 = ah[cur ^ 1], *AG0g = AGg[cur ^ 1], *AG0h = AGh[cur ^ 1];
         double *abf_ = abf, *abh_ = abh, *dTw_ = dTw, *apg_ = apg, *aph_ = aph, *agg_ = agg, *agh_ = agh;
-        be.template launch<k_face_adj>(q.nf, ADJ_FN { face_adj(x, q, i, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_); });
+        double *aUf_ = aUf;
+        const double *mac = a.mac;
+        be.template launch<k_face_adj>(q.nf, ADJ_FN { face_adj(x, q, i, mac, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_, aUf_); });
         if (n == 0) return;   // the initial state's adjoint is not needed
         be.template launch<k_recon_adj>(q.nc, ADJ_FN { recon_adj(x, q, i, abf_, abh_, ag1, ah1, AG1g, AG1h, apg_, aph_, agg_, agh_); });
         be.template launch<k_grad_adj>(q.nc, ADJ_FN { grad_adj(x, q, i, agg_, agh_, apg_, aph_, AG0g, AG0h); });
-        be.template launch<k_s1_adj>(q.nc, ADJ_FN { s1_adj(x, q, a, i, ag1, ah1, apg_, aph_, ag0, ah0); });
+        be.template launch<k_s1_adj>(q.nc, ADJ_FN { s1_adj(x, q, a, i, ag1, ah1, apg_, aph_, aUf_, ag0, ah0); });
         cur ^= 1;
     }
 

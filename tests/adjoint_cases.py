@@ -9,11 +9,13 @@ import numpy as np
 from dugksfoam_amd import cases
 
 
-T_PATCHES = (cases.PATCH_WALL, cases.PATCH_FIXED)   # boundary kinds whose temperature J depends on
+# boundary kinds whose temperature J depends on
+T_PATCHES = (cases.PATCH_WALL, cases.PATCH_FIXED, cases.PATCH_FARFIELD)
 
 
 def wall_mask(c) -> np.ndarray:
-    """Boundary faces with a temperature the adjoint differentiates: walls and fixed faces."""
+    """Boundary faces with a temperature the adjoint differentiates: walls, fixed and far-field
+    faces."""
     nfi, nb = c.n_internal_faces, c.n_faces - c.n_internal_faces
     m = np.zeros(nb, bool)
     for p in c.patches:
@@ -23,7 +25,8 @@ def wall_mask(c) -> np.ndarray:
 
 
 def with_wall_T(c, dT):
-    """The case with the temperature of boundary face b (walls, fixed faces) raised by dT[b]."""
+    """The case with the temperature of boundary face b (walls, fixed and far-field faces) raised
+    by dT[b]."""
     c2 = copy.deepcopy(c)
     nfi = c.n_internal_faces
     for p in c2.patches:
@@ -65,6 +68,8 @@ CASES = [
     ("channel_hot", lambda: cases.channel(nx=8, ny=4, Lx=1.0, nv=8, kn=0.2, n_modes=3, amp=0.2), 8, 5, 3),
     # fixed-Maxwellian inlet (dJ/dT_b) and zeroGradient outlet
     ("open_zg", lambda: cases.open_channel(6, 4, nv=8, outlet="zeroGradient"), 6, 2, 2),
+    # far-field outlet: incoming rho_b M(U of the outlet cell, T_b)
+    ("open_far", lambda: cases.open_channel(6, 4, nv=8), 6, 2, 3),
     # symmetry plane: incoming values are the mirrored velocities' outgoing reconstructions
     ("symmetry_nv6", lambda: cases.symmetry_cavity(4, 6, 6), 6, 2, 0),
     ("symmetry_nv7_x0", lambda: cases.symmetry_cavity(3, 6, 7, plane="x0"), 5, 1, 2),  # xi.S == 0: self-mirror
