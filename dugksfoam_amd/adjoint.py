@@ -1,7 +1,7 @@
 """ctypes binding of the discrete adjoint (the dugks_adjoint_* entry points of include/dugks.h).
 
 Adjoint(case).run(steps, first, coef) returns J and dJ/dT for every boundary face (the wall
-temperature of walls, the boundary temperature of fixed and far-field faces; 0 elsewhere), where
+temperature of walls, the boundary temperature of fixed, far-field and pressure faces; 0 elsewhere), where
 
     J = sum over steps n = first .. steps of  sum_f coef[f] (rho U)_f^n . S_f
 
@@ -51,8 +51,7 @@ def _f64(a):
 
 
 class Adjoint:
-    """Discrete adjoint of the DUGKS step for one case (wall, cyclic, fixed, far-field,
-    zeroGradient and symmetry patches) on one GPU."""
+    """Discrete adjoint of the DUGKS step for one case (every patch kind) on one GPU."""
 
     def __init__(self, case, device: int = 0):
         L = _lib()

@@ -44,11 +44,15 @@ constexpr int FK_FIXED = 3;   // frozen incoming Maxwellian rho_b M(U_b, T_b) (r
 constexpr int FK_ZG = 4;      // zeroGradient: every face value is the owner cell's gBarPvol
 constexpr int FK_SYM = 5;     // symmetry plane: incoming value = outgoing value of the mirrored velocity
 constexpr int FK_FAR = 6;     // far field: incoming rho_b M(U of the owner cell, T_b)
+constexpr int FK_PIN = 7;     // pressureIn: a far field with rho_b = p_b / (R T_b) after the first step
+constexpr int FK_POUT = 8;    // pressureOut: a far field whose rho_b, T_b follow the owner cell
 
 // Non-coupled boundary faces: a lagged fixedGradient row, only the outgoing half relaxed.
 ADJ_HD bool open_kind(int fk) { return fk != FK_INTERNAL && fk != FK_CYCLIC; }
 // Boundary faces whose incoming half is a Maxwellian at the boundary temperature (dJ/dT output).
-ADJ_HD bool maxwell_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_FAR; }
+ADJ_HD bool maxwell_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_FAR || fk == FK_PIN || fk == FK_POUT; }
+// Far-field kinds: incoming rho_b M(U of the owner cell, T_b).
+ADJ_HD bool far_kind(int fk) { return fk == FK_FAR || fk == FK_PIN || fk == FK_POUT; }
 
 // Mesh, velocity set and gas; every array is indexed by global face / cell numbers.
 struct AP {
@@ -65,6 +69,7 @@ struct AP {
     const double *Tw, *Uw;       // walls, fixed and far-field faces: T [nf]; walls, fixed: U [nf][3]
     const double *rb;            // fixed and far-field faces: rho_b [nf]
     const double *win, *dwin;    // walls: inComingByRho and its T_w derivative
+    const double *pb;            // pressure faces: p_b [nf]
     const int *sax;              // symmetry faces: the mirror axis [nf]
     const int *mir;              // [3][nv]: the velocity mirrored across axis a
 };
@@ -152,6 +157,33 @@ ADJ_HD double maxwell(const AP &p, const double *U, double T, const double *xi, 
     const double M = 1.0 / pow(sqrt(2.0 * PI * p.R * T), (double)p.D) * exp(-u2 / (2.0 * p.R * T));
     dMdT = M * (-0.5 * p.D / T + u2 / (2.0 * p.R * T * T));
     return M;
+}
+
+// rho_b and T_b of a far-field face in the step that starts from the state with cell macros mac.
+// Pressure patches take them from S10 of the previous step (fvDVM.C:730-806): inlet rho_b =
+// p_b / (R T_b); outlet rho_b = rho_i + (p_b - rho_i R T_i) / a_i^2, T_b = p_b / (R rho_i) with
+// a_i^2 = R T_i (K + 5) / (K + 3) of the owner cell i.  In the first step they keep the patch
+// constructor's rho_b = 1 and the patch temperature.
+ADJ_HD void far_bc(const AP &p, int f, int fk, const double *mac, bool first, double &rb, double &Tb)
+{
+    Tb = p.Tw[f];
+    if (fk == FK_FAR) {
+        rb = p.rb[f];
+        return;
+    }
+    if (first) {
+        rb = 1.0;
+        return;
+    }
+    if (fk == FK_PIN) {
+        rb = p.pb[f] / p.R / Tb;
+        return;
+    }
+    const double *mi = mac + 8 * (size_t)p.owner[f];
+    const double rhoi = mi[0], Ti = mi[4];
+    const double ai = sqrt(p.R * Ti * (p.K + 5) / (p.K + 3));
+    rb = rhoi + (p.pb[f] - rhoi * p.R * Ti) / ai / ai;
+    Tb = p.pb[f] / (p.R * rhoi);
 }
 
 // Moments of one point about its velocity U: A0 = 1/2 sum w (|c|^2 g + h),
@@ -404,7 +436,7 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
         return;
     }
     const double gP = gp[ix(P, nv, k)] + dot3(GP, dO), hP = hp[ix(P, nv, k)] + dot3(HP, dO);
-    if (fk == FK_WALL || fk == FK_FIXED || fk == FK_SYM || fk == FK_FAR) {
+    if (fk == FK_WALL || fk == FK_FIXED || fk == FK_SYM || far_kind(fk)) {
         g = gP;
         h = hP;
         return;
@@ -461,7 +493,7 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
 // macros (S6) and, when gr is given, the relaxed face values (S7).  jacc[f] += c_f (rho U)_f . S_f
 // when jacc is given.
 template <class X>
-ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, const double *gp, const double *hp,
+ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, bool first, const double *gp, const double *hp,
                      const double *gg, const double *gh, double *gf, double *hf, FaceRec *fr, double *gr,
                      double *hr, const double *coef, double *jacc)
 {
@@ -491,11 +523,12 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, const double *
             hf[ix(f, nv, k)] = g * (p.R * Tw) * p.kc;
         });
     }
-    if (fk == FK_FIXED || fk == FK_FAR) {
+    if (fk == FK_FIXED || far_kind(fk)) {
         // incoming rho_b M(U, T_b): fixed faces freeze U_b (discreteVelocity.C:312-344, 556-573),
         // far-field faces take the owner cell's velocity (discreteVelocity.C:588-602)
-        const double Tb = p.Tw[f], rb = p.rb[f];
-        const double *Ub = fk == FK_FAR ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
+        double Tb = p.Tw[f], rb = p.rb[f];
+        if (far_kind(fk)) far_bc(p, f, fk, mac, first, rb, Tb);
+        const double *Ub = far_kind(fk) ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
         x.for_k(nv, [&](int k) {
             const double *xi = p.xi + 3 * k;
             if (sdot(xi, S) > 0) return;
@@ -625,7 +658,7 @@ ADJ_HD void cell_fwd(X &x, const AP &p, St s, int c, const double *gp, const dou
 // Writes the adjoint of the reconstructed face values (zero for incoming wall values) and adds
 // this step's dJ/dT_w of a wall face.  coef null: the step is outside the objective's window.
 template <class X>
-ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, const FaceRec *fr, const double *gf,
+ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, bool first, const FaceRec *fr, const double *gf,
                      const double *hf, const double *ag1, const double *ah1, const double *coef, double *abf,
                      double *abh, double *dTw, double *aUf)
 {
@@ -683,10 +716,13 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, const FaceRec 
     }
     double al, be[3], ga, de[3];
     macro_adj(p, m, acc + 1, th, al, be, ga, de);
-    const double Tw = maxwell_kind(fk) ? p.Tw[f] : 0.0, hfac = p.R * Tw * p.kc;
-    const double rbf = fk == FK_FIXED || fk == FK_FAR ? p.rb[f] : 0.0;
-    const double *Ub = fk == FK_FAR ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
-    double wa[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // walls: M sum, dT; fixed, far field: dT, dU (far)
+    double Tw = maxwell_kind(fk) ? p.Tw[f] : 0.0;
+    double rbf = fk == FK_FIXED || fk == FK_FAR ? p.rb[f] : 0.0;
+    if (far_kind(fk)) far_bc(p, f, fk, mac, first, rbf, Tw);
+    const double hfac = p.R * Tw * p.kc;
+    const double *Ub = far_kind(fk) ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
+    // walls: M sum, dT; fixed and far-field kinds: dM sum (d rho_b), dT_b, dU of the owner (far)
+    double wa[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     x.for_k(nv, [&](int k) {
         const double *xi = p.xi + 3 * k;
         const size_t i = ix(f, nv, k);
@@ -700,14 +736,15 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, const FaceRec 
             wa[1] += t * R.rho_w * dM + bb * R.rho_w * M * p.R * p.kc;
             ab = 0.0;
             bb = 0.0;
-        } else if ((fk == FK_FIXED || fk == FK_FAR) && !(sdot(xi, S) > 0)) {
-            // incoming rho_b M(U, T_b): the T_b derivative, and on far-field faces the derivative
-            // in the owner cell's velocity, dM/dU = M (xi - U) / (R T_b)
+        } else if ((fk == FK_FIXED || far_kind(fk)) && !(sdot(xi, S) > 0)) {
+            // incoming rho_b M(U, T_b): the rho_b and T_b derivatives, and on far-field faces the
+            // derivative in the owner cell's velocity, dM/dU = M (xi - U) / (R T_b)
             double dM;
             const double M = maxwell(p, Ub, Tw, xi, dM);
             const double t = ab + bb * hfac;
+            wa[0] += t * M;
             wa[1] += t * rbf * dM + bb * rbf * M * p.R * p.kc;
-            if (fk == FK_FAR)
+            if (far_kind(fk))
                 for (int d = 0; d < 3; ++d) wa[2 + d] += t * rbf * M * (xi[d] - Ub[d]) / (p.R * Tw);
             ab = 0.0;
             bb = 0.0;
@@ -715,12 +752,26 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, const FaceRec 
         abf[i] = ab;
         abh[i] = bb;
     });
-    if (fk == FK_FIXED || fk == FK_FAR) {
+    if (fk == FK_FIXED || far_kind(fk)) {
         x.template sum<5>(wa);
         if (x.leader()) {
-            dTw[f] += wa[1];
-            if (fk == FK_FAR)
-                for (int d = 0; d < 3; ++d) aUf[3 * (size_t)f + d] = wa[2 + d];
+            // wa[0] = dJ/drho_b, wa[1] = dJ/dT_b at fixed rho_b, wa[2..4] = dJ/dU of the owner
+            double dT = wa[1], drho_i = 0.0, dT_i = 0.0;
+            if (fk == FK_PIN && !first) dT += wa[0] * (-p.pb[f] / (p.R * Tw * Tw));
+            if (fk == FK_POUT && !first) {
+                // S10 of the previous step: rho_b(rho_i, T_i), T_b(rho_i); the patch T is unused
+                const double *mi = mac + 8 * (size_t)p.owner[f];
+                const double rhoi = mi[0], Ti = mi[4], cK = p.R * (p.K + 5) / (p.K + 3);
+                drho_i = wa[0] * (1.0 - p.R / cK) + wa[1] * (-p.pb[f] / (p.R * rhoi * rhoi));
+                dT_i = wa[0] * (-p.pb[f] / (cK * Ti * Ti));
+                dT = 0.0;
+            }
+            dTw[f] += dT;
+            if (far_kind(fk)) {
+                for (int d = 0; d < 3; ++d) aUf[5 * (size_t)f + d] = wa[2 + d];
+                aUf[5 * (size_t)f + 3] = drho_i;
+                aUf[5 * (size_t)f + 4] = dT_i;
+            }
         }
         return;
     }
@@ -774,8 +825,7 @@ ADJ_HD void recon_adj(X &x, const AP &p, int c, const double *abf, const double 
                     G[q] += bg * p.nhat[3 * f + q];
                     H[q] += bh * p.nhat[3 * f + q];
                 }
-            } else if (p.fkind[f] == FK_WALL || p.fkind[f] == FK_FIXED || p.fkind[f] == FK_SYM ||
-                       p.fkind[f] == FK_FAR) {
+            } else if (p.fkind[f] != FK_CYCLIC) {
                 // outgoing value of this velocity; on a symmetry plane also the incoming value of
                 // its mirror, which is the same reconstruction
                 double wg = s > 0 ? vg : 0.0, wh = s > 0 ? vh : 0.0;
@@ -899,10 +949,14 @@ ADJ_HD void s1_adj(X &x, const AP &p, St s, int c, const double *ag1, const doub
     });
     x.template sum<19>(acc);
     // far-field faces of the cell: their incoming Maxwellians took this state's cell velocity
+    // (and, on pressure outlets, rho_b and T_b from its density and temperature)
     for (int e = p.cstart[c]; e < p.cstart[c + 1]; ++e) {
         const int f = p.cface[e];
-        if (f >= p.nfi && p.fkind[f] == FK_FAR)
-            for (int d = 0; d < 3; ++d) acc[2 + d] += aUf[3 * (size_t)f + d];
+        if (f >= p.nfi && far_kind(p.fkind[f])) {
+            acc[1] += aUf[5 * (size_t)f + 3];
+            for (int d = 0; d < 3; ++d) acc[2 + d] += aUf[5 * (size_t)f + d];
+            acc[5] += aUf[5 * (size_t)f + 4];
+        }
     }
     const double th = acc[0] * (-2.0 * r / (2.0 * tau + p.dt));
     Mom m;

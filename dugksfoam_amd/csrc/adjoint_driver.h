@@ -50,7 +50,7 @@ struct Mesh {
     int D = 0, K = 0, nc = 0, nfi = 0, nf = 0, nv = 0;
     double R = 0, omega = 0, Tref = 0, mu = 0, Pr = 0;
     std::vector<int> cstart, cface, owner, nei, fkind, partner, pcell;
-    std::vector<double> S, Cf, C, V, wf, pvec, nvec, nhat, adel, xi, wv, Tw, Uw, rb;
+    std::vector<double> S, Cf, C, V, wf, pvec, nvec, nhat, adel, xi, wv, Tw, Uw, rb, pb;
     std::vector<int> sax, mir;
 };
 
@@ -79,6 +79,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     m.Uw.assign(3 * (size_t)nf, 0.0);
     m.rb.assign(nf, 0.0);
     m.sax.assign(nf, 0);
+    m.pb.assign(nf, 0.0);
     for (int pi = 0; pi < cs->n_patches; ++pi) {
         const dugks_patch &pp = cs->patches[pi];
         code = DUGKS_ERR_INVALID;
@@ -86,8 +87,9 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
         code = DUGKS_ERR_UNSUPPORTED;
         if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_FIXED &&
             pp.type != DUGKS_PATCH_ZEROGRADIENT && pp.type != DUGKS_PATCH_SYMMETRY &&
-            pp.type != DUGKS_PATCH_FARFIELD)
-            return "the adjoint supports wall, cyclic, fixed, farField, zeroGradient and symmetry patches only";
+            pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_PRESSURE_IN &&
+            pp.type != DUGKS_PATCH_PRESSURE_OUT)
+            return "unknown patch type";
         int axis = 0;
         if (pp.type == DUGKS_PATCH_SYMMETRY && pp.size) {
             // the mirror axis from the patch's first face normal (discreteVelocity.C:766-768)
@@ -102,6 +104,9 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
         if (pp.type == DUGKS_PATCH_WALL && (!pp.U || !pp.T)) return "wall patch without U/T";
         if (pp.type == DUGKS_PATCH_FIXED && (!pp.U || !pp.T || !pp.rho)) return "fixed patch without rho, U, T";
         if (pp.type == DUGKS_PATCH_FARFIELD && (!pp.T || !pp.rho)) return "far-field patch without rho, T";
+        if ((pp.type == DUGKS_PATCH_PRESSURE_IN || pp.type == DUGKS_PATCH_PRESSURE_OUT) &&
+            (!pp.T || !(pp.pressure > 0)))
+            return "pressure patch without T or a positive pressure";
         if (pp.type == DUGKS_PATCH_CYCLIC &&
             (pp.neighbour_patch < 0 || pp.neighbour_patch >= cs->n_patches ||
              cs->patches[pp.neighbour_patch].size != pp.size ||
@@ -119,6 +124,10 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
                     m.rb[f] = pp.rho[i];
                     if (!(m.rb[f] > 0)) return "boundary density must be > 0";
                 }
+            } else if (pp.type == DUGKS_PATCH_PRESSURE_IN || pp.type == DUGKS_PATCH_PRESSURE_OUT) {
+                m.Tw[f] = pp.T[i];
+                m.pb[f] = pp.pressure;
+                if (!(m.Tw[f] > 0)) return "boundary temperature must be > 0";
             } else if (pp.type == DUGKS_PATCH_FARFIELD) {
                 m.Tw[f] = pp.T[i];
                 m.rb[f] = pp.rho[i];
@@ -167,6 +176,8 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
                          : bkind[b] == DUGKS_PATCH_FIXED   ? FK_FIXED
                          : bkind[b] == DUGKS_PATCH_SYMMETRY ? FK_SYM
                          : bkind[b] == DUGKS_PATCH_FARFIELD ? FK_FAR
+                         : bkind[b] == DUGKS_PATCH_PRESSURE_IN ? FK_PIN
+                         : bkind[b] == DUGKS_PATCH_PRESSURE_OUT ? FK_POUT
                                                             : FK_ZG;
             const double mS = dugks_lsq::mag3(cs->Sf + 3 * f);
             for (int d = 0; d < 3; ++d) m.nhat[3 * (size_t)f + d] = cs->Sf[3 * f + d] / mS;
@@ -235,7 +246,7 @@ struct Engine {
     double *apg = nullptr, *aph = nullptr, *agg = nullptr, *agh = nullptr;
     FaceRec *fr = nullptr;
     double *win = nullptr, *dwin = nullptr, *jacc = nullptr, *dTw = nullptr, *coef = nullptr;
-    double *aUf = nullptr;   // far-field faces: adjoint of the owner cell's velocity [nf][3]
+    double *aUf = nullptr;   // far-field faces: adjoint of the owner cell's U, rho, T [nf][5]
     double *rho0 = nullptr, *U0 = nullptr, *T0 = nullptr, *finmac = nullptr;
     double *states = nullptr;
     size_t states_n = 0;
@@ -277,7 +288,7 @@ struct Engine {
         p.S = up(m.S); p.Cf = up(m.Cf); p.C = up(m.C); p.V = up(m.V); p.wf = up(m.wf);
         p.pvec = up(m.pvec); p.nvec = up(m.nvec); p.nhat = up(m.nhat); p.adel = up(m.adel);
         p.xi = up(m.xi); p.wv = up(m.wv); p.Tw = up(m.Tw); p.Uw = up(m.Uw); p.rb = up(m.rb);
-        p.sax = up(m.sax); p.mir = up(m.mir);
+        p.sax = up(m.sax); p.mir = up(m.mir); p.pb = up(m.pb);
         win = be.template alloc<double>(m.nf);
         dwin = be.template alloc<double>(m.nf);
         p.win = win;
@@ -295,7 +306,7 @@ struct Engine {
         agg = be.template alloc<double>(ngv); agh = be.template alloc<double>(ngv);
         fr = be.template alloc<FaceRec>(m.nf);
         jacc = be.template alloc<double>(m.nf); dTw = be.template alloc<double>(m.nf);
-        aUf = be.template alloc<double>(3 * (size_t)m.nf);
+        aUf = be.template alloc<double>(5 * (size_t)m.nf);
         coef = be.template alloc<double>(m.nf);
         rho0 = be.template alloc<double>(m.nc); U0 = be.template alloc<double>(3 * (size_t)m.nc);
         T0 = be.template alloc<double>(m.nc); finmac = be.template alloc<double>(8 * (size_t)m.nc);
@@ -327,7 +338,7 @@ struct Engine {
 
     // one forward step's phases from state a; with fwd, the new state into b.  The face values
     // and records go to the work arrays, or to face-tape slot `slot` when it is >= 0.
-    void recompute(St a, St b, bool fwd, bool j, int slot = -1)
+    void recompute(St a, St b, int step, bool fwd, bool j, int slot = -1)
     {
         const AP q = p;
         double *gp_ = gp, *hp_ = hp, *gg_ = gg, *gh_ = gh;
@@ -340,7 +351,8 @@ struct Engine {
         be.template launch<k_s1>(q.nc, ADJ_FN { s1_cell(x, q, a, i, gp_, hp_); });
         be.template launch<k_grad>(q.nc, ADJ_FN { grad_cell(x, q, a, i, gp_, hp_, gg_, gh_, Gng, Gnh); });
         const double *mac = a.mac;
-        be.template launch<k_face>(q.nf, ADJ_FN { face_fwd(x, q, i, mac, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
+        const bool first = step == 0;
+        be.template launch<k_face>(q.nf, ADJ_FN { face_fwd(x, q, i, mac, first, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
         if (fwd) be.template launch<k_cell>(q.nc, ADJ_FN { cell_fwd(x, q, a, i, gp_, hp_, gr, hr, b); });
     }
 
@@ -348,7 +360,7 @@ struct Engine {
     // step's face values and records come from face-tape slot `slot` (>= 0) or are recomputed.
     void reverse(St a, int n, bool seed, int &cur, int slot = -1)
     {
-        if (slot < 0) recompute(a, a, false, false);
+        if (slot < 0) recompute(a, a, n, false, false);
         const AP q = p;
         const FaceRec *fr_ = slot >= 0 ? fslot_r(slot) : fr;
         const double *gf_ = slot >= 0 ? fslot_g(slot) : gf, *hf_ = slot >= 0 ? fslot_h(slot) : hf;
@@ -358,7 +370,8 @@ struct Engine {
         double *abf_ = abf, *abh_ = abh, *dTw_ = dTw, *apg_ = apg, *aph_ = aph, *agg_ = agg, *agh_ = agh;
         double *aUf_ = aUf;
         const double *mac = a.mac;
-        be.template launch<k_face_adj>(q.nf, ADJ_FN { face_adj(x, q, i, mac, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_, aUf_); });
+        const bool first = n == 0;
+        be.template launch<k_face_adj>(q.nf, ADJ_FN { face_adj(x, q, i, mac, first, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_, aUf_); });
         if (n == 0) return;   // the initial state's adjoint is not needed
         be.template launch<k_recon_adj>(q.nc, ADJ_FN { recon_adj(x, q, i, abf_, abh_, ag1, ah1, AG1g, AG1h, apg_, aph_, agg_, agh_); });
         be.template launch<k_grad_adj>(q.nc, ADJ_FN { grad_adj(x, q, i, agg_, agh_, apg_, aph_, AG0g, AG0h); });
@@ -431,7 +444,7 @@ struct Engine {
         be.tic(0);
         for (int n = 0; n < nsteps; ++n) {
             if (n % C == 0) copy_state(state(n / C), cur);
-            recompute(cur, nxt, true, n + 1 >= first);
+            recompute(cur, nxt, n, true, n + 1 >= first);
             std::swap(cur, nxt);
         }
         be.tic(1);
@@ -450,7 +463,7 @@ struct Engine {
             const int n0 = seg * C, n1 = std::min(nsteps, n0 + C);
             auto tape = [&](int n) { return n == n0 ? state(seg) : state(ncp + (n - n0 - 1)); };
             auto slot = [&](int n) { return n + 1 < n1 && n - n0 < ftape_slots ? n - n0 : -1; };
-            for (int n = n0; n + 1 < n1; ++n) recompute(tape(n), tape(n + 1), true, false, slot(n));
+            for (int n = n0; n + 1 < n1; ++n) recompute(tape(n), tape(n + 1), n, true, false, slot(n));
             for (int n = n1 - 1; n >= n0; --n) reverse(tape(n), n, n + 1 >= first, ca, slot(n));
         }
         be.tic(3);

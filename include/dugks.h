@@ -227,13 +227,13 @@ int dugks_abi_version(void);
  * (rho U)_f^n = sum_k w_k xi_k g_f,k the face momentum of step n (the S6 face moments of
  * fvDVM.C:473-483, i.e. dugks_get_face_macro's rho * U after that step), c_f = face_coef[f].
  * dJ_dTw[b] = dJ/dT of boundary face n_internal_faces + b: the wall temperature of a wall face,
- * the boundary temperature of a fixed or far-field face, 0 on cyclic, zeroGradient and symmetry
- * faces.
+ * the boundary temperature of a fixed, far-field or pressureIn face, the patch temperature of a
+ * pressureOut face (it enters the first step only), 0 on cyclic, zeroGradient and symmetry faces.
  * The run starts from the Shakhov equilibrium of rho, U, T with q = 0 (as
  * dugks_init_equilibrium) and advances n_steps FP64 steps of size dt, keeping a checkpoint every
  * checkpoint_every steps (0: ceil(sqrt(n_steps))); the reverse sweep re-runs each segment.
- * Cases: one case; wall, cyclic, fixed, far-field, zeroGradient and symmetry patches; FP64 (the
- * case's storage field is ignored).  Pressure patches give DUGKS_ERR_UNSUPPORTED.  Device memory: about
+ * Cases: one case (n_cases > 1 gives DUGKS_ERR_UNSUPPORTED), every patch kind, FP64 (the case's
+ * storage field is ignored).  Device memory: about
  * (ceil(n_steps / C) + C + 1) x (2 n_cells + 2 n_boundary_faces) x nv x 8 bytes for the trajectory
  * plus about (8 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;
  * DUGKS_ERR_MEMORY when that does not fit.  Where it also fits, a face tape of

@@ -10,12 +10,13 @@ from dugksfoam_amd import cases
 
 
 # boundary kinds whose temperature J depends on
-T_PATCHES = (cases.PATCH_WALL, cases.PATCH_FIXED, cases.PATCH_FARFIELD)
+T_PATCHES = (cases.PATCH_WALL, cases.PATCH_FIXED, cases.PATCH_FARFIELD, cases.PATCH_PRESSURE_IN,
+             cases.PATCH_PRESSURE_OUT)
 
 
 def wall_mask(c) -> np.ndarray:
-    """Boundary faces with a temperature the adjoint differentiates: walls, fixed and far-field
-    faces."""
+    """Boundary faces with a temperature the adjoint differentiates: walls, fixed, far-field and
+    pressure faces."""
     nfi, nb = c.n_internal_faces, c.n_faces - c.n_internal_faces
     m = np.zeros(nb, bool)
     for p in c.patches:
@@ -25,8 +26,8 @@ def wall_mask(c) -> np.ndarray:
 
 
 def with_wall_T(c, dT):
-    """The case with the temperature of boundary face b (walls, fixed and far-field faces) raised
-    by dT[b]."""
+    """The case with the temperature of boundary face b (walls, fixed, far-field and pressure
+    faces) raised by dT[b]."""
     c2 = copy.deepcopy(c)
     nfi = c.n_internal_faces
     for p in c2.patches:
@@ -70,6 +71,8 @@ CASES = [
     ("open_zg", lambda: cases.open_channel(6, 4, nv=8, outlet="zeroGradient"), 6, 2, 2),
     # far-field outlet: incoming rho_b M(U of the outlet cell, T_b)
     ("open_far", lambda: cases.open_channel(6, 4, nv=8), 6, 2, 3),
+    # pressureIn / pressureOut: rho_b (and the outlet's T_b) from S10 of the previous step
+    ("pressure", lambda: cases.pressure_channel(6, 4, nv=8), 6, 1, 2),
     # symmetry plane: incoming values are the mirrored velocities' outgoing reconstructions
     ("symmetry_nv6", lambda: cases.symmetry_cavity(4, 6, 6), 6, 2, 0),
     ("symmetry_nv7_x0", lambda: cases.symmetry_cavity(3, 6, 7, plane="x0"), 5, 1, 2),  # xi.S == 0: self-mirror

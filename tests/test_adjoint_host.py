@@ -73,6 +73,6 @@ def test_planted_mistake_is_caught():
 
 
 def test_unsupported_patches_rejected():
-    c = cases.pressure_channel(4, 3, nv=6)
-    with pytest.raises(RuntimeError, match="wall, cyclic"):
+    c = cases.union([cases.cavity(3, 5), cases.cavity(3, 5)])
+    with pytest.raises(RuntimeError, match="one case"):
         HostAdjoint(c)
