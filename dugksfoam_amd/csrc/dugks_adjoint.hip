@@ -22,7 +22,10 @@
 
 namespace {
 
-constexpr int NT = 256;              // lanes per workgroup (4 wavefronts)
+#ifndef DUGKS_ADJ_NT
+#define DUGKS_ADJ_NT 128
+#endif
+constexpr int NT = DUGKS_ADJ_NT;     // lanes per workgroup (a multiple of the 64-lane wavefront)
 constexpr int SUM_MAX = 32;          // widest workgroup reduction of adjoint_core.h
 
 // Executor of adjoint_core.h inside a workgroup.
@@ -41,7 +44,7 @@ struct GX {
     }
 
     // Deterministic workgroup sum of v[N], result in every lane: butterfly within each wavefront,
-    // then the four wavefront totals added in order.
+    // then the wavefront totals added in order.
     template <int N>
     __host__ __device__ void sum(double *v)
     {
