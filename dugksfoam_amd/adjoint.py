@@ -51,7 +51,7 @@ def _f64(a):
 
 
 class Adjoint:
-    """Discrete adjoint of the DUGKS step for one case (every patch kind) on one GPU."""
+    """Discrete adjoint of the DUGKS step for a case or a batch (every patch kind) on one GPU."""
 
     def __init__(self, case, device: int = 0):
         L = _lib()

@@ -70,6 +70,7 @@ struct AP {
     const double *rb;            // fixed and far-field faces: rho_b [nf]
     const double *win, *dwin;    // walls: inComingByRho and its T_w derivative
     const double *pb;            // pressure faces: p_b [nf]
+    const double *muc;           // muRef of each cell's case [nc] (faces: the owner's)
     const int *sax;              // symmetry faces: the mirror axis [nf]
     const int *mir;              // [3][nv]: the velocity mirrored across axis a
 };
@@ -92,9 +93,9 @@ ADJ_HD double sdot(const double *a, const double *b)
     return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
 
-ADJ_HD double tau_of(const AP &p, double rho, double T)
+ADJ_HD double tau_of(const AP &p, double mu, double rho, double T)
 {
-    return p.mu * exp(p.omega * log(T / p.Tref)) / rho / T / p.R;   // fvDVM.C:808-817
+    return mu * exp(p.omega * log(T / p.Tref)) / rho / T / p.R;   // fvDVM.C:808-817
 }
 
 struct Eq {
@@ -340,7 +341,7 @@ ADJ_HD void s1_cell(X &x, const AP &p, St s, int c, double *gp, double *hp)
 {
     double rho, U[3], T, q[3];
     cell_macros(p, s.mac + 8 * (size_t)c, rho, U, T, q);
-    const double tau = tau_of(p, rho, T);
+    const double tau = tau_of(p, p.muc[c], rho, T);
     const double r = 1.5 * p.dt / (2.0 * tau + p.dt);
     Eq e;
     eq_set(p, rho, U, T, q, e);
@@ -551,7 +552,7 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, bool first, co
     m.rho = a[0];
     for (int d = 0; d < 3; ++d) m.U[d] = a[1 + d] / m.rho;
     m.T = (a[4] - 0.5 * m.rho * dot3(m.U, m.U)) / ((p.K + 3) / 2.0 * p.R * m.rho);
-    m.tau = tau_of(p, m.rho, m.T);
+    m.tau = tau_of(p, p.muc[p.owner[f]], m.rho, m.T);
     double sm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     x.for_k(nv, [&](int k) { mom_acc(sm, p.xi + 3 * k, p.wv[k], m.U, gf[ix(f, nv, k)], hf[ix(f, nv, k)]); });
     x.template sum<10>(sm);
@@ -634,7 +635,7 @@ ADJ_HD void cell_fwd(X &x, const AP &p, St s, int c, const double *gp, const dou
     double U[3];
     for (int d = 0; d < 3; ++d) U[d] = a[1 + d] / rho;
     const double T = (a[4] - 0.5 * rho * dot3(U, U)) / ((p.K + 3) / 2.0 * p.R * rho);
-    const double tau = tau_of(p, rho, T);
+    const double tau = tau_of(p, p.muc[c], rho, T);
     double sm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     x.for_k(nv, [&](int k) { mom_acc(sm, p.xi + 3 * k, p.wv[k], U, nx.g[ix(c, nv, k)], nx.h[ix(c, nv, k)]); });
     x.template sum<10>(sm);
@@ -931,7 +932,7 @@ ADJ_HD void s1_adj(X &x, const AP &p, St s, int c, const double *ag1, const doub
     const int nv = p.nv;
     double rho, U[3], T, q[3];
     cell_macros(p, s.mac + 8 * (size_t)c, rho, U, T, q);
-    const double tau = tau_of(p, rho, T);
+    const double tau = tau_of(p, p.muc[c], rho, T);
     const double r = 1.5 * p.dt / (2.0 * tau + p.dt);
     Eq e;
     eq_set(p, rho, U, T, q, e);

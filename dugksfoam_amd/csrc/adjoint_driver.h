@@ -50,7 +50,7 @@ struct Mesh {
     int D = 0, K = 0, nc = 0, nfi = 0, nf = 0, nv = 0;
     double R = 0, omega = 0, Tref = 0, mu = 0, Pr = 0;
     std::vector<int> cstart, cface, owner, nei, fkind, partner, pcell;
-    std::vector<double> S, Cf, C, V, wf, pvec, nvec, nhat, adel, xi, wv, Tw, Uw, rb, pb;
+    std::vector<double> S, Cf, C, V, wf, pvec, nvec, nhat, adel, xi, wv, Tw, Uw, rb, pb, muc;
     std::vector<int> sax, mir;
 };
 
@@ -67,9 +67,19 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
         return "missing mesh array";
     if (cs->n_xi <= 0 || cs->n_xi > 255 || !cs->xi || !cs->xi_weights) return "bad velocity set";
     if (!(cs->R > 0) || !(cs->T_ref > 0) || !(cs->mu_ref > 0) || cs->K < 0) return "bad gas properties";
-    code = DUGKS_ERR_UNSUPPORTED;
-    if (cs->n_cases > 1) return "the adjoint takes one case";
     const int nc = cs->n_cells, nfi = cs->n_internal_faces, nf = cs->n_faces, nb = nf - nfi;
+    // batches: one muRef per case (the time step is shared, as in dugks_step)
+    m.muc.assign(nc, cs->mu_ref);
+    if (cs->n_cases > 1) {
+        if (!cs->cell_case) return "n_cases > 1 without cell_case";
+        for (int c = 0; c < nc; ++c) {
+            const int k = cs->cell_case[c];
+            if (k < 0 || k >= cs->n_cases) return "cell_case out of range";
+            if (cs->mu_ref_case) m.muc[c] = cs->mu_ref_case[k];
+            if (!(m.muc[c] > 0)) return "muRef must be > 0";
+        }
+    }
+    code = DUGKS_ERR_UNSUPPORTED;
     m.D = cs->dim; m.K = cs->K; m.nc = nc; m.nfi = nfi; m.nf = nf;
     m.R = cs->R; m.omega = cs->omega; m.Tref = cs->T_ref; m.mu = cs->mu_ref; m.Pr = cs->Pr;
     const int n = cs->n_xi;
@@ -288,7 +298,7 @@ struct Engine {
         p.S = up(m.S); p.Cf = up(m.Cf); p.C = up(m.C); p.V = up(m.V); p.wf = up(m.wf);
         p.pvec = up(m.pvec); p.nvec = up(m.nvec); p.nhat = up(m.nhat); p.adel = up(m.adel);
         p.xi = up(m.xi); p.wv = up(m.wv); p.Tw = up(m.Tw); p.Uw = up(m.Uw); p.rb = up(m.rb);
-        p.sax = up(m.sax); p.mir = up(m.mir); p.pb = up(m.pb);
+        p.sax = up(m.sax); p.mir = up(m.mir); p.pb = up(m.pb); p.muc = up(m.muc);
         win = be.template alloc<double>(m.nf);
         dwin = be.template alloc<double>(m.nf);
         p.win = win;

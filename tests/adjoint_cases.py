@@ -73,6 +73,8 @@ CASES = [
     ("open_far", lambda: cases.open_channel(6, 4, nv=8), 6, 2, 3),
     # pressureIn / pressureOut: rho_b (and the outlet's T_b) from S10 of the previous step
     ("pressure", lambda: cases.pressure_channel(6, 4, nv=8), 6, 1, 2),
+    # a batch of two cavities with one time step and their own muRef
+    ("batch", lambda: cases.union([cases.cavity(4, 6, kn=0.5), cases.cavity(4, 6, kn=0.1)]), 5, 2, 2),
     # symmetry plane: incoming values are the mirrored velocities' outgoing reconstructions
     ("symmetry_nv6", lambda: cases.symmetry_cavity(4, 6, 6), 6, 2, 0),
     ("symmetry_nv7_x0", lambda: cases.symmetry_cavity(3, 6, 7, plane="x0"), 5, 1, 2),  # xi.S == 0: self-mirror

@@ -72,7 +72,8 @@ def test_planted_mistake_is_caught():
     assert abs(float(g @ v) - fd_wrong) > 1e-3 * abs(fd_wrong)
 
 
-def test_unsupported_patches_rejected():
-    c = cases.union([cases.cavity(3, 5), cases.cavity(3, 5)])
-    with pytest.raises(RuntimeError, match="one case"):
+def test_unknown_patch_type_rejected():
+    c = cases.cavity(3, 5)
+    c.patches[0]["type"] = 99
+    with pytest.raises(RuntimeError, match="unknown patch type"):
         HostAdjoint(c)

@@ -79,8 +79,10 @@ def test_checkpoint_spacing_and_reuse_are_exact():
 
 def test_unsupported_and_invalid_inputs():
     with pytest.raises(DugksError) as e:
-        Adjoint(cases.union([cases.cavity(3, 5), cases.cavity(3, 5)]))
-    assert e.value.code == -6 and "one case" in str(e.value)
+        c = cases.cavity(3, 5)
+        c.patches[0]["type"] = 99
+        Adjoint(c)
+    assert e.value.code == -6 and "unknown patch type" in str(e.value)
     c = cases.cavity(3, 5)
     with Adjoint(c) as A:
         with pytest.raises(DugksError) as e:
