@@ -878,13 +878,13 @@ __global__ __launch_bounds__(MAXNT) void k_face_open(P<TS> p)
             const double hm = H ? ld(hrowC, km) : 0.0;
             double GM[3] = {0, 0, 0}, HM[3] = {0, 0, 0};
             grad_at<D, H, NS>(p, c, km, gm, hm, GM, HM);
-            fg[k] = (TS)(gm + dotg<D>(GM, dP, hxm));
-            if (H) fh[k] = (TS)(hm + dotg<D>(HM, dP, hxm));
+            st_stream(fg, k, gm + dotg<D>(GM, dP, hxm));
+            if (H) st_stream(fh, k, hm + dotg<D>(HM, dP, hxm));
         } else {
             const double cx = Ui[0] - v.x, cy = Ui[1] - v.y, cz = Ui[2] - v.z;
             const double gin = W.rho_b * (W.pre * exp(-(cx * cx + cy * cy + cz * cz) / (2.0 * RT)));
-            fg[k] = (TS)(gin - row_off<D>(p, v, 0, vt));
-            if (H) fh[k] = (TS)(gin * RT * p.kc - row_off<D>(p, v, 1, vt));
+            st_stream(fg, k, gin - row_off<D>(p, v, 0, vt));
+            if (H) st_stream(fh, k, gin * RT * p.kc - row_off<D>(p, v, 1, vt));
         }
     }
 }
@@ -1517,8 +1517,8 @@ __global__ __launch_bounds__(MAXNT) void k_cell_s1(P<TS> p)
                 double geq, heq;
                 eq_eval<D, H>(en, tabN, p.nxi, vv[u], p.kc, p.K, geq, heq);
                 const double og = row_off<D>(p, vv[u], 0, vt), oh = H ? row_off<D>(p, vv[u], 1, vt) : 0.0;
-                go[k] = (TS)s1_value(gv[u], og, rn, geq);
-                if (H) ho[k] = (TS)s1_value(hv[u], oh, rn, heq);
+                st_stream(go, k, s1_value(gv[u], og, rn, geq));
+                if (H) st_stream(ho, k, s1_value(hv[u], oh, rn, heq));
             }
         }
     }
