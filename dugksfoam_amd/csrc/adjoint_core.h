@@ -44,7 +44,7 @@ constexpr int FK_FIXED = 3;   // frozen incoming Maxwellian rho_b M(U_b, T_b) (r
 constexpr int FK_ZG = 4;      // zeroGradient: every face value is the owner cell's gBarPvol
 constexpr int FK_SYM = 5;     // symmetry plane: incoming value = outgoing value of the mirrored velocity
 constexpr int FK_FAR = 6;     // far field: incoming rho_b M(U of the owner cell, T_b)
-constexpr int FK_PIN = 7;     // pressureIn: a far field with rho_b = p_b / (R T_b) after the first step
+constexpr int FK_PIN = 7;     // pressureIn: a far field with rho_b = p_b / (R T_b)
 constexpr int FK_POUT = 8;    // pressureOut: a far field whose rho_b, T_b follow the owner cell
 
 // Non-coupled boundary faces: a lagged fixedGradient row, only the outgoing half relaxed.
@@ -163,17 +163,13 @@ ADJ_HD double maxwell(const AP &p, const double *U, double T, const double *xi, 
 // rho_b and T_b of a far-field face in the step that starts from the state with cell macros mac.
 // Pressure patches take them from S10 of the previous step (fvDVM.C:730-806): inlet rho_b =
 // p_b / (R T_b); outlet rho_b = rho_i + (p_b - rho_i R T_i) / a_i^2, T_b = p_b / (R rho_i) with
-// a_i^2 = R T_i (K + 5) / (K + 3) of the owner cell i.  In the first step they keep the patch
-// constructor's rho_b = 1 and the patch temperature.
-ADJ_HD void far_bc(const AP &p, int f, int fk, const double *mac, bool first, double &rb, double &Tb)
+// a_i^2 = R T_i (K + 5) / (K + 3) of the owner cell i.  The first step takes them from the initial
+// state, as the constructor runs S10 once before the first step (fvDVM.C:1071-1072).
+ADJ_HD void far_bc(const AP &p, int f, int fk, const double *mac, double &rb, double &Tb)
 {
     Tb = p.Tw[f];
     if (fk == FK_FAR) {
         rb = p.rb[f];
-        return;
-    }
-    if (first) {
-        rb = 1.0;
         return;
     }
     if (fk == FK_PIN) {
@@ -494,7 +490,7 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
 // macros (S6) and, when gr is given, the relaxed face values (S7).  jacc[f] += c_f (rho U)_f . S_f
 // when jacc is given.
 template <class X>
-ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, bool first, const double *gp, const double *hp,
+ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, const double *gp, const double *hp,
                      const double *gg, const double *gh, double *gf, double *hf, FaceRec *fr, double *gr,
                      double *hr, const double *coef, double *jacc)
 {
@@ -528,7 +524,7 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, bool first, co
         // incoming rho_b M(U, T_b): fixed faces freeze U_b (discreteVelocity.C:312-344, 556-573),
         // far-field faces take the owner cell's velocity (discreteVelocity.C:588-602)
         double Tb = p.Tw[f], rb = p.rb[f];
-        if (far_kind(fk)) far_bc(p, f, fk, mac, first, rb, Tb);
+        if (far_kind(fk)) far_bc(p, f, fk, mac, rb, Tb);
         const double *Ub = far_kind(fk) ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
         x.for_k(nv, [&](int k) {
             const double *xi = p.xi + 3 * k;
@@ -659,7 +655,7 @@ ADJ_HD void cell_fwd(X &x, const AP &p, St s, int c, const double *gp, const dou
 // Writes the adjoint of the reconstructed face values (zero for incoming wall values) and adds
 // this step's dJ/dT_w of a wall face.  coef null: the step is outside the objective's window.
 template <class X>
-ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, bool first, const FaceRec *fr, const double *gf,
+ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, const FaceRec *fr, const double *gf,
                      const double *hf, const double *ag1, const double *ah1, const double *coef, double *abf,
                      double *abh, double *dTw, double *aUf)
 {
@@ -719,7 +715,7 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, bool first, co
     macro_adj(p, m, acc + 1, th, al, be, ga, de);
     double Tw = maxwell_kind(fk) ? p.Tw[f] : 0.0;
     double rbf = fk == FK_FIXED || fk == FK_FAR ? p.rb[f] : 0.0;
-    if (far_kind(fk)) far_bc(p, f, fk, mac, first, rbf, Tw);
+    if (far_kind(fk)) far_bc(p, f, fk, mac, rbf, Tw);
     const double hfac = p.R * Tw * p.kc;
     const double *Ub = far_kind(fk) ? mac + 8 * (size_t)p.owner[f] + 1 : p.Uw + 3 * f;
     // walls: M sum, dT; fixed and far-field kinds: dM sum (d rho_b), dT_b, dU of the owner (far)
@@ -758,8 +754,8 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, bool first, co
         if (x.leader()) {
             // wa[0] = dJ/drho_b, wa[1] = dJ/dT_b at fixed rho_b, wa[2..4] = dJ/dU of the owner
             double dT = wa[1], drho_i = 0.0, dT_i = 0.0;
-            if (fk == FK_PIN && !first) dT += wa[0] * (-p.pb[f] / (p.R * Tw * Tw));
-            if (fk == FK_POUT && !first) {
+            if (fk == FK_PIN) dT += wa[0] * (-p.pb[f] / (p.R * Tw * Tw));
+            if (fk == FK_POUT) {
                 // S10 of the previous step: rho_b(rho_i, T_i), T_b(rho_i); the patch T is unused
                 const double *mi = mac + 8 * (size_t)p.owner[f];
                 const double rhoi = mi[0], Ti = mi[4], cK = p.R * (p.K + 5) / (p.K + 3);

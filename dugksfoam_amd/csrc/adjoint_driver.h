@@ -361,8 +361,7 @@ struct Engine {
         be.template launch<k_s1>(q.nc, ADJ_FN { s1_cell(x, q, a, i, gp_, hp_); });
         be.template launch<k_grad>(q.nc, ADJ_FN { grad_cell(x, q, a, i, gp_, hp_, gg_, gh_, Gng, Gnh); });
         const double *mac = a.mac;
-        const bool first = step == 0;
-        be.template launch<k_face>(q.nf, ADJ_FN { face_fwd(x, q, i, mac, first, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
+        be.template launch<k_face>(q.nf, ADJ_FN { face_fwd(x, q, i, mac, gp_, hp_, gg_, gh_, gf_, hf_, fr_, gr, hr, cf, ja); });
         if (fwd) be.template launch<k_cell>(q.nc, ADJ_FN { cell_fwd(x, q, a, i, gp_, hp_, gr, hr, b); });
     }
 
@@ -380,8 +379,7 @@ struct Engine {
         double *abf_ = abf, *abh_ = abh, *dTw_ = dTw, *apg_ = apg, *aph_ = aph, *agg_ = agg, *agh_ = agh;
         double *aUf_ = aUf;
         const double *mac = a.mac;
-        const bool first = n == 0;
-        be.template launch<k_face_adj>(q.nf, ADJ_FN { face_adj(x, q, i, mac, first, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_, aUf_); });
+        be.template launch<k_face_adj>(q.nf, ADJ_FN { face_adj(x, q, i, mac, fr_, gf_, hf_, ag1, ah1, cf, abf_, abh_, dTw_, aUf_); });
         if (n == 0) return;   // the initial state's adjoint is not needed
         be.template launch<k_recon_adj>(q.nc, ADJ_FN { recon_adj(x, q, i, abf_, abh_, ag1, ah1, AG1g, AG1h, apg_, aph_, agg_, agh_); });
         be.template launch<k_grad_adj>(q.nc, ADJ_FN { grad_adj(x, q, i, agg_, agh_, apg_, aph_, AG0g, AG0h); });

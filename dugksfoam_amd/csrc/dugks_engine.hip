@@ -1442,7 +1442,8 @@ __global__ __launch_bounds__(256) void k_cell_fin(P<TS> p)
 }
 
 // ---------------------------------------------------------------------------------------------
-// k_pressure_bc: S10, updatePressureInOutBC (fvDVM.C:730-806), one thread per open face.  Pressure
+// k_pressure_bc: S10, updatePressureInOutBC (fvDVM.C:730-806), one thread per open face, after
+// every step and once at initialisation (the constructor's call, fvDVM.C:1071-1072).  Pressure
 // inlet: rho_b = p / (R T_b).  Pressure outlet, from the boundary cell's new state (rho_i, T_i),
 // a_i = sqrt(R T_i (K + 5) / (K + 3)): rho_b = rho_i + (p - rho_i R T_i) / a_i^2, T_b = p / (R rho_i).
 // The next step's incoming Maxwellian uses them (k_face_open).
@@ -1966,6 +1967,10 @@ struct InitOp {
         if (h->nw > 0)
             hipLaunchKernelGGL((k_wall_init<D, TS>), dim3(h->nw), dim3(256), 0, h->stream, p,
                                (const int *)h->wfslot);
+        // set initial rho in pressureIn/Out BC: the constructor runs S10 once on the initial
+        // state before the first step (fvDVM.C:1071-1072)
+        if (h->npress)
+            hipLaunchKernelGGL((k_pressure_bc<D, TS>), dim3((h->nopen + 255) / 256), dim3(256), 0, h->stream, p);
     }
 };
 
@@ -2223,7 +2228,8 @@ int build(dugks_engine *h, const dugks_case *cs)
                 if (pp.type == DUGKS_PATCH_FIXED)
                     for (int d = 0; d < 3; ++d) Ub[3 * b + d] = pp.U[3 * i + d];
                 Tb[b] = pp.T[i];
-                // pressureIn / pressureOut rho patch values start at one (pIn.C:75)
+                // pressureIn / pressureOut rho patch values start at one (pIn.C:75); S10 in
+                // dugks_init_equilibrium replaces them before the first step
                 rhob[b] = press ? 1.0 : pp.rho[i];
                 pressb[b] = press ? pp.pressure : 0.0;
                 if (!(Tb[b] > 0) || !(rhob[b] > 0)) return fail(h, DUGKS_ERR_INVALID, "boundary rho and T must be > 0");

@@ -59,11 +59,12 @@ enum dugks_patch_type {
                                   boundary Maxwellian rho_b M(U_b, T_b)
                                   (discreteVelocity.C:312-344, 556-573, 1046-1060)           */
     DUGKS_PATCH_PRESSURE_IN = 6,  /* rho pressureIn: a far field whose rho_b = p / (R T_b) is
-                                  reset after every step (fvDVM.C:743-770); rho_b starts at 1
-                                  as the patch field's constructor sets it                    */
+                                  set by dugks_init_equilibrium and after every step
+                                  (fvDVM.C:743-770, the constructor's call 1071-1072)         */
     DUGKS_PATCH_PRESSURE_OUT = 7, /* rho pressureOut: a far field whose rho_b and T_b follow the
-                                  boundary cells' characteristics after every step
-                                  (fvDVM.C:772-804); rho_b starts at 1                        */
+                                  boundary cells' characteristics, set from the initial state
+                                  by dugks_init_equilibrium and after every step
+                                  (fvDVM.C:772-804, 1071-1072)                                */
     DUGKS_PATCH_ZEROGRADIENT = 8  /* rho zeroGradient (gSurf zeroGradient): every face value is
                                   the owner cell's gBarPvol (discreteVelocity.C:551-555); only
                                   the outgoing half is relaxed in S7 (its field operators are

@@ -297,8 +297,8 @@ oracle *oracle_create(const dugks_case *cs)
                 o->Tb[b] = pp->T ? pp->T[i] : cs->T_ref;
                 o->rhob[b] = pp->rho ? pp->rho[i] : 0.0;
                 o->pressb[b] = pp->pressure;
-                /* pressureIn / pressureOut rho patch values start at one
-                 * (pressureInFvPatchField.C:75, pressureOutFvPatchField.C) */
+                /* pressureIn / pressureOut rho patch values start at one (pressureInFvPatchField.C:75,
+                 * pressureOutFvPatchField.C); init_equilibrium replaces them by S10's before step 1 */
                 if (pp->type == DUGKS_PATCH_PRESSURE_IN || pp->type == DUGKS_PATCH_PRESSURE_OUT) o->rhob[b] = 1.0;
             }
         }
@@ -435,6 +435,27 @@ int oracle_set_cache_faces(oracle *o, int on)
     return 0;
 }
 
+/* S10 updatePressureInOutBC (fvDVM.C:730-806): boundary rho (and T) of pressure patches from the
+ * boundary cells' current state, used by the next step's incoming Maxwellian.  Pressure inlet:
+ * rho_b = p / (R T_b) (fvDVM.C:764).  Pressure outlet, a_i = sqrt(R T_i (K + 5) / (K + 3)):
+ * rho_b = rho_i + (p - rho_i R T_i) / a_i^2, T_b = p / (R rho_i) (fvDVM.C:795-796).  Called after
+ * every step and once by the constructor (fvDVM.C:1071-1072), so step 1 already uses these. */
+static void update_pressure_bc(oracle *o)
+{
+    const int nb = o->nb, nfi = o->nfi, K = o->K;
+    for (int b = 0; b < nb; ++b) {
+        if (o->bkind[b] == DUGKS_PATCH_PRESSURE_IN) {
+            o->rhob[b] = o->pressb[b]/o->R/o->Tb[b];
+        } else if (o->bkind[b] == DUGKS_PATCH_PRESSURE_OUT) {
+            int i = o->owner[nfi + b];
+            double Ti = o->T[i], rhoi = o->rho[i];
+            double ai = sqrt(o->R*Ti*(K + 5)/(K + 3));
+            o->rhob[b] = rhoi + (o->pressb[b] - rhoi*o->R*Ti)/ai/ai;
+            o->Tb[b] = o->pressb[b]/(o->R*rhoi);
+        }
+    }
+}
+
 int oracle_init_equilibrium(oracle *o, const double *rho, const double *U, const double *T)
 {
     const int nc = o->nc, nv = o->nv, nb = o->nb;
@@ -480,6 +501,8 @@ int oracle_init_equilibrium(oracle *o, const double *rho, const double *U, const
             }
         }
     }
+    /* set initial rho in pressureIn/Out BC (fvDVM.C:1071-1072) */
+    update_pressure_bc(o);
     o->initialised = 1;
     return 0;
 }
@@ -945,19 +968,8 @@ static int do_step(oracle *o)
             for (int d = 0; d < 3; ++d) o->q[3*c + d] = fac*(double)o->qacc_c[3*c + d];
         }
     }
-    /* S10 updatePressureInOutBC (fvDVM.C:730-806): boundary rho (and T) of pressure patches from
-     * the boundary cells' new state, used by the next step's incoming Maxwellian */
-    for (int b = 0; b < nb; ++b) {
-        if (o->bkind[b] == DUGKS_PATCH_PRESSURE_IN) {
-            o->rhob[b] = o->pressb[b]/o->R/o->Tb[b];
-        } else if (o->bkind[b] == DUGKS_PATCH_PRESSURE_OUT) {
-            int i = o->owner[nfi + b];
-            double Ti = o->T[i], rhoi = o->rho[i];
-            double ai = sqrt(o->R*Ti*(K + 5)/(K + 3));
-            o->rhob[b] = rhoi + (o->pressb[b] - rhoi*o->R*Ti)/ai/ai;
-            o->Tb[b] = o->pressb[b]/(o->R*rhoi);
-        }
-    }
+    /* S10: the boundary cells' new state sets the next step's pressure-patch rho_b, T_b */
+    update_pressure_bc(o);
     (void)D;
     return bad ? DUGKS_ERR_NONFINITE : 0;
 }
@@ -1091,6 +1103,17 @@ int oracle_velocities(oracle *o, double *xi, double *w)
     if (xi) memcpy(xi, o->xi, sizeof(double)*3*o->nv);
     if (w) memcpy(w, o->wv, sizeof(double)*o->nv);
     return o->nv;
+}
+
+/* Boundary density and temperature of every boundary face [nb]: rho_b, T_b of far-field, fixed
+ * and pressure faces (pressure patches: as set by the last S10), the patch T elsewhere. */
+int oracle_boundary_state(oracle *o, double *rho_b, double *T_b)
+{
+    for (int b = 0; b < o->nb; ++b) {
+        if (rho_b) rho_b[b] = o->rhob[b];
+        if (T_b) T_b[b] = o->Tb[b];
+    }
+    return 0;
 }
 
 int oracle_incoming(oracle *o, double *incoming)

@@ -62,6 +62,7 @@ def lib(variant: str = "fp64"):
         L.oracle_tau.argtypes = [vp, C.c_double, C.c_double]
         L.oracle_velocities.argtypes = [vp, dp, dp]
         L.oracle_incoming.argtypes = [vp, dp]
+        L.oracle_boundary_state.argtypes = [vp, dp, dp]
         L.oracle_total_mass.restype = C.c_double
         L.oracle_total_mass.argtypes = [vp]
         _libs[variant] = L
@@ -188,6 +189,13 @@ class Oracle:
         a = np.zeros(max(1, nb))
         self._L.oracle_incoming(self._h, dptr(a))
         return a[:nb]
+
+    def boundary_state(self):
+        """rho_b, T_b of every boundary face (pressure patches: as the last S10 set them)."""
+        nb = self.case.n_faces - self.case.n_internal_faces
+        rb, Tb = np.zeros(max(1, nb)), np.zeros(max(1, nb))
+        self._L.oracle_boundary_state(self._h, dptr(rb), dptr(Tb))
+        return rb[:nb], Tb[:nb]
 
     def total_mass(self):
         return self._L.oracle_total_mass(self._h)

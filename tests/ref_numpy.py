@@ -40,7 +40,7 @@ class NumpyDUGKS:
                 self.Tb[b] = p["T"]
             elif p["type"] in (PATCH_PRESSURE_IN, PATCH_PRESSURE_OUT):
                 self.Tb[b] = p["T"]
-                self.rhob[b] = 1.0            # the pressure patch fields start at one
+                self.rhob[b] = 1.0            # the patch constructor's value, replaced by S10 below
                 self.press[b] = p["pressure"]
             elif p["type"] in (PATCH_FARFIELD, PATCH_FIXED):
                 self.Tb[b] = p["T"]
@@ -83,6 +83,7 @@ class NumpyDUGKS:
         M = self._maxwell_by_rho(self.Ub, self.Tb)          # [nv, nb]
         s = self.xi @ c.Sf[nfi:].T                           # [nv, nb]
         self.incoming = np.sum(np.where(s < 0, -self.w[:, None] * s * M, 0.0), axis=0)
+        self._pressure_bc()
 
     # ---- gas ----
     def _tau(self, T, rho, mu):
@@ -269,12 +270,17 @@ class NumpyDUGKS:
         q = 0.5 * np.einsum("k,kcd,kc->cd", w, cc, np.sum(cc ** 2, axis=2) * self.gt + self.ht)
         q = (2 * tau / (2 * tau + dt * Pr))[:, None] * q
         self.rho, self.U, self.T, self.q, self.tau = rho, U, T, q, tau
-        # S10 updatePressureInOutBC (fvDVM.C:730-806)
+        self._pressure_bc()
+
+    def _pressure_bc(self):
+        """S10 updatePressureInOutBC (fvDVM.C:730-806) from the current cell state: after every
+        step, and once in the constructor before the first step (fvDVM.C:1071-1072)."""
+        R, K = self.c.R, self.c.K
         pin = self.kind == PATCH_PRESSURE_IN
         self.rhob[pin] = self.press[pin] / R / self.Tb[pin]
         pout = np.nonzero(self.kind == PATCH_PRESSURE_OUT)[0]
         if pout.size:
-            Ti, rhoi = T[self.bP[pout]], rho[self.bP[pout]]
+            Ti, rhoi = self.T[self.bP[pout]], self.rho[self.bP[pout]]
             ai = np.sqrt(R * Ti * (K + 5) / (K + 3))
             self.rhob[pout] = rhoi + (self.press[pout] - rhoi * R * Ti) / ai / ai
             self.Tb[pout] = self.press[pout] / (R * rhoi)

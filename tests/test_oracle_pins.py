@@ -358,3 +358,35 @@ def test_symmetry_independent_of_velocity_order(oracle_lib):
             assert rel_l2(mb[k], ma[k]) < 1e-13, (plane, k)
         for k in ("U", "q"):
             assert rel_l2(mb[k], ma[k]) < 1e-11, (plane, k)
+
+
+def test_pressure_patches_set_before_the_first_step(oracle_lib):
+    """The fvDVM constructor runs updatePressureInOutBC() once before the first evolution()
+    (fvDVM.C:1071-1072), so step 1 already uses S10's boundary state, formed from the initial
+    cell state.  Checked by its physical meaning on a non-uniform initial state: the inlet state
+    (rho_b, T_b) has the patch pressure, p = rho_b R T_b (fvDVM.C:764); the outlet T_b gives the
+    patch pressure with the boundary cell's density, p = rho_i R T_b, and rho_b follows the
+    acoustic characteristic, (rho_b - rho_i) a_i^2 = p - rho_i R T_i with a_i^2 = R T_i (K + 5) /
+    (K + 3) (fvDVM.C:789-796).  The patch constructor's rho_b = 1 fails all of them."""
+    c = cases.pressure_channel(8, 4, nv=8, K=1)
+    rng = np.random.default_rng(3)
+    rho = c.rho0 * (1.0 + 0.2 * rng.uniform(-1, 1, c.n_cells))
+    T = c.T0 * (1.0 + 0.1 * rng.uniform(-1, 1, c.n_cells))
+    o = _oracle(c)
+    o.init_equilibrium(rho=rho, T=T)
+    rb, Tb = o.boundary_state()
+    nfi, R, K = c.n_internal_faces, c.R, c.K
+    inlet, outlet = (next(p for p in c.patches if p["name"] == n) for n in ("inlet", "outlet"))
+    bi = np.arange(inlet["size"]) + inlet["start"] - nfi
+    assert np.allclose(Tb[bi], inlet["T"], rtol=0, atol=0)
+    assert np.max(np.abs(rb[bi] * R * Tb[bi] / inlet["pressure"] - 1.0)) < 1e-14
+    bo = np.arange(outlet["size"]) + outlet["start"] - nfi
+    cell = c.owner[outlet["start"]: outlet["start"] + outlet["size"]]
+    p = outlet["pressure"]
+    assert np.max(np.abs(rho[cell] * R * Tb[bo] / p - 1.0)) < 1e-14
+    a2 = R * T[cell] * (K + 5) / (K + 3)
+    lhs, rhs = (rb[bo] - rho[cell]) * a2, p - rho[cell] * R * T[cell]
+    assert np.max(np.abs(lhs - rhs)) < 1e-12 * p
+    # the initial state is far from the patch pressures, so the constructor's value 1 is not
+    # accidentally right
+    assert np.min(np.abs(rb[bo] - 1.0)) > 1e-3 and np.min(np.abs(rb[bi] - 1.0)) > 1e-3
