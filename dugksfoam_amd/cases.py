@@ -697,22 +697,26 @@ def _hilbert_key(x, y, order: int = 16):
 def batch(n_cases: int = 256, nmin: int = 20, nmax: int = 80, nv: int = 28, seed: int = 1,
           kn_range=(0.01, 10.0), lid_range=(0.05, 0.3), tw_range=(250.0, 350.0),
           R: float = 208.0, T0: float = 273.0, rho0: float = 1.0, xi_sigma: float = 4.0,
-          Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0, co: float = 0.5) -> Case:
+          Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0, co: float = 0.5,
+          indices=None) -> Case:
     """Disjoint union of independent 2-D cavities (side s ~ U{nmin..nmax}, Kn log-uniform,
-    lid speed and wall temperature uniform), advanced together with per-case dt and muRef."""
+    lid speed and wall temperature uniform), advanced together with per-case dt and muRef.
+    `indices` keeps only those cases of the n_cases drawn (the draws are unchanged)."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(nmin, nmax + 1, size=n_cases)
     kns = np.exp(rng.uniform(np.log(kn_range[0]), np.log(kn_range[1]), size=n_cases))
     lids = rng.uniform(lid_range[0], lid_range[1], size=n_cases)
     tws = rng.uniform(tw_range[0], tw_range[1], size=n_cases)
-    xi_max = xi_sigma * math.sqrt(R * T0)
+    keep = np.arange(n_cases) if indices is None else np.asarray(indices, dtype=np.int64)
     parts = []
-    for ci in range(n_cases):
+    for ci in keep:
         c = cavity(int(sizes[ci]), nv, float(kns[ci]), 2, R, T0, rho0, xi_sigma, float(lids[ci]),
                    Pr, omega, K, co, T_wall=float(tws[ci]))
         parts.append(c)
-    return union(parts, name=f"batch{n_cases}_{nv}",
-                 meta=dict(sizes=sizes, kn=kns, lid=lids, T_wall=tws, seed=seed))
+    name = f"batch{n_cases}_{nv}" if indices is None else f"batch{n_cases}_{nv}_sel{keep.size}"
+    return union(parts, name=name,
+                 meta=dict(sizes=sizes[keep], kn=kns[keep], lid=lids[keep], T_wall=tws[keep], seed=seed,
+                           indices=keep))
 
 
 def union(parts: List[Case], name: str = "union", meta: dict = None) -> Case:

@@ -124,9 +124,11 @@ def adjoint_gradient(a: Sequence[float], steps: int = 2000, avg_last: int = 500,
 
 
 def mass_flow_function(steps: int = 2000, avg_last: int = 500, h: float = 1e-6, method: str = "adjoint",
-                       **channel_kw):
+                       backend: Optional[Callable] = None, **channel_kw):
     """A torch.autograd.Function computing J(a) (forward) and dJ/da (backward), the gradient
-    from the discrete adjoint (method "adjoint") or central differences (method "fd")."""
+    from the discrete adjoint (method "adjoint"; backend(case) as for adjoint_gradient) or central
+    differences (method "fd"; backend(case) as for mass_flow).  J and the gradient are returned
+    on the device and in the dtype of the coefficients a."""
     import torch
 
     if method not in ("adjoint", "fd"):
@@ -135,13 +137,13 @@ def mass_flow_function(steps: int = 2000, avg_last: int = 500, h: float = 1e-6, 
     class MassFlow(torch.autograd.Function):
         @staticmethod
         def forward(ctx, a):
-            an = a.detach().cpu().numpy()
+            an = a.detach().cpu().numpy().astype(np.float64)
             if method == "adjoint":
-                J, g = adjoint_gradient(an, steps, avg_last, **channel_kw)
+                J, g = adjoint_gradient(an, steps, avg_last, backend=backend, **channel_kw)
             else:
-                J, g = gradient(an, steps, avg_last, h, **channel_kw)
-            ctx.save_for_backward(torch.as_tensor(g, dtype=a.dtype))
-            return torch.as_tensor(J, dtype=a.dtype)
+                J, g = gradient(an, steps, avg_last, h, backend=backend, **channel_kw)
+            ctx.save_for_backward(torch.as_tensor(g, dtype=a.dtype, device=a.device))
+            return torch.as_tensor(J, dtype=a.dtype, device=a.device)
 
         @staticmethod
         def backward(ctx, grad_out):

@@ -20,6 +20,9 @@ LIB = os.path.join(HERE, "_build", "liboracle.so")
 LIB_EXT = os.path.join(HERE, "_build", "liboracle_ext.so")
 # Same source with every velocity pass reversed: sensitivity to summation order.
 LIB_REV = os.path.join(HERE, "_build", "liboracle_rev.so")
+# Same source built with AddressSanitizer and UndefinedBehaviorSanitizer (make san); loadable only
+# into a process started with the compiler's libasan preloaded.
+LIB_SAN = os.path.join(HERE, "_build", "liboracle_san.so")
 _libs = {}
 
 
@@ -33,9 +36,11 @@ def build(force: bool = False) -> str:
 
 
 def lib(variant: str = "fp64"):
-    path = {"fp64": LIB, "ext": LIB_EXT, "rev": LIB_REV}[variant]
+    path = {"fp64": LIB, "ext": LIB_EXT, "rev": LIB_REV, "san": LIB_SAN}[variant]
     if variant not in _libs:
-        if not os.path.exists(path):
+        if variant == "san":
+            subprocess.run(["make", "-s", "-C", HERE, "san"], check=True)
+        elif not os.path.exists(path):
             build()
         L = C.CDLL(path)
         vp = C.c_void_p

@@ -71,7 +71,8 @@ CASES = [
     ("open_zg", lambda: cases.open_channel(6, 4, nv=8, outlet="zeroGradient"), 6, 2, 2),
     # far-field outlet: incoming rho_b M(U of the outlet cell, T_b)
     ("open_far", lambda: cases.open_channel(6, 4, nv=8), 6, 2, 3),
-    # pressureIn / pressureOut: rho_b (and the outlet's T_b) from S10 of the previous step
+    # pressureIn / pressureOut: rho_b (and the outlet's T_b) from S10 of the previous step (of the
+    # initial state before the first step)
     ("pressure", lambda: cases.pressure_channel(6, 4, nv=8), 6, 1, 2),
     # a batch of two cavities with one time step and their own muRef
     ("batch", lambda: cases.union([cases.cavity(4, 6, kn=0.5), cases.cavity(4, 6, kn=0.1)]), 5, 2, 2),

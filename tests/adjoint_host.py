@@ -21,21 +21,29 @@ DEPS = [SRC] + [os.path.join(ROOT, "dugksfoam_amd", "csrc", n) for n in ("adjoin
 _lib = None
 
 
-def build(force: bool = False) -> str:
-    fresh = os.path.exists(OUT) and all(os.path.getmtime(OUT) >= os.path.getmtime(d) for d in DEPS)
+# AddressSanitizer + UndefinedBehaviorSanitizer build (CPU test suite only; loadable into a process
+# started with the compiler's libasan preloaded), chosen with DUGKS_ADJOINT_HOST_SAN=1
+OUT_SAN = os.path.join(HERE, "_build", "libadjoint_host_san.so")
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+
+
+def build(force: bool = False, sanitize: bool = False) -> str:
+    out = OUT_SAN if sanitize else OUT
+    fresh = os.path.exists(out) and all(os.path.getmtime(out) >= os.path.getmtime(d) for d in DEPS)
     if force or not fresh:
-        os.makedirs(os.path.dirname(OUT), exist_ok=True)
-        tmp = OUT + ".tmp"
-        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        tmp = out + ".tmp"
+        opt = SAN_FLAGS if sanitize else ["-O2"]
+        subprocess.run(["g++", *opt, "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
                         "-Wno-unknown-pragmas", "-I", os.path.join(ROOT, "include"), SRC, "-o", tmp], check=True)
-        os.replace(tmp, OUT)
-    return OUT
+        os.replace(tmp, out)
+    return out
 
 
 def lib():
     global _lib
     if _lib is None:
-        L = C.CDLL(build())
+        L = C.CDLL(build(sanitize=os.environ.get("DUGKS_ADJOINT_HOST_SAN") == "1"))
         vp, dp = C.c_void_p, C.POINTER(C.c_double)
         L.adjoint_host_create.argtypes = [vp, C.POINTER(vp)]
         L.adjoint_host_run.argtypes = [vp, dp, dp, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp, dp]

@@ -1,5 +1,6 @@
 """bench.py's contract pieces that need no GPU: the refusal of several GPUs, the algorithmic
-byte model behind roofline.achieved, and the committed rocprof traffic it reports."""
+bytes behind roofline.achieved (48 B per cell-velocity update in FP64, 24 B in FP32), the
+engine's own design byte model, the committed rocprof traffic and the host description."""
 import os
 import subprocess
 import sys
@@ -44,3 +45,22 @@ def test_committed_traffic_is_above_the_algorithmic_bytes():
     kb, _ = bench.bytes_model(c, 8, 1, 1, face_values=True, fused_cell=True)
     t = bench.traffic_from_profile("k_cell", "fp64")
     assert t is not None and t >= kb["k_cell"]
+
+
+def test_contract_bytes_per_update():
+    """roofline.achieved counts B = 6 s per update: g and h streamed in twice and out once."""
+    assert bench.contract_bytes_per_update(8) == 48
+    assert bench.contract_bytes_per_update(4) == 24
+
+
+def test_step_traffic_sums_the_step_kernels():
+    kernels = ["k_face_values", "k_face_moments", "k_face_fin", "k_cell"]
+    t = bench.step_traffic_from_profile("fp64", kernels)
+    assert t == sum(bench.traffic_from_profile(k, "fp64") for k in kernels)
+    assert bench.step_traffic_from_profile("fp64", kernels + ["no_such_kernel"]) is None
+
+
+def test_host_description():
+    h = bench.host_description(3)
+    assert h["threads_used"] == 3 and h["nproc"] >= 1
+    assert "cpu_model" in h and h["ram_GiB"] > 0

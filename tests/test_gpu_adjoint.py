@@ -96,7 +96,7 @@ def test_unsupported_and_invalid_inputs():
 def test_full_channel_gradient_matches_engine_differences():
     """The full micro-channel workload (200 x 40 cells, 32^2 velocities, 16 modes, 2000 steps,
     mid-plane mass flow over the last 500): dJ/da from the adjoint against central differences
-    of the FP64 engine for the two largest components (h = 1e-6; difference round-off ~1e-10)."""
+    of the FP64 engine for all 16 components (h = 1e-6), each within 1e-6 relative."""
     c = cases.config("channel")
     a = np.asarray(c.meta["a"], dtype=np.float64)
     kw = dict(nx=c.meta["nx"], ny=c.meta["ny"], Lx=c.meta["Lx"], kn=c.meta["kn"])
@@ -104,13 +104,17 @@ def test_full_channel_gradient_matches_engine_differences():
     J, g = sensitivity.adjoint_gradient(a, steps=2000, avg_last=500, checkpoint_every=100, **kw)
     Je = sensitivity.mass_flow(cases.channel(a=a, n_modes=a.size, **kw), 2000, 500)
     assert abs(J - Je) <= 1e-10 * abs(Je)
-    for k in np.argsort(-np.abs(g))[:2]:
+    bad = []
+    for k in range(a.size):
         ap, am = a.copy(), a.copy()
         ap[k] += 1e-6
         am[k] -= 1e-6
         Jp = sensitivity.mass_flow(cases.channel(a=ap, n_modes=a.size, **kw), 2000, 500)
         Jm = sensitivity.mass_flow(cases.channel(a=am, n_modes=a.size, **kw), 2000, 500)
         fd = (Jp - Jm) / 2e-6
-        print(f"mode {k + 1}: adjoint {g[k]:.10e} engine differences {fd:.10e}", flush=True)
-        assert abs(g[k] - fd) <= 1e-6 * abs(fd)
+        r = abs(g[k] - fd) / abs(fd)
+        print(f"mode {k + 1}: adjoint {g[k]:.10e} engine differences {fd:.10e} rel {r:.1e}", flush=True)
+        if not r <= 1e-6:
+            bad.append((k + 1, g[k], fd, r))
+    assert not bad, bad
 

@@ -2,7 +2,10 @@
 
 CPU (oracle backend): J vanishes for a uniform wall temperature (gas at rest, no driving) and is
 odd in the coefficients (the reflection x -> L - x maps T(x; a) to T(x; -a) and reverses the
-flow through the mid plane).  GPU: the engine's central-difference gradient equals the oracle's.
+flow through the mid plane).  The torch.autograd.Function: backward() gives the adjoint
+gradient (the host build of the adjoint on the CPU, the HIP adjoint on the GPU), which matches
+the oracle's central differences.  GPU: the engine's central-difference gradient equals the
+oracle's.
 """
 import numpy as np
 import pytest
@@ -40,3 +43,52 @@ def test_engine_gradient_matches_oracle(oracle_lib):
     print("J", Je, Jo, "grad", ge, go, flush=True)
     assert abs(Je - Jo) <= 1e-11 * abs(Jo)
     assert np.linalg.norm(ge - go) <= 1e-5 * np.linalg.norm(go)
+
+
+def _oracle_fd(oracle_lib, a, steps, avg_last):
+    return sensitivity.gradient(a, steps=steps, avg_last=avg_last, h=1e-6, backend=_oracle_backend(oracle_lib),
+                                **SMALL)
+
+
+def test_autograd_function_on_cpu(oracle_lib):
+    """mass_flow_function's backward() with the host build of the adjoint: equal to
+    adjoint_gradient, and within 1e-6 of the oracle's central differences; J equals the
+    oracle's mass flow; J and the gradient keep the coefficients' dtype and device."""
+    import torch
+    from adjoint_host import HostAdjoint
+    a0 = np.array([0.02, -0.01, 0.015])
+    steps, avg = 12, 4
+    fn = sensitivity.mass_flow_function(steps, avg, backend=HostAdjoint, **SMALL)
+    a = torch.tensor(a0, dtype=torch.float64, requires_grad=True)
+    J = fn.apply(a)
+    J.backward()
+    assert J.dtype == torch.float64 and J.device == a.device and a.grad.device == a.device
+    Ja, ga = sensitivity.adjoint_gradient(a0, steps, avg, backend=HostAdjoint, **SMALL)
+    np.testing.assert_array_equal(a.grad.numpy(), ga)
+    Jo, go = _oracle_fd(oracle_lib, a0, steps, avg)
+    assert abs(float(J) - Jo) <= 1e-12 * abs(Jo)
+    assert np.linalg.norm(a.grad.numpy() - go) <= 1e-6 * np.linalg.norm(go)
+    # a scaled output gradient scales the input gradient
+    a.grad = None
+    (3.0 * fn.apply(a)).backward()
+    np.testing.assert_allclose(a.grad.numpy(), 3.0 * ga, rtol=1e-15)
+
+
+@pytest.mark.gpu
+def test_autograd_function_on_gpu(oracle_lib):
+    """The same with the HIP adjoint and the coefficients on the GPU: J and dJ/da come back on
+    the GPU, backward() equals adjoint_gradient and the oracle's central differences (1e-6)."""
+    import torch
+    a0 = np.array([0.02, -0.01, 0.015])
+    steps, avg = 12, 4
+    fn = sensitivity.mass_flow_function(steps, avg, **SMALL)
+    a = torch.tensor(a0, dtype=torch.float64, device="cuda", requires_grad=True)
+    J = fn.apply(a)
+    J.backward()
+    assert J.device == a.device and a.grad.device == a.device
+    Ja, ga = sensitivity.adjoint_gradient(a0, steps, avg, **SMALL)
+    np.testing.assert_array_equal(a.grad.cpu().numpy(), ga)
+    Jo, go = _oracle_fd(oracle_lib, a0, steps, avg)
+    print("J", float(J), Jo, "grad", a.grad.cpu().numpy(), go, flush=True)
+    assert abs(float(J) - Jo) <= 1e-12 * abs(Jo)
+    assert np.linalg.norm(a.grad.cpu().numpy() - go) <= 1e-6 * np.linalg.norm(go)
