@@ -33,6 +33,7 @@ PATCH_FIXED = 5
 PATCH_PRESSURE_IN = 6
 PATCH_PRESSURE_OUT = 7
 PATCH_ZEROGRADIENT = 8
+PATCH_SYMMETRY_PLANE = 9
 
 # Boltzmann constant and argon data of demo/cavity/para.py:13-20
 K_BOLTZMANN = 1.38066e-23
@@ -402,18 +403,21 @@ def cavity(n: int = 60, nv: int = 41, kn: float = 0.075, dim: int = 2, R: float 
 def symmetry_cavity(nx: int = 30, ny: int = 60, nv: int = 28, kn: float = 0.075, R: float = 208.0,
                     T0: float = 273.0, rho0: float = 1.0, xi_sigma: float = 4.0, lid_mach: float = 0.16,
                     Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0, co: float = 0.5,
-                    plane: str = "x1") -> Case:
+                    plane: str = "x1", kind: str = "symmetryMod") -> Case:
     """Half of a 2-D cavity, nx x ny cells on [0, 0.5] x [0, 1], with a symmetry plane: the layout
     of the reference's demo/testSymmetry/symmetry case (lid y = 1 moving along +x, diffuse walls
     elsewhere, symmetryMod patch at x = 0.5).  plane = "x1" puts the plane on the right side
-    (outward normal +x), "x0" on the left (normal -x)."""
+    (outward normal +x), "x0" on the left (normal -x).  kind = "symmetryPlane" makes the plane an
+    OpenFOAM symmetryPlane patch (the serial form, doc/usage.tex:77-81) instead of symmetryMod."""
     d, C, V, own, nei, Sf, Cf, sides = box_mesh(nx, ny, 0, 0.5, 1.0)
     u_lid = lid_mach * math.sqrt(GAMMA_MONATOMIC * R * T0)
     other = "x0" if plane == "x1" else "x1"
     lid = _wall("movingWall", sides["y1"], (u_lid, 0, 0), T0)
     fixed = _wall("fixedWalls", _concat_sides([sides[other], sides["y0"]]), (0, 0, 0), T0)
-    sym = Patch("symmetryWall", PATCH_SYMMETRY, sides[plane]["owner"], sides[plane]["Sf"], sides[plane]["Cf"])
-    geo = _assemble(f"symcavity_{nx}x{ny}_{nv}", d, C, V, own, nei, Sf, Cf, [lid, fixed, sym])
+    ptype = {"symmetryMod": PATCH_SYMMETRY, "symmetryPlane": PATCH_SYMMETRY_PLANE}[kind]
+    sym = Patch("symmetryWall", ptype, sides[plane]["owner"], sides[plane]["Sf"], sides[plane]["Cf"])
+    tag = "spl" if kind == "symmetryPlane" else ""
+    geo = _assemble(f"symcavity{tag}_{nx}x{ny}_{nv}", d, C, V, own, nei, Sf, Cf, [lid, fixed, sym])
     xi_max = xi_sigma * math.sqrt(R * T0)
     xi, w = dv_newton_cotes(xi_max, nv)
     mu = mu_from_kn(kn, R, T0, rho0)

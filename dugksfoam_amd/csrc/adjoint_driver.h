@@ -95,6 +95,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
         code = DUGKS_ERR_INVALID;
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf) return "patch face range outside the boundary";
         code = DUGKS_ERR_UNSUPPORTED;
+        if (pp.type == DUGKS_PATCH_SYMMETRY_PLANE) return "symmetryPlane patches are not differentiated (symmetryMod is)";
         if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_FIXED &&
             pp.type != DUGKS_PATCH_ZEROGRADIENT && pp.type != DUGKS_PATCH_SYMMETRY &&
             pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_PRESSURE_IN &&

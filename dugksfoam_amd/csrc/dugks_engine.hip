@@ -62,6 +62,8 @@ constexpr int KIND_CYC = 3;   // cyclic (coupled) patch face
 constexpr int KIND_SYM = 4;   // symmetry plane (symmetryMod rho, DVMsymmetry gSurf, fixedGradient gBarPvol)
 constexpr int KIND_FAR = 5;   // incoming Maxwellian: far field (farField rho) or fixed (fixedValue rho)
 constexpr int KIND_ZG = 6;    // zeroGradient: every face value is the owner cell's gBarPvol
+constexpr int KIND_SPL = 7;   // OpenFOAM symmetryPlane: mirror plane, gBarPvol phi_b = phi_P (no lagged
+                              // row), every value relaxed by the field operation, outgoing ones twice
 
 // Boundary faces with a lagged fixedGradient gBarPvol row; open faces relax only outgoing values.
 __host__ __device__ constexpr inline bool grad_row(int kind)
@@ -785,6 +787,15 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
             const double dCo[3] = {S.dCo[0], S.dCo[1], S.dCo[2]};
             TS *__restrict__ fg = p.fvg + (size_t)S.face * p.nvp;
             TS *__restrict__ fh = H ? p.fvh + (size_t)S.face * p.nvp : nullptr;
+            if (kind == KIND_SPL) {
+                // outgoing values reconstructed from this cell (discreteVelocity.C:673-689); the
+                // incoming half (the mirrored velocities' values) is written by k_face_open
+                if (s > 0) {
+                    st_stream(fg, k, face_value<D>(KIND_WALL, s, gC, GC, dCs, 0.0, GC, dCo, 0.0, hx));
+                    if (H) st_stream(fh, k, face_value<D>(KIND_WALL, s, hC, HC, dCs, 0.0, HC, dCo, 0.0, hx));
+                }
+                continue;
+            }
             if (grad_row(kind)) {
                 const size_t bi = (size_t)S.bidx * p.nvp + k;
                 double ng = GC[0] * dCo[0] + GC[1] * dCo[1];
@@ -834,9 +845,9 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
 }
 
 // k_face_open: the incoming values (xi.S <= 0) of symmetry, far-field and fixed faces, one
-// workgroup per (face, velocity block).  Symmetry plane (discreteVelocity.C:746-790): the outgoing
-// value of the velocity mirrored across the plane, rebuilt at its index from the owner cell's gbar
-// and gradient.  Far field (discreteVelocity.C:588-602): rho_b M(U of the owner cell, T_b).
+// workgroup per (face, velocity block).  Symmetry plane (DVMsymmetry, discreteVelocity.C:746-790;
+// symmetryPlane, 791-815): the outgoing value of the velocity mirrored across the plane, rebuilt at
+// its index from the owner cell's gbar and gradient.  Far field (discreteVelocity.C:588-602): rho_b M(U of the owner cell, T_b).
 // Fixed (mixed, frozen at the initial boundary Maxwellian, discreteVelocity.C:312-344):
 // rho_b M(U_b, T_b).  h = g R T_b (K + 3 - D) for the Maxwellians.
 // ---------------------------------------------------------------------------------------------
@@ -855,7 +866,7 @@ __global__ __launch_bounds__(MAXNT) void k_face_open(P<TS> p)
     const double S[3] = {fx.S[0], fx.S[1], fx.S[2]};
     const double dP[3] = {fx.dP[0], fx.dP[1], fx.dP[2]};
     const CONST_AS WallC &W = cptr(p.wall)[fx.bidx];
-    const bool sym = fx.kind == KIND_SYM;
+    const bool sym = fx.kind == KIND_SYM || fx.kind == KIND_SPL;
     const bool fixedU = W.mode == 1.0;
     const double Ui[3] = {fixedU ? W.U[0] : cptr(p.cU)[3 * c], fixedU ? W.U[1] : cptr(p.cU)[3 * c + 1],
                           fixedU ? W.U[2] : cptr(p.cU)[3 * c + 2]};
@@ -1309,7 +1320,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
             // roles: this cell is the face owner P except on KIND_NEI faces
             const bool nei = kind == KIND_NEI;
             bool needO;
-            if (grad_row(kind)) needO = false;
+            if (grad_row(kind) || kind == KIND_SPL) needO = false;
             else if (nei) needO = need_P(KIND_OWN, s);
             else needO = need_N(kind, s);
             double gO = 0.0, hO = 0.0;
@@ -1347,9 +1358,12 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
                     if (H) hf = face_value<D>(kind, s, hC, HC, dCs, hO, HO, dCo, S.aux, hx) + oh;
                 }
                 // S7 (discreteVelocity.C:867-920): relax towards the face equilibrium; on symmetry,
-                // far-field and fixed patches only the outgoing values (their operators are no-ops)
-                if (kind == KIND_OWN || kind == KIND_NEI || kind == KIND_CYC || s > 0)
+                // far-field and fixed patches only the outgoing values (their operators are no-ops);
+                // on a symmetryPlane every value (the field operation, 880-881) and the outgoing
+                // ones once more (the loop over non-coupled patches, 905-920)
+                if (kind == KIND_OWN || kind == KIND_NEI || kind == KIND_CYC || kind == KIND_SPL || s > 0)
                     face_relax<D, H>(fe, tabF + j, xo, yo, zo, v, p.kc, p.K, gf, hf);
+                if (kind == KIND_SPL && s > 0) face_relax<D, H>(fe, tabF + j, xo, yo, zo, v, p.kc, p.K, gf, hf);
             }
             // ag -/+ tg: the sign is exact, so the scalar-signed FMA rounds as the subtraction
             const double sgn = nei ? 1.0 : -1.0;
@@ -2136,6 +2150,7 @@ int kind_of(int patch_type)
     switch (patch_type) {
     case DUGKS_PATCH_WALL: return KIND_WALL;
     case DUGKS_PATCH_SYMMETRY: return KIND_SYM;
+    case DUGKS_PATCH_SYMMETRY_PLANE: return KIND_SPL;
     case DUGKS_PATCH_ZEROGRADIENT: return KIND_ZG;
     default: return KIND_FAR;   // far field, fixed, pressure inlet / outlet
     }
@@ -2165,7 +2180,8 @@ int build(dugks_engine *h, const dugks_case *cs)
         const dugks_patch &pp = cs->patches[pi];
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf)
             return fail(h, DUGKS_ERR_INVALID, "patch " + std::to_string(pi) + " face range outside the boundary");
-        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_SYMMETRY &&
+        const bool symp = pp.type == DUGKS_PATCH_SYMMETRY || pp.type == DUGKS_PATCH_SYMMETRY_PLANE;
+        if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && !symp &&
             pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_FIXED &&
             pp.type != DUGKS_PATCH_PRESSURE_IN && pp.type != DUGKS_PATCH_PRESSURE_OUT &&
             pp.type != DUGKS_PATCH_ZEROGRADIENT)
@@ -2183,8 +2199,9 @@ int build(dugks_engine *h, const dugks_case *cs)
         if (pp.type == DUGKS_PATCH_WALL) {
             if (!pp.U || !pp.T) return fail(h, DUGKS_ERR_INVALID, "wall patch without U/T");
             h->patch_wall0[pi] = nw;
-        } else if (pp.type == DUGKS_PATCH_SYMMETRY) {
-            // the mirror axis comes from the patch's first face normal (discreteVelocity.C:766-768)
+        } else if (symp) {
+            // the mirror axis comes from the patch's first face normal (discreteVelocity.C:766-768,
+            // 801-803)
             h->has_sym = 1;
             if (pp.size) {
                 const double *S0 = cs->Sf + 3 * (size_t)pp.start;
@@ -2214,8 +2231,9 @@ int build(dugks_engine *h, const dugks_case *cs)
                 Tb[b] = pp.T[i];
                 if (!(Tb[b] > 0)) return fail(h, DUGKS_ERR_INVALID, "wall temperature must be > 0");
                 bwall[b] = nw++;
-            } else if (pp.type == DUGKS_PATCH_SYMMETRY) {
-                // a fixedGradient row like a wall's; the wall constants are placeholders
+            } else if (symp) {
+                // symmetryMod: a fixedGradient row like a wall's; symmetryPlane: a record whose row
+                // is never used (phi_b = phi_P); the wall constants are placeholders
                 Tb[b] = cs->T_ref;
                 bsym[b] = sym_axis;
                 bwall[b] = nw++;
@@ -2310,7 +2328,7 @@ int build(dugks_engine *h, const dugks_case *cs)
                 if (bkind[b] != DUGKS_PATCH_CYCLIC) {
                     s.kind = kind_of(bkind[b]);
                     s.bidx = bwall[b];
-                    if (s.kind == KIND_SYM) s.other = bsym[b];
+                    if (s.kind == KIND_SYM || s.kind == KIND_SPL) s.other = bsym[b];
                     for (int d = 0; d < 3; ++d) s.dCo[d] = cs->Sf[3 * f + d] / magSf[f];
                     const double *dl = cs->delta + 3 * b;
                     s.aux = 1.0 / (1.0 / std::sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]));
@@ -2348,9 +2366,9 @@ int build(dugks_engine *h, const dugks_case *cs)
             if (bkind[b] != DUGKS_PATCH_CYCLIC) {
                 x.kind = kind_of(bkind[b]);
                 x.bidx = bwall[b];
-                if (x.kind == KIND_SYM) x.N = bsym[b];
+                if (x.kind == KIND_SYM || x.kind == KIND_SPL) x.N = bsym[b];
                 // incoming halves written by k_face_open
-                if (x.kind == KIND_SYM || x.kind == KIND_FAR) open_faces.push_back(f);
+                if (x.kind == KIND_SYM || x.kind == KIND_SPL || x.kind == KIND_FAR) open_faces.push_back(f);
                 for (int d = 0; d < 3; ++d) x.dN[d] = cs->Sf[3 * f + d] / magSf[f];
             } else {
                 x.kind = KIND_CYC;
@@ -2423,6 +2441,10 @@ int build(dugks_engine *h, const dugks_case *cs)
                 t.off[j] = (int64_t)(nc + q.bidx) * h->nvp;
                 t.a[j] = q.aux;
                 t.wmask |= 1 << j;
+            } else if (q.kind == KIND_SPL) {
+                // symmetryPlane: phi_b = phi_P, so the entry reads the cell's own row and adds
+                // L (phi_P - phi_P) = 0 (leastSquaresGrad's boundary term)
+                t.off[j] = (int64_t)c * h->nvp;
             } else {
                 t.off[j] = (int64_t)q.other * h->nvp;
             }
@@ -2881,7 +2903,8 @@ int dugks_init_equilibrium(dugks_engine *h, const double *rho, const double *U, 
                 const int f = s.face;
                 if (s.kind == KIND_WALL) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];
-                } else if (s.kind == KIND_SYM || s.kind == KIND_ZG || (s.kind == KIND_FAR && wc[s.bidx].mode != 1.0)) {
+                } else if (s.kind == KIND_SYM || s.kind == KIND_SPL || s.kind == KIND_ZG ||
+                           (s.kind == KIND_FAR && wc[s.bidx].mode != 1.0)) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = U[3 * c + d];   // U zeroGradient there
                 } else if (s.kind == KIND_FAR) {
                     for (int d = 0; d < 3; ++d) fU[3 * f + d] = wc[s.bidx].U[d];

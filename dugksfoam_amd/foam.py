@@ -6,7 +6,8 @@ read_case(path) builds a Case (dugksfoam_amd.cases.Case) from the files a dugksF
   0/{rho,U,T}                     initial fields (uniform or nonuniform internalField) and the
                                   boundary types: rho calculatedMaxwell + U, T fixedValue is a
                                   diffuse Maxwell wall (doc/usage.tex:61-75); rho symmetryMod is a
-                                  mirror plane; rho farField / fixedValue with T fixedValue are
+                                  mirror plane, and so is a symmetryPlane patch (mesh and fields,
+                                  doc/usage.tex:77-81); rho farField / fixedValue with T fixedValue are
                                   open boundaries with an incoming Maxwellian; empty patches are
                                   omitted (fvPatch size 0); cyclic patches are coupled pairs
   constant/DVMProperties          fvDVMparas {nDV, xiMax, xiMin}, gasProperties
@@ -19,8 +20,9 @@ Geometry follows OpenFOAM's primitiveMesh: face centres and area vectors by tria
 decomposition about the point average of the face; cell centres and volumes by pyramid
 decomposition about the average of the cell's face centres.  Interpolation weights and
 boundary deltas are those of cases._assemble (surfaceInterpolation::makeWeights, the
-patch-normal fvPatch::delta).  Patch types other than walls, symmetryMod planes, farField and
-fixedValue open boundaries, cyclics and empty are rejected.
+patch-normal fvPatch::delta).  Patch types other than walls, symmetryMod and symmetryPlane planes,
+farField, fixedValue, pressureIn/pressureOut and zeroGradient open boundaries, cyclics and empty
+are rejected.
 """
 from __future__ import annotations
 
@@ -296,6 +298,15 @@ def read_case(path: str, dt: float = None) -> cases.Case:
         s, n = b["startFace"], b["nFaces"]
         sl = slice(s, s + n)
         rtype = rho_b[b["name"]]["type"]
+        if b["type"] == "symmetryPlane" or rtype == "symmetryPlane":
+            # OpenFOAM constraint type: the mesh patch and rho, U, T are all symmetryPlane
+            # (doc/usage.tex:80-81); gSurf follows as symmetryPlane (discreteVelocity.C:257-261)
+            types = {b["type"], rtype, U_b[b["name"]]["type"], T_b[b["name"]]["type"]}
+            if types != {"symmetryPlane"}:
+                raise NotImplementedError(f"patch {b['name']}: a symmetryPlane patch needs the symmetryPlane "
+                                          f"type in polyMesh/boundary and in 0/rho, 0/U, 0/T")
+            plist.append(cases.Patch(b["name"], cases.PATCH_SYMMETRY_PLANE, owner[sl], Sf_all[sl], Cf_all[sl]))
+            continue
         if b["type"] == "cyclic":
             plist.append(cases.Patch(b["name"], cases.PATCH_CYCLIC, owner[sl], Sf_all[sl], Cf_all[sl],
                                      neighbour_patch=index[b["neighbourPatch"]]))

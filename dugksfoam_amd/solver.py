@@ -118,8 +118,9 @@ class Solver:
         return self.macro
 
     def write(self):
-        """<case>/<time>/{rho,U,T,q}; wall patches carry rho_w, the wall U and T, and qWall; the
-        other non-coupled patches carry their boundary cells' values (calculated)."""
+        """<case>/<time>/{rho,U,T,q}; wall patches carry rho_w, the wall U and T, and qWall;
+        cyclic and symmetryPlane patches their type alone; the other non-coupled patches carry
+        their boundary cells' values (calculated)."""
         m = self.fields()
         tn = time_name(self.t, self.ctl["timePrecision"])
         d = os.path.join(self.case_dir, tn)
@@ -133,9 +134,10 @@ class Solver:
                     bf[k][name] = ("empty", None)
                 continue
             p = self.case.patches[live[name]]
-            if p["type"] == cases.PATCH_CYCLIC:
+            if p["type"] in (cases.PATCH_CYCLIC, cases.PATCH_SYMMETRY_PLANE):
+                # OpenFOAM constraint types: written by type alone
                 for k in bf:
-                    bf[k][name] = ("cyclic", None)
+                    bf[k][name] = ("cyclic" if p["type"] == cases.PATCH_CYCLIC else "symmetryPlane", None)
                 continue
             if p["type"] != cases.PATCH_WALL:
                 cells = self.case.owner[p["start"]:p["start"] + p["size"]]

@@ -65,10 +65,19 @@ enum dugks_patch_type {
                                   boundary cells' characteristics, set from the initial state
                                   by dugks_init_equilibrium and after every step
                                   (fvDVM.C:772-804, 1071-1072)                                */
-    DUGKS_PATCH_ZEROGRADIENT = 8  /* rho zeroGradient (gSurf zeroGradient): every face value is
+    DUGKS_PATCH_ZEROGRADIENT = 8, /* rho zeroGradient (gSurf zeroGradient): every face value is
                                   the owner cell's gBarPvol (discreteVelocity.C:551-555); only
                                   the outgoing half is relaxed in S7 (its field operators are
                                   no-ops, zeroGradientFvsPatchField.H:163-181)                */
+    DUGKS_PATCH_SYMMETRY_PLANE = 9 /* OpenFOAM symmetryPlane patch (rho, U, T symmetryPlane; the
+                                  serial symmetry boundary of doc/usage.tex:77-81): axis-aligned
+                                  mirror plane.  gBarPvol takes the constraint value phi_b =
+                                  phi_P (no lagged gradient row; the face still enters the
+                                  least-squares geometry); outgoing values (xi.S > -VSMALL) are
+                                  reconstructed, incoming ones are the mirrored velocity's face
+                                  value (discreteVelocity.C:673-689, 791-815); S7 relaxes every
+                                  value through the field operation and the outgoing ones
+                                  (xi.S > 0) a second time (discreteVelocity.C:880-881, 905-920) */
 };
 
 /* Storage precision of the resident distributions. */
@@ -233,7 +242,7 @@ int dugks_abi_version(void);
  * The run starts from the Shakhov equilibrium of rho, U, T with q = 0 (as
  * dugks_init_equilibrium) and advances n_steps FP64 steps of size dt, keeping a checkpoint every
  * checkpoint_every steps (0: ceil(sqrt(n_steps))); the reverse sweep re-runs each segment.
- * Cases: every patch kind; batches (n_cases > 1) with their per-case muRef and the one dt; FP64 (the case's
+ * Cases: every patch kind except symmetryPlane (DUGKS_ERR_UNSUPPORTED); batches (n_cases > 1) with their per-case muRef and the one dt; FP64 (the case's
  * storage field is ignored).  Device memory: about
  * (ceil(n_steps / C) + C + 1) x (2 n_cells + 2 n_boundary_faces) x nv x 8 bytes for the trajectory
  * plus about (8 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;

@@ -17,6 +17,10 @@
  *   S5 updateGHbarSurfSymmetryIn     fvDVM.C:375-454, discreteVelocity.C:733-817
  *      (symmetryMod rho patches, DVMsymmetry gSurf: incoming = the mirrored
  *      velocity's outgoing face value; gBarPvol fixedGradient as on walls)
+ *      symmetryPlane patches (OpenFOAM constraint type, the serial symmetry
+ *      boundary): the same incoming copy; gBarPvol takes phi_b = phi_P (no lagged
+ *      gradient), and S7 relaxes every value through the field operation
+ *      (discreteVelocity.C:880-881) and the outgoing ones again (905-920)
  *   far-field (rho farField) and fixed (rho fixedValue, gSurf mixed) patches:
  *      incoming values rho_b M(U, T_b), discreteVelocity.C:556-604
  *   S10 updatePressureInOutBC   fvDVM.C:730-806 (pressureIn / pressureOut patches,
@@ -309,7 +313,7 @@ oracle *oracle_create(const dugks_case *cs)
     o->bmirror = (int *)xmalloc(sizeof(int)*(nb + 1));
     for (int p = 0; p < cs->n_patches; ++p) {
         const dugks_patch *pp = cs->patches + p;
-        if (pp->type != DUGKS_PATCH_SYMMETRY || pp->size == 0) continue;
+        if ((pp->type != DUGKS_PATCH_SYMMETRY && pp->type != DUGKS_PATCH_SYMMETRY_PLANE) || pp->size == 0) continue;
         const double *S0 = cs->Sf + 3*pp->start;
         double m = sqrt(magsqr3(S0));
         int ax = 0;
@@ -557,6 +561,9 @@ static void sym_recon(const oracle *o, int k, int b, double *gout, double *hout)
                 gbar_at(o, k, o->owner[nfi + o->bpartner[bb]], &gn, &hn);
                 dg = gn - gc;
                 dh = hn - hc;
+            } else if (o->bkind[bb] == DUGKS_PATCH_SYMMETRY_PLANE) {
+                dg = gc - gc;   /* symmetryPlane: phi_b = phi_P */
+                dh = hc - hc;
             } else {
                 dg = (gc + gbk[bb]/o->dcoef[bb]) - gc;
                 dh = (hc + hbk[bb]/o->dcoef[bb]) - hc;
@@ -596,7 +603,11 @@ static void compute_k(oracle *o, int k, int want_faces)
     #pragma omp for schedule(static)
     for (int b = 0; b < nb; ++b) {
         int f = nfi + b, P = o->owner[f];
-        if (o->bkind[b] != DUGKS_PATCH_CYCLIC) { /* fixedGradient: walls, symmetry */
+        if (o->bkind[b] == DUGKS_PATCH_SYMMETRY_PLANE) {
+            /* symmetryPlaneFvPatchField of a scalar: (phi_P + transform(phi_P))/2 = phi_P */
+            o->phib_g[b] = o->gbar[P];
+            o->phib_h[b] = o->hbar[P];
+        } else if (o->bkind[b] != DUGKS_PATCH_CYCLIC) { /* fixedGradient: walls, symmetry */
             o->phib_g[b] = o->gbar[P] + gbk[b]/o->dcoef[b];
             o->phib_h[b] = o->hbar[P] + hbk[b]/o->dcoef[b];
         } else { /* cyclic */
@@ -706,7 +717,7 @@ static void compute_k(oracle *o, int k, int want_faces)
                  * discreteVelocity.C:551-555) */
                 o->gs[f] = o->gbar[own];
                 o->hs[f] = o->hbar[own];
-            } else if (o->bkind[b] == DUGKS_PATCH_SYMMETRY) {
+            } else if (o->bkind[b] == DUGKS_PATCH_SYMMETRY || o->bkind[b] == DUGKS_PATCH_SYMMETRY_PLANE) {
                 if (s > 0) {
                     /* outgoing, discreteVelocity.C:673-689 */
                     o->gs[f] = o->gbar[own] + dot3(o->grad_g + 3*own, dO);
@@ -898,12 +909,24 @@ static int do_step(oracle *o)
                 double h = 0.5*dt;
                 double rf = h/(2*o->tau_f[f] + h);
                 double s = dot3(xi, o->Sf + 3*f);
-                int relax = (f < nfi) || o->bkind[f - nfi] == DUGKS_PATCH_CYCLIC || s > 0;
-                if (relax) {
+                /* the field operation (discreteVelocity.C:880-881) acts on internal faces and on
+                 * patches of OpenFOAM types (cyclic, symmetryPlane); the DF-tag patches ignore it.
+                 * The loop of 905-920 then relaxes the outgoing values of every non-coupled patch,
+                 * so a symmetryPlane's outgoing values are relaxed twice */
+                int spl = f >= nfi && o->bkind[f - nfi] == DUGKS_PATCH_SYMMETRY_PLANE;
+                int field = (f < nfi) || o->bkind[f - nfi] == DUGKS_PATCH_CYCLIC || spl;
+                int loop = f >= nfi && o->bkind[f - nfi] != DUGKS_PATCH_CYCLIC && s > 0;
+                if (field || loop) {
                     double geq, heq;
                     shakhov(o, o->rho_f[f], o->U_f + 3*f, o->T_f[f], o->q_f + 3*f, xi, &geq, &heq);
-                    gs[f] = (1.0 - rf)*gs[f] + rf*geq;
-                    hs[f] = (1.0 - rf)*hs[f] + rf*heq;
+                    if (field) {
+                        gs[f] = (1.0 - rf)*gs[f] + rf*geq;
+                        hs[f] = (1.0 - rf)*hs[f] + rf*heq;
+                    }
+                    if (loop) {
+                        gs[f] = (1.0 - rf)*gs[f] + rf*geq;
+                        hs[f] = (1.0 - rf)*hs[f] + rf*heq;
+                    }
                 }
             }
             /* S8, discreteVelocity.C:937-975; per cell in face order */

@@ -26,18 +26,28 @@ FP64_TOL = 1e-12       # FP64 parity bar, relative L2
 FLOOR_FACTOR = 2.0
 
 
-def parity_ok(err_oracle, err_ext, floor):
+def parity_ok(err_oracle, err_ext, floor, floor_ext=0.0):
     """The FP64 parity bar.  Either the engine is within 1e-12 of the FP64 oracle, or the field
     is one whose sums cancel almost to zero (the velocity one step after rest, a slow
-    transpiration flow, the heat flux of cells along two walls), where the reference's
-    sequential FP64 sums are themselves that far from the exact sums: then the engine must be
-    within 1e-12 of the extended-precision oracle (80-bit intermediates and velocity sums;
-    err_ext, None when it was not evaluated) and within FLOOR_FACTOR floors of the FP64 one
-    (floor: the FP64 oracle's distance from the extended-precision and reversed-sum oracles).
+    transpiration flow, the heat flux of cells along two walls), where the reference's FP64
+    arithmetic is itself that far from the exact result.  Then two checks must hold:
+      - the engine is within 1e-12 of the extended-precision oracle (80-bit intermediates and
+        velocity sums; err_ext, None when it was not evaluated), or, where the FP64 oracle is
+        itself farther than that from it (floor_ext: their distance), no farther from it than
+        the FP64 oracle is: no FP64 evaluation of the step can do better;
+      - the engine is within FLOOR_FACTOR floors of the FP64 oracle (floor: the FP64 oracle's
+        distance from the extended-precision and reversed-sum oracles).
     A difference of the step itself fails both."""
     if err_oracle <= FP64_TOL:
         return True
-    return err_ext is not None and err_ext <= FP64_TOL and err_oracle <= FLOOR_FACTOR * floor
+    if err_ext is None:
+        return False
+    return err_ext <= max(FP64_TOL, floor_ext) and err_oracle <= FLOOR_FACTOR * floor
+
+
+def seq_mass(V, rho):
+    """Total mass summed cell by cell in cell order, as the oracle's total_mass sums it."""
+    return float(np.cumsum(np.asarray(V) * np.asarray(rho))[-1])
 
 
 @pytest.fixture(scope="session")

@@ -341,3 +341,39 @@ def test_solver_writes_open_patches_and_df(tmp_path, oracle_lib):
     assert rb["symmetryWall"]["type"] == "calculated"
     p = case.patches[2]
     np.testing.assert_array_equal(rb["symmetryWall"]["value"], rho[case.owner[p["start"]:p["start"] + p["size"]]])
+
+
+def test_symmetry_plane_case_reads_and_writes(tmp_path, oracle_lib):
+    """A symmetryPlane patch (polyMesh type and rho, U, T types symmetryPlane, doc/usage.tex:77-81)
+    reads as a symmetryPlane patch and runs like the generated case; time directories write it
+    by type.  A symmetryPlane rho on a plain wall-type mesh patch is rejected, as OpenFOAM
+    rejects a constraint field type on a non-constraint patch."""
+    c = cases.symmetry_cavity(4, 8, nv=8, kind="symmetryPlane")
+    lid, fixed = foam_box.wall(c.patches[0]["U"][0], c.T0[0]), foam_box.wall((0, 0, 0), c.T0[0])
+    spl = "        type symmetryPlane;\n"
+    fields = {}
+    for k in ("rho", "U", "T"):
+        internal = {"rho": f"uniform {float(c.rho0[0])!r}", "U": "uniform (0 0 0)", "T": f"uniform {float(c.T0[0])!r}"}[k]
+        fields[k] = (internal, {"movingWall": lid[k], "fixedWalls": fixed[k], "symmetryWall": spl,
+                                "frontAndBack": "        type empty;\n"})
+    patches = [("movingWall", "wall", ["y1"], ""), ("fixedWalls", "wall", ["x0", "y0"], ""),
+               ("symmetryWall", "symmetryPlane", ["x1"], ""), ("frontAndBack", "empty", ["z0", "z1"], "")]
+    foam_box.write_box(str(tmp_path), 4, 8, 0, L=(0.5, 1.0, 1.0), patches=patches, fields=fields,
+                       dvm=foam_box.dvm_props(c), xi=c.xi, w=c.xi_weights, control=foam_box.control(c.dt, writeInterval=2))
+    case = foam.read_case(str(tmp_path))
+    assert [p["type"] for p in case.patches] == [cases.PATCH_WALL, cases.PATCH_WALL, cases.PATCH_SYMMETRY_PLANE]
+    _same_case(case, c)
+    s = solver.Solver(str(tmp_path), backend=_OracleBackend(case, oracle_lib), log=None)
+    s.run(max_steps=2)
+    sizes = {p["name"]: p["nFaces"] for p in case.meta["boundary"]}
+    _, rb = foam.read_field(str(tmp_path / solver.time_name(s.t, 9) / "rho"), case.n_cells, sizes, vector=False)
+    assert rb["symmetryWall"]["type"] == "symmetryPlane" and rb["symmetryWall"]["value"] is None
+    o = oracle_lib.Oracle(c)
+    o.init_equilibrium()
+    o.step(nsteps=2)
+    for k, v in o.macro().items():
+        np.testing.assert_allclose(s.fields()[k], v, rtol=1e-12, atol=1e-12 * np.abs(v).max())
+    bnd = tmp_path / "constant" / "polyMesh" / "boundary"
+    bnd.write_text(bnd.read_text().replace("type            symmetryPlane;", "type            wall;"))
+    with pytest.raises(NotImplementedError, match="symmetryPlane"):
+        foam.read_case(str(tmp_path))

@@ -4,10 +4,11 @@ The reference's demo/cavity (from its array fixture: 3600 cells, 28^2 velocities
 rarefied argon-like gas, adjustTimeStep at maxCo 0.8), its demo/testSymmetry/symmetry (the
 left half with a symmetryMod plane) and a cyclic channel written in OpenFOAM format run through dugksfoam_amd.solver.Solver on the HIP engine and on the FP64
 oracle.  Bar: conftest.parity_ok on rho, U, T, q -- relative L2 <= 1e-12 against the FP64
-oracle, or, where the oracle's own sums cancel (floor: its distance from the oracle with every
-velocity sum reversed and from the extended-precision oracle, both run live), <= 1e-12 against
-the extended-precision oracle -- and the same time-step sequence to 1e-13 (1e-8 with FP32
-storage, whose bar is 1e-5).
+oracle, or, where the oracle's own FP64 arithmetic is that far from the exact result (floor: its
+distance from the oracle with every velocity sum reversed and from the extended-precision
+oracle, both run live), within 1e-12 of the extended-precision oracle or no farther from it than
+the FP64 oracle is -- and the same time-step sequence to 1e-13 (1e-8 with FP32 storage, whose bar
+is 1e-5).
 """
 import os
 import sys
@@ -88,10 +89,11 @@ def _compare(name, path, snaps, storage="fp64", tol=None):
                     _say(f"  {name} step {n} {k}: {r:.2e} (bar {tol:.1e})")
                     assert r <= tol, (name, n, k, r)
                     continue
-                floor = max(rel_l2(rr[n][k], ro[n][k]), rel_l2(rx[n][k], ro[n][k]))
+                flx = rel_l2(rx[n][k], ro[n][k])
+                floor = max(rel_l2(rr[n][k], ro[n][k]), flx)
                 r_ext = rel_l2(re[n][k], rx[n][k])
-                _say(f"  {name} step {n} {k}: {r:.2e} (ext {r_ext:.2e}, floor {floor:.2e})")
-                assert parity_ok(r, r_ext, floor), (name, n, k, r, r_ext, floor)
+                _say(f"  {name} step {n} {k}: {r:.2e} (ext {r_ext:.2e}, oracle-ext {flx:.2e}, floor {floor:.2e})")
+                assert parity_ok(r, r_ext, floor, flx), (name, n, k, r, r_ext, floor, flx)
     finally:
         se.b.close()
 
