@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1661,9 +1662,11 @@ __global__ __launch_bounds__(256) void k_wall_init(P<TS> p, const int *wall_face
 }
 
 // g~ of selected cells for get_df / save: mode-2 conversion into a scratch row buffer.
+// ref_nv > 0: rows of ref_nv values in the reference velocity order (k = (iz nY + iy) nX + ix),
+// padding slots skipped; else rows of nvp values in the engine's sign-sorted order.
 template <int D, bool H, typename TS>
 __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, const double *dt_old, int raw,
-                                                  double *g_out, double *h_out)
+                                                  double *g_out, double *h_out, int ref_nv)
 {
     __shared__ double tab[NAXMAX];
     __shared__ double vt[VT_SIZE];
@@ -1693,12 +1696,26 @@ __global__ __launch_bounds__(MAXNT) void k_extract(P<TS> p, const int *cells, co
             g = (g - ro * geq) * inv1mr;
             h = (h - ro * heq) * inv1mr;
         }
-        g_out[(size_t)blockIdx.x * p.nvp + k] = g;
-        h_out[(size_t)blockIdx.x * p.nvp + k] = H ? h : 0.0;
+        size_t o = (size_t)blockIdx.x * p.nvp + k;
+        if (ref_nv > 0) {
+            const uint32_t id = p.vidx[k];
+            if (id & VPAD) continue;
+            const int ix = id & 0xff, iy = (id >> 8) & 0xff, iz = (id >> 16) & 0xff;
+            o = (size_t)blockIdx.x * ref_nv + ((size_t)iz * p.nxi + iy) * p.nxi + ix;
+        }
+        g_out[o] = g;
+        h_out[o] = H ? h : 0.0;
     }
 }
 
 }  // namespace
+
+// Device scratch of dugks_get_df_noalloc: the cell list and the g, h rows of one call.  One copy per
+// GPU, held until the process ends; every call on a GPU shares it, so calls that overlap on one
+// GPU race.  Host code reaches it only through the address hipGetSymbolAddress gives for the
+// current device (df_scratch_ptr); the array's name is not a device address on the host.
+constexpr size_t DF_SCRATCH_DOUBLES = DUGKS_DF_NOALLOC_MAX_DOUBLES;
+static __device__ __align__(256) double scratch[DF_SCRATCH_DOUBLES];
 
 // =============================================================================================
 // host side
@@ -1762,6 +1779,8 @@ struct dugks_engine {
     void *fvg = nullptr, *fvh = nullptr;
     int cur = 0;         // buffer holding the current state
     int raw = 1;         // state holds g~ (1) or gbar prepared with dt_prep (0)
+    double *df_scratch = nullptr;            // this device's copy of `scratch` (dugks_get_df_noalloc)
+    hipError_t df_scratch_status = hipSuccess;
     int initialised = 0;
     std::vector<double> dt_prep;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -2005,12 +2024,13 @@ struct ExtractOp {
     const int *cells;
     int n;
     double *g, *hh;
+    int ref_nv;   // > 0: rows in reference velocity order (see k_extract)
     template <int D, bool H, typename TS>
     void operator()()
     {
         P<TS> p = params<TS>(h);
         hipLaunchKernelGGL((k_extract<D, H, TS>), dim3(n), dim3(h->nt), 0, h->stream, p, cells,
-                           (const double *)h->dtp, h->raw, g, hh);
+                           (const double *)h->dtp, h->raw, g, hh, ref_nv);
     }
 };
 
@@ -2811,6 +2831,28 @@ extern "C" {
 
 int dugks_abi_version(void) { return DUGKS_ABI_VERSION; }
 
+namespace {
+// Address of `scratch` on each device, resolved once per device with that device current.
+std::mutex df_scratch_mu;
+constexpr int DF_SCRATCH_MAX_DEVICES = 64;
+double *df_scratch_dev[DF_SCRATCH_MAX_DEVICES];
+
+hipError_t df_scratch_ptr(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(df_scratch_mu);
+    if (device < 0 || device >= DF_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!df_scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        df_scratch_dev[device] = (double *)p;
+    }
+    *out = df_scratch_dev[device];
+    return hipSuccess;
+}
+}  // namespace
+
 int dugks_create(const dugks_case *cs, dugks_engine **out)
 {
     if (!out) return DUGKS_ERR_INVALID;
@@ -2853,6 +2895,12 @@ int dugks_create(const dugks_case *cs, dugks_engine **out)
         // DUGKS_FUSE_S1=0 keeps k_cell_fin and k_cell_s1 as kernels of their own
         const char *f1 = std::getenv("DUGKS_FUSE_S1");
         h->fuse_s1 = f1 && !std::atoi(f1) ? 0 : 1;
+    }
+    if (!r) {
+        // this device's copy of the dugks_get_df_noalloc scratch; a failure is reported by that
+        // entry point only
+        h->df_scratch_status = hipSetDevice(h->device);
+        if (h->df_scratch_status == hipSuccess) h->df_scratch_status = df_scratch_ptr(h->device, &h->df_scratch);
     }
     if (r) {
         // the handle is returned without device resources so that dugks_last_error can
@@ -3044,7 +3092,7 @@ int dugks_get_df(dugks_engine *h, int32_t n, const int32_t *cells, double *g, do
     std::vector<double> hg((size_t)n * h->nvp), hh((size_t)n * h->nvp);
     hipError_t e = hipMemcpyAsync(dcells, cells, sizeof(int) * n, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
-        ExtractOp op{h, dcells, n, dg, dh};
+        ExtractOp op{h, dcells, n, dg, dh, 0};
         dispatch(h, op);
         e = hipGetLastError();
     }
@@ -3062,6 +3110,40 @@ int dugks_get_df(dugks_engine *h, int32_t n, const int32_t *cells, double *g, do
             if (g) g[(size_t)i * h->nv + rk] = hg[(size_t)i * h->nvp + k];
             if (h_out) h_out[(size_t)i * h->nv + rk] = hh[(size_t)i * h->nvp + k];
         }
+    return 0;
+}
+
+int dugks_get_df_noalloc(dugks_engine *h, int32_t n, const int32_t *cells, double *g, double *h_out)
+{
+    if (!h || !h->stream || !h->initialised) return h ? fail(h, DUGKS_ERR_STATE, "not initialised") : DUGKS_ERR_STATE;
+    if (n < 0 || (n && !cells)) return fail(h, DUGKS_ERR_INVALID, "cells");
+    for (int i = 0; i < n; ++i)
+        if (cells[i] < 0 || cells[i] >= h->nc) return fail(h, DUGKS_ERR_INVALID, "cell index out of range");
+    if (n == 0) return 0;
+    // scratch layout: the cell list (padded to 32 doubles = 256 bytes), then g[n][nv], h[n][nv]
+    const size_t cell_slots = (((size_t)n * sizeof(int32_t) + 255) / 256) * 32;
+    const size_t rows = (size_t)n * h->nv;
+    if (cell_slots + 2 * rows > DF_SCRATCH_DOUBLES)
+        return fail(h, DUGKS_ERR_INVALID,
+                    "get_df_noalloc: n * nv = " + std::to_string(rows) + " exceeds the scratch capacity (2 n nv + " +
+                        std::to_string(cell_slots) + " <= " + std::to_string(DF_SCRATCH_DOUBLES) + " doubles)");
+    if (h->df_scratch_status != hipSuccess || !h->df_scratch)
+        return fail(h, DUGKS_ERR_DEVICE,
+                    std::string("get_df_noalloc: scratch address: ") + hipGetErrorString(h->df_scratch_status));
+    HIPCHK(h, hipSetDevice(h->device));
+    int *dcells = (int *)h->df_scratch;
+    double *dg = h->df_scratch + cell_slots, *dh = dg + rows;
+    hipError_t e = hipMemcpyAsync(dcells, cells, sizeof(int32_t) * n, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        ExtractOp op{h, dcells, n, dg, dh, h->nv};
+        dispatch(h, op);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && g) e = hipMemcpyAsync(g, dg, sizeof(double) * rows, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && h_out)
+        e = hipMemcpyAsync(h_out, dh, sizeof(double) * rows, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(h, DUGKS_ERR_DEVICE, std::string("get_df_noalloc: ") + hipGetErrorString(e));
     return 0;
 }
 

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libdugks.so")
 # exported symbols of include/dugks.h
 SYMBOLS = (
     "dugks_create", "dugks_init_equilibrium", "dugks_step", "dugks_step_cases", "dugks_get_macro",
-    "dugks_get_face_macro", "dugks_get_wall", "dugks_get_courant", "dugks_get_df", "dugks_get_sizes",
+    "dugks_get_face_macro", "dugks_get_wall", "dugks_get_courant", "dugks_get_df", "dugks_get_df_noalloc", "dugks_get_sizes",
     "dugks_state_size", "dugks_save_state", "dugks_load_state", "dugks_last_step_ms", "dugks_destroy",
     "dugks_last_error", "dugks_abi_version", "dugks_set_profiling", "dugks_get_profile",
     "dugks_get_launch", "dugks_set_graphs", "dugks_set_autotune", "dugks_get_cell_launch",
@@ -52,6 +52,7 @@ def load_library(path: str = LIB_PATH):
     L.dugks_get_wall.argtypes = [vp, C.c_int32, dp, dp, dp]
     L.dugks_get_courant.argtypes = [vp, C.c_double, dp, dp]
     L.dugks_get_df.argtypes = [vp, C.c_int32, ip, dp, dp]
+    L.dugks_get_df_noalloc.argtypes = [vp, C.c_int32, ip, dp, dp]
     L.dugks_get_sizes.argtypes = [vp, ip, ip, ip]
     L.dugks_state_size.argtypes = [vp, C.POINTER(C.c_int64)]
     L.dugks_save_state.argtypes = [vp, vp, C.c_int64]
@@ -178,6 +179,15 @@ class Engine:
         g = np.zeros((cells.size, self.nv))
         h = np.zeros((cells.size, self.nv))
         self._chk(load_library().dugks_get_df(self._h, cells.size, iptr(cells), dptr(g), dptr(h)))
+        return g, h
+
+    def df_noalloc(self, cells):
+        """As df(), through dugks_get_df_noalloc (no allocation in the library; the rows pass
+        through its fixed per-GPU scratch array)."""
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        g = np.zeros((cells.size, self.nv))
+        h = np.zeros((cells.size, self.nv))
+        self._chk(load_library().dugks_get_df_noalloc(self._h, cells.size, iptr(cells), dptr(g), dptr(h)))
         return g, h
 
     def save_state(self) -> bytes:

@@ -171,6 +171,18 @@ int dugks_get_courant(dugks_engine* h, double dt, double* co_max, double* co_mea
  * Either output may be NULL. */
 int dugks_get_df(dugks_engine* h, int32_t n, const int32_t* cells, double* g, double* h_out);
 
+/* As dugks_get_df, but allocates nothing: no hipMalloc, hipFree or other device or host
+ * allocation on its success path.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight time loops that dump distributions every
+ * step, harnesses that forbid allocation).  The cell list and the g, h rows pass through a fixed
+ * device array of DUGKS_DF_NOALLOC_MAX_DOUBLES doubles (8 MiB) per GPU, held until the process
+ * ends: a call needs 2 n nv + 32 ceil(n / 64) <= DUGKS_DF_NOALLOC_MAX_DOUBLES (e.g. n <= 37 cells
+ * at nv = 24^3, n <= 311 at nv = 41^2), else DUGKS_ERR_INVALID.  Every call on one GPU uses
+ * the same array: calls that overlap on one GPU (other threads, other engines on that GPU)
+ * race and must be serialised by the caller. */
+#define DUGKS_DF_NOALLOC_MAX_DOUBLES (1u << 20)
+int dugks_get_df_noalloc(dugks_engine* h, int32_t n, const int32_t* cells, double* g, double* h_out);
+
 /* Number of velocities, cells and faces. */
 int dugks_get_sizes(dugks_engine* h, int32_t* nv, int32_t* n_cells, int32_t* n_faces);
 
