@@ -128,7 +128,10 @@ def mass_flow_function(steps: int = 2000, avg_last: int = 500, h: float = 1e-6, 
     """A torch.autograd.Function computing J(a) (forward) and dJ/da (backward), the gradient
     from the discrete adjoint (method "adjoint"; backend(case) as for adjoint_gradient) or central
     differences (method "fd"; backend(case) as for mass_flow).  J and the gradient are returned
-    on the device and in the dtype of the coefficients a."""
+    on the device and in the dtype of the coefficients a.  With a on the GPU, torch's device
+    must be initialised before the engine library first loads (torch.ones(1, device="cuda")),
+    so that both use torch's HIP runtime: a process whose first HIP user is the engine library
+    keeps torch from seeing the device."""
     import torch
 
     if method not in ("adjoint", "fd"):

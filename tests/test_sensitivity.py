@@ -74,21 +74,44 @@ def test_autograd_function_on_cpu(oracle_lib):
     np.testing.assert_allclose(a.grad.numpy(), 3.0 * ga, rtol=1e-15)
 
 
+_GPU_CHILD = r"""
+import sys
+import numpy as np
+import torch
+# torch's HIP runtime first: the engine library then binds to the same runtime (a process whose
+# first HIP user is the engine library keeps torch from seeing the device)
+torch.ones(1, device="cuda")
+sys.path.insert(0, sys.argv[1])
+from dugksfoam_amd import sensitivity
+SMALL = dict(nx=16, ny=6, Lx=1.0, nv=10, kn=0.5)
+a0 = np.array([0.02, -0.01, 0.015])
+fn = sensitivity.mass_flow_function(12, 4, **SMALL)
+a = torch.tensor(a0, dtype=torch.float64, device="cuda", requires_grad=True)
+J = fn.apply(a)
+J.backward()
+assert J.device == a.device and a.grad.device == a.device, (J.device, a.grad.device)
+Ja, ga = sensitivity.adjoint_gradient(a0, 12, 4, **SMALL)
+np.testing.assert_array_equal(a.grad.cpu().numpy(), ga)
+print(repr(float(J.detach())), " ".join(repr(float(x)) for x in a.grad.cpu().numpy()), flush=True)
+"""
+
+
 @pytest.mark.gpu
 def test_autograd_function_on_gpu(oracle_lib):
     """The same with the HIP adjoint and the coefficients on the GPU: J and dJ/da come back on
-    the GPU, backward() equals adjoint_gradient and the oracle's central differences (1e-6)."""
-    import torch
-    a0 = np.array([0.02, -0.01, 0.015])
-    steps, avg = 12, 4
-    fn = sensitivity.mass_flow_function(steps, avg, **SMALL)
-    a = torch.tensor(a0, dtype=torch.float64, device="cuda", requires_grad=True)
-    J = fn.apply(a)
-    J.backward()
-    assert J.device == a.device and a.grad.device == a.device
-    Ja, ga = sensitivity.adjoint_gradient(a0, steps, avg, **SMALL)
-    np.testing.assert_array_equal(a.grad.cpu().numpy(), ga)
-    Jo, go = _oracle_fd(oracle_lib, a0, steps, avg)
-    print("J", float(J), Jo, "grad", a.grad.cpu().numpy(), go, flush=True)
-    assert abs(float(J) - Jo) <= 1e-12 * abs(Jo)
-    assert np.linalg.norm(a.grad.cpu().numpy() - go) <= 1e-6 * np.linalg.norm(go)
+    the GPU, backward() equals adjoint_gradient and the oracle's central differences (1e-6).
+    Runs in a child process that initialises torch's device before the engine library loads,
+    so that torch and the library share one HIP runtime."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", _GPU_CHILD, root], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    vals = [float(x) for x in r.stdout.strip().splitlines()[-1].split()]
+    J, g = vals[0], np.array(vals[1:])
+    Jo, go = _oracle_fd(oracle_lib, np.array([0.02, -0.01, 0.015]), 12, 4)
+    print("J", J, Jo, "grad", g, go, flush=True)
+    assert abs(J - Jo) <= 1e-12 * abs(Jo)
+    assert np.linalg.norm(g - go) <= 1e-6 * np.linalg.norm(go)
