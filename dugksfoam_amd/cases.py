@@ -202,11 +202,15 @@ class Case:
             ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
             xi = np.stack([self.xi[ix], self.xi[iy], self.xi[iz]], axis=1)
             w = self.xi_weights[iz] * self.xi_weights[iy] * self.xi_weights[ix]
-        else:
+        elif self.dim == 2:
             iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
             ix, iy = ix.ravel(), iy.ravel()
             xi = np.stack([self.xi[ix], self.xi[iy], np.zeros(ix.size)], axis=1)
             w = self.xi_weights[iy] * self.xi_weights[ix] * 1
+        else:
+            ix = np.arange(n)
+            xi = np.stack([self.xi[ix], np.zeros(n), np.zeros(n)], axis=1)
+            w = self.xi_weights[ix] * 1 * 1
         return xi, w
 
 
@@ -398,6 +402,59 @@ def cavity(n: int = 60, nv: int = 41, kn: float = 0.075, dim: int = 2, R: float 
     dt = courant_dt(float(n), d, xi_max, co)
     return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
                    meta=dict(kn=kn, u_lid=u_lid, n=n, nv=nv))
+
+
+def line_mesh(n: int, L: float = 1.0, dyz: float = 0.01):
+    """1-D mesh: n cells on [0, L] along x, one cell thick in y and z (both empty directions,
+    as a dugksFoam 1-D case has them).  Returns the box_mesh tuple with dim 1 and only the x
+    sides."""
+    d, C, V, own, nei, Sf, Cf, sides = box_mesh(n, 1, 0, L, dyz, dz2d=dyz)
+    return 1, C, V, own, nei, Sf, Cf, {"x0": sides["x0"], "x1": sides["x1"]}
+
+
+def fourier_flow(n: int = 100, nv: int = 40, kn: float = 0.1, T_left: float = 263.0, T_right: float = 283.0,
+                 R: float = 208.0, T0: float = 273.0, rho0: float = 1.0, xi_sigma: float = 5.0,
+                 Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0, co: float = 0.5) -> Case:
+    """1-D heat conduction between two parallel diffuse plates (Fourier flow): n cells on
+    [0, 1], the plate at x = 0 at T_left and the plate at x = 1 at T_right, the gas at rest at T0
+    to start.  A 1-D dugksFoam case (fvDVM.C:198-215: nv velocities along x)."""
+    d, C, V, own, nei, Sf, Cf, sides = line_mesh(n)
+    walls = [_wall("leftWall", sides["x0"], (0, 0, 0), T_left), _wall("rightWall", sides["x1"], (0, 0, 0), T_right)]
+    geo = _assemble(f"fourier_{n}_{nv}", d, C, V, own, nei, Sf, Cf, walls)
+    xi_max = xi_sigma * math.sqrt(R * T0)
+    xi, w = dv_newton_cotes(xi_max, nv)
+    mu = mu_from_kn(kn, R, T0, rho0)
+    dt = courant_dt(float(n), d, xi_max, co)
+    return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
+                   meta=dict(kn=kn, n=n, nv=nv, T_left=T_left, T_right=T_right))
+
+
+def shock_tube(n: int = 100, nv: int = 40, kn: float = 0.01, rho_ratio: float = 2.0, T_ratio: float = 1.25,
+               R: float = 208.0, T0: float = 273.0, rho0: float = 1.0, xi_sigma: float = 6.0,
+               Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0, co: float = 0.5,
+               ends: str = "zeroGradient") -> Case:
+    """1-D shock tube on [0, 1]: gas at rest, rho0 and T0 * T_ratio for x < 0.5, rho0 / rho_ratio
+    and T0 for x > 0.5; both ends zeroGradient patches (ends="zeroGradient") or diffuse walls at
+    T0 (ends="wall").  The scheme reconstructs with unlimited least-squares gradients (the
+    reference's limiter returns the unlimited gradient, VenkatakrishnanLimitedGrad.C:221-225), so
+    a density ratio of 8 overshoots to a negative face temperature in the first step; 2 runs."""
+    d, C, V, own, nei, Sf, Cf, sides = line_mesh(n)
+    if ends == "zeroGradient":
+        pl = [Patch("left", PATCH_ZEROGRADIENT, sides["x0"]["owner"], sides["x0"]["Sf"], sides["x0"]["Cf"]),
+              Patch("right", PATCH_ZEROGRADIENT, sides["x1"]["owner"], sides["x1"]["Sf"], sides["x1"]["Cf"])]
+    else:
+        pl = [_wall("left", sides["x0"], (0, 0, 0), T0), _wall("right", sides["x1"], (0, 0, 0), T0)]
+    geo = _assemble(f"shocktube_{n}_{nv}_{ends}", d, C, V, own, nei, Sf, Cf, pl)
+    xi_max = xi_sigma * math.sqrt(R * T0 * T_ratio)
+    xi, w = dv_newton_cotes(xi_max, nv)
+    mu = mu_from_kn(kn, R, T0, rho0)
+    dt = courant_dt(float(n), d, xi_max, co)
+    c = _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
+                meta=dict(kn=kn, n=n, nv=nv, ends=ends))
+    left = c.C[:, 0] < 0.5
+    c.rho0 = np.where(left, rho0, rho0 / rho_ratio)
+    c.T0 = np.where(left, T0 * T_ratio, T0)
+    return c
 
 
 def symmetry_cavity(nx: int = 30, ny: int = 60, nv: int = 28, kn: float = 0.075, R: float = 208.0,

@@ -60,7 +60,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     code = DUGKS_ERR_INVALID;
     if (!cs) return "null case";
     if (cs->abi_version != DUGKS_ABI_VERSION) return "abi_version mismatch";
-    if (cs->dim != 2 && cs->dim != 3) return "dim must be 2 or 3";
+    if (cs->dim < 1 || cs->dim > 3) return "dim must be 1, 2 or 3";
     if (cs->n_cells <= 0 || cs->n_internal_faces < 0 || cs->n_faces < cs->n_internal_faces) return "bad mesh sizes";
     if (!cs->owner || (cs->n_internal_faces && !cs->neighbour) || !cs->C || !cs->V || !cs->Sf || !cs->Cf ||
         !cs->weights || (cs->n_faces > cs->n_internal_faces && !cs->delta))
@@ -83,7 +83,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     m.D = cs->dim; m.K = cs->K; m.nc = nc; m.nfi = nfi; m.nf = nf;
     m.R = cs->R; m.omega = cs->omega; m.Tref = cs->T_ref; m.mu = cs->mu_ref; m.Pr = cs->Pr;
     const int n = cs->n_xi;
-    m.nv = m.D == 3 ? n * n * n : n * n;
+    m.nv = m.D == 3 ? n * n * n : (m.D == 2 ? n * n : n);
     std::vector<int> bkind(nb, 0), bpartner(nb, -1);
     m.Tw.assign(nf, 0.0);
     m.Uw.assign(3 * (size_t)nf, 0.0);
@@ -219,25 +219,25 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
     m.wv.assign(m.nv, 0.0);
     const double *X = cs->xi, *W = cs->xi_weights;
     int k = 0;
-    for (int iz = 0; iz < (m.D == 3 ? n : 1); ++iz)
-        for (int iy = 0; iy < n; ++iy)
+    const int ny = m.D >= 2 ? n : 1, nz = m.D == 3 ? n : 1;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy)
             for (int ixx = 0; ixx < n; ++ixx, ++k) {
-                m.wv[k] = m.D == 3 ? W[iz] * W[iy] * W[ixx] : W[iy] * W[ixx] * 1;
+                m.wv[k] = m.D == 3 ? W[iz] * W[iy] * W[ixx] : (m.D == 2 ? W[iy] * W[ixx] * 1 : W[ixx] * 1 * 1);
                 m.xi[3 * k] = X[ixx];
-                m.xi[3 * k + 1] = X[iy];
+                m.xi[3 * k + 1] = m.D >= 2 ? X[iy] : 0.0;
                 m.xi[3 * k + 2] = m.D == 3 ? X[iz] : 0.0;
             }
     // mirrored velocities (symmXtgID / symmYtgID / symmZtgID, fvDVM.C:152-166, 183-195): the 1-D
     // set is symmetric, so index i mirrors to n - 1 - i
     m.mir.assign(3 * (size_t)m.nv, 0);
-    const int nz = m.D == 3 ? n : 1;
     for (int iz = 0; iz < nz; ++iz)
-        for (int iy = 0; iy < n; ++iy)
+        for (int iy = 0; iy < ny; ++iy)
             for (int ixx = 0; ixx < n; ++ixx) {
-                const int kk = (iz * n + iy) * n + ixx;
-                m.mir[kk] = (iz * n + iy) * n + (n - 1 - ixx);
-                m.mir[m.nv + kk] = (iz * n + (n - 1 - iy)) * n + ixx;
-                m.mir[2 * m.nv + kk] = ((nz - 1 - iz) * n + iy) * n + ixx;
+                const int kk = (iz * ny + iy) * n + ixx;
+                m.mir[kk] = (iz * ny + iy) * n + (n - 1 - ixx);
+                m.mir[m.nv + kk] = (iz * ny + (ny - 1 - iy)) * n + ixx;
+                m.mir[2 * m.nv + kk] = ((nz - 1 - iz) * ny + iy) * n + ixx;
             }
     code = DUGKS_OK;
     return nullptr;

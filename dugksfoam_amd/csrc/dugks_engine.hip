@@ -289,7 +289,8 @@ DEVI double row_off(const P<TS> &p, const Vel &v, int which, const double *vt)
 {
     if (sizeof(TS) == 8) return 0.0;
     const double *e = vt + 2 * p.nax;
-    double m = e[v.ix] * e[p.nxi + v.iy];
+    double m = e[v.ix];
+    if (D >= 2) m *= e[p.nxi + v.iy];
     if (D == 3) m *= e[2 * p.nxi + v.iz];
     return m * vt[3 * p.nax + which];
 }
@@ -305,8 +306,8 @@ DEVI void stage_vt(const P<TS> &p, double *vt)
         vt[t] = t < p.nax ? p.axis[t] : (t < 2 * p.nax ? p.axw[t - p.nax] : p.moff[t - 2 * p.nax]);
 }
 
-// Velocity k: w = (w_z w_y) w_x and |xi|^2 = (x x + y y) + z z in the order the velocity set is
-// built (fvDVM.C:183-195), without contraction.
+// Velocity k: w = (w_z w_y) w_x (2-D: (w_y w_x) 1, 1-D: (w_x 1) 1) and |xi|^2 = (x x + y y) + z z in
+// the order the velocity set is built (fvDVM.C:140-221), without contraction.
 template <int D, typename TS>
 DEVI Vel load_vel(const P<TS> &p, unsigned k, const double *vt)
 {
@@ -318,10 +319,10 @@ DEVI Vel load_vel(const P<TS> &p, unsigned k, const double *vt)
     v.iy = (id >> 8) & 0xff;
     v.iz = (id >> 16) & 0xff;
     v.x = vt[v.ix];
-    v.y = vt[nxi + v.iy];
+    v.y = D >= 2 ? vt[nxi + v.iy] : 0.0;
     v.z = D == 3 ? vt[2 * nxi + v.iz] : 0.0;
-    const double wx = vt[nax + v.ix], wy = vt[nax + nxi + v.iy];
-    v.w = D == 3 ? (vt[nax + 2 * nxi + v.iz] * wy) * wx : (wy * wx) * 1.0;
+    const double wx = vt[nax + v.ix], wy = D >= 2 ? vt[nax + nxi + v.iy] : 1.0;
+    v.w = D == 3 ? (vt[nax + 2 * nxi + v.iz] * wy) * wx : (D == 2 ? (wy * wx) * 1.0 : (wx * 1.0) * 1.0);
     if (id & VPAD) v.w = 0.0;
     v.xi2 = (v.x * v.x + v.y * v.y) + v.z * v.z;
     return v;
@@ -336,7 +337,7 @@ DEVI void eq_consts(double rho, const double *U, double T, const double *q, doub
     e.RT = R * T;
     e.invRT = 1.0 / e.RT;
     double tp = 2.0 * PI * e.RT;
-    e.pre = (D == 3) ? rho / (tp * sqrt(tp)) : rho / tp;
+    e.pre = (D == 3) ? rho / (tp * sqrt(tp)) : (D == 2 ? rho / tp : rho / sqrt(tp));
     double den = 5.0 * rho * R * T * R * T;
     double f = (1.0 - Pr) / den;
     e.qc[0] = f * q[0];
@@ -353,7 +354,8 @@ DEVI void eq_eval(const EqC &e, const double *tab, int nxi, const Vel &v, double
     double cx = e.U[0] - v.x, cy = e.U[1] - v.y, cz = e.U[2] - v.z;
     double c2 = (cx * cx + cy * cy + cz * cz) * e.invRT;
     double pc = -(cx * e.qc[0] + cy * e.qc[1] + cz * e.qc[2]);
-    double gB = e.pre * tab[v.ix] * tab[nxi + v.iy];
+    double gB = e.pre * tab[v.ix];
+    if (D >= 2) gB *= tab[nxi + v.iy];
     if (D == 3) gB *= tab[2 * nxi + v.iz];
     geq = (1.0 + pc * ((c2 - D) - 2.0)) * gB;
     if (H) heq = (kc + pc * ((c2 - D) * kc - 2.0 * K)) * gB * e.RT;
@@ -369,7 +371,8 @@ DEVI void face_relax(const FE &fe, const double *tf, int xo, int yo, int zo, con
                      double &gf, double &hf)
 {
 #pragma clang fp contract(on)
-    double rgB = tf[xo] * tf[yo];
+    double rgB = tf[xo];
+    if (D >= 2) rgB *= tf[yo];
     if (D == 3) rgB *= tf[zo];
     const double P = v.xi2 * fe.invRT + fe.C0 + fe.B[0] * v.x + fe.B[1] * v.y + fe.B[2] * v.z;
     const double pc = fe.Q0 + fe.qc[0] * v.x + fe.qc[1] * v.y + fe.qc[2] * v.z;
@@ -465,7 +468,8 @@ DEVI double s1_value(double gv, double og, double r, double geq)
 template <int D>
 DEVI double dotg(const double *G, const double *dC, const double *hx)
 {
-    double r = G[0] * (dC[0] - hx[0]) + G[1] * (dC[1] - hx[1]);
+    double r = G[0] * (dC[0] - hx[0]);
+    if (D >= 2) r += G[1] * (dC[1] - hx[1]);
     if (D == 3) r += G[2] * (dC[2] - hx[2]);
     return r;
 }
@@ -1340,7 +1344,8 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(FV ? CELL
                 // S4 incoming Maxwellian rho_w M_w, left unrelaxed by S7 (maxwellWall operators
                 // are no-ops); the x entries of the table carry rho_w pre_w
                 const double *tw = tabW + j;
-                gf = tw[xo] * tw[yo];
+                gf = tw[xo];
+                if (D >= 2) gf *= tw[yo];
                 if (D == 3) gf *= tw[zo];
                 if (H) hf = gf * (p.R * fe.pad[0]) * p.kc;
             } else {
@@ -1480,7 +1485,7 @@ __global__ __launch_bounds__(256) void k_pressure_bc(P<TS> p)
         W.rho_b = rhoi + (pr - rhoi * p.R * Ti) / ai / ai;
         W.T = pr / (p.R * rhoi);
         const double tp = 2.0 * PI * (p.R * W.T);
-        W.pre = D == 3 ? 1.0 / (tp * sqrt(tp)) : 1.0 / tp;
+        W.pre = D == 3 ? 1.0 / (tp * sqrt(tp)) : (D == 2 ? 1.0 / tp : 1.0 / sqrt(tp));
     }
 }
 
@@ -1649,7 +1654,8 @@ __global__ __launch_bounds__(256) void k_wall_init(P<TS> p, const int *wall_face
     for (int k = threadIdx.x; k < p.nvp; k += blockDim.x) {
         const Vel v = load_vel<D>(p, k, vt);
         const double s = dotnc(v.x, v.y, v.z, Sf);
-        double m = W.pre * tab[v.ix] * tab[p.nxi + v.iy];
+        double m = W.pre * tab[v.ix];
+        if (D >= 2) m *= tab[p.nxi + v.iy];
         if (D == 3) m *= tab[2 * p.nxi + v.iz];
         if (s < 0) acc[NM] = fma(-v.w * s, m, acc[NM]);
         if (s <= 0) acc_moments<D, true>(acc, m, m * RT * p.kc, v, Uw);
@@ -1957,12 +1963,17 @@ void launch_step_ns(dugks_engine *h, hipEvent_t *ev)
     else launch_step_fv<D, H, NS, TS, false>(h, ev);
 }
 
-// stencil sizes: 4 (triangles, quads) or 8 in 2-D, 6 (hexahedra, tetrahedra) or 8 in 3-D
+// stencil sizes: 4 in 1-D (2 faces per cell), 4 (triangles, quads) or 8 in 2-D, 6 (hexahedra,
+// tetrahedra) or 8 in 3-D
 template <int D, bool H, typename TS>
 void launch_step_t(dugks_engine *h, hipEvent_t *ev)
 {
-    if (h->ns_max <= (D == 2 ? 4 : 6)) launch_step_ns<D, H, D == 2 ? 4 : 6, TS>(h, ev);
-    else launch_step_ns<D, H, MAXS, TS>(h, ev);
+    if constexpr (D == 1) {
+        launch_step_ns<D, H, 4, TS>(h, ev);
+    } else {
+        if (h->ns_max <= (D == 2 ? 4 : 6)) launch_step_ns<D, H, D == 2 ? 4 : 6, TS>(h, ev);
+        else launch_step_ns<D, H, MAXS, TS>(h, ev);
+    }
 }
 
 template <typename F>
@@ -1975,9 +1986,12 @@ void dispatch(dugks_engine *h, F &&f)
         } else {
             if (fp32) f.template operator()<3, false, float>(); else f.template operator()<3, false, double>();
         }
-    } else {
+    } else if (h->D == 2) {
         // in 2-D, K + 3 - D >= 1: h is always carried
         if (fp32) f.template operator()<2, true, float>(); else f.template operator()<2, true, double>();
+    } else {
+        // 1-D (fvDVM.C:198-215): K + 3 - D >= 2, h carried
+        if (fp32) f.template operator()<1, true, float>(); else f.template operator()<1, true, double>();
     }
 }
 
@@ -2184,7 +2198,7 @@ int build(dugks_engine *h, const dugks_case *cs)
     h->mu_ref = cs->mu_ref; h->Pr = cs->Pr; h->xi_max = cs->xi_max; h->storage = cs->storage;
     h->n_cases = cs->n_cases > 1 ? cs->n_cases : 1;
     h->has_h = (cs->K + 3 - cs->dim) != 0;
-    h->nv = cs->dim == 3 ? cs->n_xi * cs->n_xi * cs->n_xi : cs->n_xi * cs->n_xi;
+    h->nv = cs->dim == 3 ? cs->n_xi * cs->n_xi * cs->n_xi : (cs->dim == 2 ? cs->n_xi * cs->n_xi : cs->n_xi);
     h->nax = cs->dim * cs->n_xi;
     const int D = h->D;
 
@@ -2322,6 +2336,8 @@ int build(dugks_engine *h, const dugks_case *cs)
         h->ncell_slots[c] = ns;
         if (ns > MAXS)
             return fail(h, DUGKS_ERR_UNSUPPORTED, "cell " + std::to_string(c) + " has more than 8 faces");
+        if (D == 1 && ns > 2)
+            return fail(h, DUGKS_ERR_INVALID, "1-D mesh: cell " + std::to_string(c) + " has more than 2 faces");
         if (ns == 0) return fail(h, DUGKS_ERR_INVALID, "cell without faces");
         for (int e = cnt[c]; e < cnt[c + 1]; ++e) {
             const int f = cface[e];
@@ -2418,13 +2434,19 @@ int build(dugks_engine *h, const dugks_case *cs)
                         rid[i] = ix | (iy << 8) | (iz << 16);
                         ++i;
                     }
-        } else {
+        } else if (D == 2) {
             for (int iy = 0; iy < nxi; ++iy)
                 for (int ix = 0; ix < nxi; ++ix) {
                     rx[i] = X[ix]; ry[i] = X[iy]; rz[i] = 0.0;
                     rid[i] = ix | (iy << 8);
                     ++i;
                 }
+        } else {
+            for (int ix = 0; ix < nxi; ++ix) {
+                rx[i] = X[ix]; ry[i] = 0.0; rz[i] = 0.0;
+                rid[i] = ix;
+                ++i;
+            }
         }
     }
     auto cls = [](double x) { return x < 0 ? 0 : (x == 0 ? 1 : 2); };
@@ -2488,11 +2510,11 @@ int build(dugks_engine *h, const dugks_case *cs)
             for (int a = 0; a < 3; ++a) mir[(size_t)a * nvp + k] = (uint32_t)k;
         for (int k = 0; k < nv; ++k) {
             const int r = h->perm[k];
-            const int ix = r % nxi, iy = (r / nxi) % nxi, iz = D == 3 ? r / (nxi * nxi) : 0;
-            const int nz = D == 3 ? nxi : 1;
-            mir[k] = (uint32_t)inv[(iz * nxi + iy) * nxi + (nxi - 1 - ix)];
-            mir[(size_t)nvp + k] = (uint32_t)inv[(iz * nxi + (nxi - 1 - iy)) * nxi + ix];
-            mir[2 * (size_t)nvp + k] = (uint32_t)inv[((nz - 1 - iz) * nxi + iy) * nxi + ix];
+            const int ix = r % nxi, iy = D >= 2 ? (r / nxi) % nxi : 0, iz = D == 3 ? r / (nxi * nxi) : 0;
+            const int ny = D >= 2 ? nxi : 1, nz = D == 3 ? nxi : 1;
+            mir[k] = (uint32_t)inv[(iz * ny + iy) * nxi + (nxi - 1 - ix)];
+            mir[(size_t)nvp + k] = (uint32_t)inv[(iz * ny + (ny - 1 - iy)) * nxi + ix];
+            mir[2 * (size_t)nvp + k] = (uint32_t)inv[((nz - 1 - iz) * ny + iy) * nxi + ix];
         }
     }
     std::vector<double> axis(h->nax), axw(h->nax);
@@ -2542,7 +2564,7 @@ int build(dugks_engine *h, const dugks_case *cs)
         for (int d = 0; d < 3; ++d) W.U[d] = Ub[3 * b + d];
         W.T = Tb[b];
         double tp = 2.0 * PI * (cs->R * Tb[b]);
-        W.pre = D == 3 ? 1.0 / (tp * std::sqrt(tp)) : 1.0 / tp;
+        W.pre = D == 3 ? 1.0 / (tp * std::sqrt(tp)) : (D == 2 ? 1.0 / tp : 1.0 / std::sqrt(tp));
         W.rho_b = rhob[b];
         W.mode = bkind[b] == DUGKS_PATCH_FIXED ? 1.0
                : bkind[b] == DUGKS_PATCH_PRESSURE_IN ? 2.0 : bkind[b] == DUGKS_PATCH_PRESSURE_OUT ? 3.0 : 0.0;
@@ -2675,7 +2697,7 @@ int upload_offsets(dugks_engine *h)
     std::vector<double> m((size_t)h->nax + 2, 0.0);
     if (h->storage == DUGKS_STORAGE_FP32) {
         const double RT = h->R * h->off_T, tp = 2.0 * PI * RT;
-        const double pre = h->off_rho / (h->D == 3 ? tp * std::sqrt(tp) : tp);
+        const double pre = h->off_rho / (h->D == 3 ? tp * std::sqrt(tp) : (h->D == 2 ? tp : std::sqrt(tp)));
         std::vector<double> ax((size_t)h->nax);
         HIPCHK(h, hipMemcpy(ax.data(), h->axis, sizeof(double) * h->nax, hipMemcpyDeviceToHost));
         for (int t = 0; t < h->nax; ++t) m[t] = std::exp(-(ax[t] * ax[t]) / (2.0 * RT));
@@ -2862,7 +2884,7 @@ int dugks_create(const dugks_case *cs, dugks_engine **out)
     if (!h) return DUGKS_ERR_MEMORY;
     int r = 0;
     if (cs->abi_version != DUGKS_ABI_VERSION) r = fail(h, DUGKS_ERR_INVALID, "abi_version mismatch");
-    else if (cs->dim != 2 && cs->dim != 3) r = fail(h, DUGKS_ERR_UNSUPPORTED, "dim must be 2 or 3");
+    else if (cs->dim < 1 || cs->dim > 3) r = fail(h, DUGKS_ERR_UNSUPPORTED, "dim must be 1, 2 or 3");
     else if (cs->n_cells <= 0 || cs->n_internal_faces < 0 || cs->n_faces < cs->n_internal_faces)
         r = fail(h, DUGKS_ERR_INVALID, "mesh sizes");
     else if (!cs->owner || (cs->n_internal_faces && !cs->neighbour) || !cs->C || !cs->V || !cs->Sf || !cs->Cf ||

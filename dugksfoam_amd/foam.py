@@ -285,6 +285,8 @@ def read_case(path: str, dt: float = None) -> cases.Case:
     dim = 3 - len(empty_dirs)
     if dim == 2 and empty_dirs != {2}:
         raise NotImplementedError("2-D cases must be empty in z (the velocity set has xi_z = 0)")
+    if dim == 1 and empty_dirs != {1, 2}:
+        raise NotImplementedError("1-D cases must be empty in y and z (the velocity set has xi_y = xi_z = 0)")
 
     sizes = {b["name"]: b["nFaces"] for b in bnd}
     rho_i, rho_b = read_field(os.path.join(path, "0", "rho"), n_cells, sizes, vector=False)

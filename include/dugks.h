@@ -20,7 +20,8 @@
  *  - Discrete velocities are the tensor product of the 1-D set in
  *    constant/Xis + constant/weights, ordered k = (iz*nY + iy)*nX + ix with
  *    xi = (Xis[ix], Xis[iy], Xis[iz]) and w = w[iz]*w[iy]*w[ix]
- *    (fvDVM.C:140-221; z components are 0 in 2-D).
+ *    (fvDVM.C:140-221; z components are 0 in 2-D, y and z components in 1-D,
+ *    where nv = nDV and w = w[ix]).
  */
 #ifndef DUGKS_H
 #define DUGKS_H
@@ -99,7 +100,7 @@ typedef struct dugks_patch {
 
 typedef struct dugks_case {
     int32_t abi_version;       /* = DUGKS_ABI_VERSION                                */
-    int32_t dim;               /* mesh.nSolutionD(): 2 or 3                          */
+    int32_t dim;               /* mesh.nSolutionD(): 1, 2 or 3                       */
     int32_t n_cells;
     int32_t n_internal_faces;
     int32_t n_faces;           /* internal + faces of non-empty patches              */

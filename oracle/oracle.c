@@ -263,7 +263,7 @@ oracle *oracle_create(const dugks_case *cs)
     oracle *o = (oracle *)xmalloc(sizeof(oracle));
     o->dim = cs->dim; o->nc = cs->n_cells; o->nfi = cs->n_internal_faces; o->nf = cs->n_faces;
     o->nb = o->nf - o->nfi; o->nxi = cs->n_xi;
-    o->nv = cs->dim == 3 ? cs->n_xi*cs->n_xi*cs->n_xi : cs->n_xi*cs->n_xi;
+    o->nv = cs->dim == 3 ? cs->n_xi*cs->n_xi*cs->n_xi : (cs->dim == 2 ? cs->n_xi*cs->n_xi : cs->n_xi);
     o->R = cs->R; o->omega = cs->omega; o->Tref = cs->T_ref; o->mu_ref = cs->mu_ref; o->Pr = cs->Pr;
     o->K = cs->K; o->xi_max = cs->xi_max;
     o->nthreads = 1;
@@ -355,21 +355,27 @@ oracle *oracle_create(const dugks_case *cs)
                 o->xi[3*i] = X[ix]; o->xi[3*i + 1] = X[iy]; o->xi[3*i + 2] = X[iz];
                 ++i;
             }
-        } else {
+        } else if (o->dim == 2) {
             for (int iy = 0; iy < n; ++iy) for (int ix = 0; ix < n; ++ix) {
                 o->wv[i] = W[iy]*W[ix]*1;
                 o->xi[3*i] = X[ix]; o->xi[3*i + 1] = X[iy]; o->xi[3*i + 2] = 0.0;
+                ++i;
+            }
+        } else {   /* 1-D (fvDVM.C:198-215) */
+            for (int ix = 0; ix < n; ++ix) {
+                o->wv[i] = W[ix]*1*1;
+                o->xi[3*i] = X[ix]; o->xi[3*i + 1] = 0.0; o->xi[3*i + 2] = 0.0;
                 ++i;
             }
         }
         /* mirror ids symmXtgID, symmYtgID, symmZtgID (fvDVM.C:152-166, 183-195) */
         o->mirror = (int *)xmalloc(sizeof(int)*3*nv);
         for (int k = 0; k < nv; ++k) {
-            int ix = k % n, iy = (k / n) % n, iz = o->dim == 3 ? k / (n*n) : 0;
-            int nz = o->dim == 3 ? n : 1;
-            o->mirror[k] = (iz*n + iy)*n + (n - 1 - ix);
-            o->mirror[nv + k] = (iz*n + (n - 1 - iy))*n + ix;
-            o->mirror[2*nv + k] = ((nz - 1 - iz)*n + iy)*n + ix;
+            int ix = k % n, iy = o->dim >= 2 ? (k / n) % n : 0, iz = o->dim == 3 ? k / (n*n) : 0;
+            int ny = o->dim >= 2 ? n : 1, nz = o->dim == 3 ? n : 1;
+            o->mirror[k] = (iz*ny + iy)*n + (n - 1 - ix);
+            o->mirror[nv + k] = (iz*ny + (ny - 1 - iy))*n + ix;
+            o->mirror[2*nv + k] = ((nz - 1 - iz)*ny + iy)*n + ix;
         }
     }
     o->n_cases = cs->n_cases > 1 ? cs->n_cases : 1;

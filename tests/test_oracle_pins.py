@@ -471,3 +471,73 @@ def test_symmetry_plane_step_closed_form(oracle_lib):
     assert np.abs(got - rule).max() <= 1e-13 * scale, np.abs(got - rule).max() / scale
     for wrong in (predict(1, 0, True), predict(1, 1, True), predict(2, 1, False)):
         assert np.abs(got - wrong).max() > 1e-6 * scale
+
+
+# ---------------------------------------------------------------- one-dimensional cases
+
+@pytest.fixture(scope="module")
+def fine1d(oracle_lib):
+    c = cases.fourier_flow(4, nv=120, xi_sigma=8.0)
+    return _oracle(c), c
+
+
+def test_shakhov_discrete_moments_1d(fine1d):
+    """1-D velocity set (fvDVM.C:198-215: xi = (X[ix], 0, 0), w = W[ix]): the Shakhov equilibrium
+    with the 1-D normalisation (2 pi R T)^(-1/2) reproduces rho, rho U, rho E (h carries the
+    K + 2 other degrees of freedom) and the heat flux (1 - Pr) q.  The 2-D normalisation or
+    h factor fails these."""
+    o, c = fine1d
+    xi, w = o.velocities()
+    assert xi.shape == (120, 3) and np.all(xi[:, 1:] == 0.0)
+    np.testing.assert_array_equal(w, c.xi_weights)
+    rho, T, U, q = 0.9, 290.0, np.array([35.0, 0.0, 0.0]), np.array([2.0e4, 0.0, 0.0])
+    g = np.empty(len(w))
+    h = np.empty(len(w))
+    for k in range(len(w)):
+        g[k], h[k] = o.shakhov(rho, U, T, q, xi[k])
+    assert np.sum(w * g) == pytest.approx(rho, rel=1e-9)
+    assert np.sum(w * g * xi[:, 0]) == pytest.approx(rho * U[0], rel=1e-9)
+    E = 0.5 * np.sum(w * (g * xi[:, 0] ** 2 + h))
+    assert E == pytest.approx(rho * (0.5 * U @ U + (c.K + 3) / 2 * c.R * T), rel=1e-9)
+    cx = xi[:, 0] - U[0]
+    qx = 0.5 * np.sum(w * cx * (cx ** 2 * g + h))
+    assert qx == pytest.approx((1 - c.Pr) * q[0], rel=1e-7)
+
+
+def test_lsq_exact_for_linear_fields_1d(oracle_lib):
+    """Least squares on a 1-D mesh (two empty directions, both removed from the tensor before
+    its inversion): exact for a linear field, zero in y and z."""
+    c = cases.fourier_flow(9, nv=4)
+    o = _oracle(c)
+    a = np.array([2.5, 0.0, 0.0])
+    phi = 0.7 + c.C @ a
+    nfi = c.n_internal_faces
+    phib = 0.7 + (c.C[c.owner[nfi:]] + c.delta) @ a
+    g = o.lsq_grad(phi, phib)
+    np.testing.assert_allclose(g[:, 0], 2.5, rtol=1e-12)
+    assert np.all(g[:, 1:] == 0.0)
+
+
+def test_fourier_flow_equilibrium_and_heat_flux(oracle_lib):
+    """Two plates at the gas temperature keep a gas at rest in equilibrium unchanged (the 1-D
+    wall Maxwellian and inComingByRho balance); with the plates at 263 K and 283 K heat flows
+    from the hot plate to the cold one (q_x < 0 everywhere) and, as the flow settles, the heat
+    flux becomes uniform across the gap (energy conservation with no work: dq/dx = 0)."""
+    # velocity range +-8 sqrt(R T): the truncated Maxwellian's moments are exact to ~1e-14, so
+    # rest is a fixed point to round-off (at +-5 sqrt(R T) the truncation moves rho by ~1e-6)
+    c = cases.fourier_flow(10, nv=40, T_left=273.0, T_right=273.0, xi_sigma=8.0)
+    o = _oracle(c)
+    o.init_equilibrium()
+    o.step(nsteps=20)
+    m = o.macro()
+    assert np.abs(m["rho"] - 1.0).max() < 1e-13 and np.abs(m["T"] / 273.0 - 1.0).max() < 1e-12
+    assert np.abs(m["U"]).max() < 1e-10 and np.abs(m["q"]).max() < 1e-5
+    c = cases.fourier_flow(20, nv=24, kn=0.5)
+    o = _oracle(c, threads=4)
+    o.init_equilibrium()
+    o.step(nsteps=1000)
+    m = o.macro()
+    q = m["q"][:, 0]
+    assert np.all(q < 0)
+    assert (q.max() - q.min()) < 0.01 * abs(q.mean()), (q.min(), q.max())
+    assert np.all(np.diff(m["T"]) > 0)
