@@ -457,6 +457,54 @@ def shock_tube(n: int = 100, nv: int = 40, kn: float = 0.01, rho_ratio: float = 
     return c
 
 
+def agglomerated_cavity(n: int = 12, nv: int = 12, block: int = 3, kn: float = 0.075, R: float = 208.0,
+                        T0: float = 273.0, rho0: float = 1.0, xi_sigma: float = 4.0, lid_mach: float = 0.16,
+                        Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0, co: float = 0.5) -> Case:
+    """2-D lid-driven cavity on a general polygonal mesh: an n x n quad grid whose block x block
+    groups of quads, on a checkerboard of blocks, are merged into single cells (polygons with
+    4 block faces, several of them shared with one neighbour) beside plain quads.  Faces are the
+    grid's edges between different cells, numbered in OpenFOAM's upper-triangular order
+    (owner < neighbour, sorted by owner, then neighbour).  Walls and lid as in cavity()."""
+    d, C4, V4, own4, nei4, Sf4, Cf4, sides = box_mesh(n, n)
+    nb_ = (n + block - 1) // block
+    qi, qj = np.arange(n * n) % n, np.arange(n * n) // n
+    merged = ((qi // block + qj // block) % 2 == 0) & ((qi // block + 1) * block <= n) & ((qj // block + 1) * block <= n)
+    key = np.where(merged, -1 - ((qj // block) * nb_ + qi // block), np.arange(n * n))
+    cell_of = np.empty(n * n, dtype=np.int64)
+    ids = {}
+    for q in range(n * n):           # cells numbered in the order of their first quad
+        k = int(key[q])
+        if k not in ids:
+            ids[k] = len(ids)
+        cell_of[q] = ids[k]
+    nc = len(ids)
+    V = np.bincount(cell_of, weights=V4, minlength=nc)
+    C = np.stack([np.bincount(cell_of, weights=V4 * C4[:, a], minlength=nc) for a in range(3)], axis=1) / V[:, None]
+    a, b = cell_of[own4], cell_of[nei4]
+    keep = a != b
+    a, b, Sf, Cf = a[keep], b[keep], Sf4[keep].copy(), Cf4[keep]
+    flip = a > b
+    a, b = np.where(flip, b, a), np.where(flip, a, b)
+    Sf[flip] *= -1.0
+    order = np.lexsort((np.arange(a.size), b, a))
+    own, nei, Sf, Cf = a[order], b[order], Sf[order], Cf[order]
+
+    def remap(sd):
+        return dict(owner=cell_of[sd["owner"]], Sf=sd["Sf"], Cf=sd["Cf"])
+
+    walls = [_wall("movingWall", remap(sides["y1"]), (0, 0, 0), T0),
+             _wall("fixedWalls", remap(_concat_sides([sides["x0"], sides["x1"], sides["y0"]])), (0, 0, 0), T0)]
+    u_lid = lid_mach * math.sqrt(GAMMA_MONATOMIC * R * T0)
+    walls[0].U[:, 0] = u_lid
+    geo = _assemble(f"agglomerated_{n}_{block}_{nv}", d, C, V, own, nei, Sf, Cf, walls)
+    xi_max = xi_sigma * math.sqrt(R * T0)
+    xi, w = dv_newton_cotes(xi_max, nv)
+    mu = mu_from_kn(kn, R, T0, rho0)
+    dt = courant_dt(float(n), d, xi_max, co)
+    return _finish(geo, xi, w, xi_max, R, omega, T0, mu, Pr, K, rho0, T0, dt,
+                   meta=dict(kn=kn, u_lid=u_lid, n=n, nv=nv, block=block))
+
+
 def symmetry_cavity(nx: int = 30, ny: int = 60, nv: int = 28, kn: float = 0.075, R: float = 208.0,
                     T0: float = 273.0, rho0: float = 1.0, xi_sigma: float = 4.0, lid_mach: float = 0.16,
                     Pr: float = 2.0 / 3.0, omega: float = 0.5, K: int = 0, co: float = 0.5,

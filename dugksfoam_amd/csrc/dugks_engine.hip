@@ -71,7 +71,7 @@ __host__ __device__ constexpr inline bool grad_row(int kind)
 {
     return kind == KIND_WALL || kind == KIND_SYM || kind == KIND_FAR || kind == KIND_ZG;
 }
-constexpr int MAXS = 8;       // faces per cell handled by k_cell
+constexpr int MAXS = 16;      // faces per cell (stencil templates NS = 4, 6, 8, 16)
 constexpr int NAXMAX = 192;   // per-axis exp table entries (3 axes x 64 points)
 constexpr int MAXNT = 512;     // workgroup size cap (256 VGPRs per lane available)
 constexpr int MAXWAVES = MAXNT / 64;
@@ -168,7 +168,7 @@ struct alignas(64) CellSt {
     int32_t wmask;        // bit j set: entry j is a wall face
     int32_t pad[15];
 };
-static_assert(sizeof(CellSt) == 384, "cellst size");
+static_assert(sizeof(CellSt) % 64 == 0, "cellst size");
 
 // Per-face data of k_face.
 struct alignas(16) FaceX {
@@ -1739,8 +1739,8 @@ struct dugks_engine {
     // online launch-shape tuning: candidate (nt, nib) per kernel, all tiling the same nvp
     std::vector<std::pair<int, int>> face_cand, cell_cand;
     int tuning = 0, tune_i = -1;
-    double tune_best_face = 0.0, tune_best_cell = 0.0;
-    int tune_face = 0, tune_cell = 0;
+    std::vector<double> tune_tf, tune_tc;   // per candidate: the shortest time measured
+    static constexpr int TUNE_ROUNDS = 2;   // every candidate is timed on this many steps
     int ns_max = 0;      // largest face count of a cell
     double R = 0, omega = 0, Tref = 0, mu_ref = 0, Pr = 0, xi_max = 0;
     std::vector<int> perm;          // device velocity -> reference velocity, -1 for padding
@@ -1888,7 +1888,9 @@ P<TS> params(dugks_engine *h)
 // ---- launch dispatch over (D, H, TS, NIT) ----
 
 
-size_t cell_lds(const dugks_engine *h) { return sizeof(double) * ((4 + 2 * MAXS) * (size_t)h->nax + 2); }
+// dynamic LDS of k_cell with stencil template NS: face tables (2 NS nax), velocity table
+// (3 nax + 2), tabC (nax)
+size_t cell_lds(const dugks_engine *h, int ns) { return sizeof(double) * ((4 + 2 * (size_t)ns) * h->nax + 2); }
 
 dim3 block_grid(int n, int nb) { return dim3((unsigned)(((n + NXCD - 1) / NXCD) * NXCD), (unsigned)nb); }
 
@@ -1936,7 +1938,7 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
         if (h->fuse_s1 && h->nvbc == 1) {
             // k_cell with the cell macros and the next step's S1 in its tail (k_cell_fin and
             // k_cell_s1 fused; their profile slots stay empty)
-            hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV, true>), block_grid(h->nc, 1), dim3(h->ntc), cell_lds(h),
+            hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV, true>), block_grid(h->nc, 1), dim3(h->ntc), cell_lds(h, NS),
                                h->stream, p);
             if (ev) (void)hipEventRecord(ev[4], h->stream);
             if (h->npress)
@@ -1946,7 +1948,7 @@ void launch_step_fv(dugks_engine *h, hipEvent_t *ev)
             return;
         }
     }
-    hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h), h->stream, p);
+    hipLaunchKernelGGL((k_cell<D, H, NS, TS, FV>), block_grid(h->nc, h->nvbc), dim3(h->ntc), cell_lds(h, NS), h->stream, p);
     if (ev) (void)hipEventRecord(ev[4], h->stream);
     hipLaunchKernelGGL((k_cell_fin<D, TS>), dim3((h->nc + 255) / 256), dim3(256), 0, h->stream, p);
     if (h->npress)
@@ -1964,7 +1966,7 @@ void launch_step_ns(dugks_engine *h, hipEvent_t *ev)
 }
 
 // stencil sizes: 4 in 1-D (2 faces per cell), 4 (triangles, quads) or 8 in 2-D, 6 (hexahedra,
-// tetrahedra) or 8 in 3-D
+// tetrahedra) or 8 in 3-D, and 16 for general polygons and polyhedra
 template <int D, bool H, typename TS>
 void launch_step_t(dugks_engine *h, hipEvent_t *ev)
 {
@@ -1972,6 +1974,7 @@ void launch_step_t(dugks_engine *h, hipEvent_t *ev)
         launch_step_ns<D, H, 4, TS>(h, ev);
     } else {
         if (h->ns_max <= (D == 2 ? 4 : 6)) launch_step_ns<D, H, D == 2 ? 4 : 6, TS>(h, ev);
+        else if (h->ns_max <= 8) launch_step_ns<D, H, 8, TS>(h, ev);
         else launch_step_ns<D, H, MAXS, TS>(h, ev);
     }
 }
@@ -2160,7 +2163,7 @@ void shape_candidates(dugks_engine *h)
     for (int nb = 1; nb <= 16; ++nb) {
         if (h->nvp % nb) continue;
         const int vb = h->nvp / nb;
-        for (int t = 128; t <= MAXNT; t += 128) {
+        for (int t = 64; t <= MAXNT; t += 64) {
             if (vb % t || vb / t > NIBMAX) continue;
             h->face_cand.emplace_back(t, vb / t);
         }
@@ -2335,7 +2338,7 @@ int build(dugks_engine *h, const dugks_case *cs)
         int ns = cnt[c + 1] - cnt[c];
         h->ncell_slots[c] = ns;
         if (ns > MAXS)
-            return fail(h, DUGKS_ERR_UNSUPPORTED, "cell " + std::to_string(c) + " has more than 8 faces");
+            return fail(h, DUGKS_ERR_UNSUPPORTED, "cell " + std::to_string(c) + " has more than 16 faces");
         if (D == 1 && ns > 2)
             return fail(h, DUGKS_ERR_INVALID, "1-D mesh: cell " + std::to_string(c) + " has more than 2 faces");
         if (ns == 0) return fail(h, DUGKS_ERR_INVALID, "cell without faces");
@@ -2756,10 +2759,12 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
     }
     int s0 = 0;
     // online launch-shape tuning: each step runs the next candidate shape of k_face and of
-    // k_cell (after one warm-up step), timed with events; the fastest of each is kept
+    // k_cell (after one warm-up step), timed with events; every candidate is timed in
+    // TUNE_ROUNDS rounds and the one with the shortest time of each kernel is kept
     while (h->tuning && s0 < nsteps) {
         const int nfc = (int)h->face_cand.size(), ncc = (int)h->cell_cand.size();
-        const int i = h->tune_i;
+        const int ncand = std::max(nfc, ncc);
+        const int i = h->tune_i < 0 ? -1 : h->tune_i % ncand;
         if (i >= 0) {
             const auto &fc = h->face_cand[std::min(i, nfc - 1)];
             const auto &cc = h->cell_cand[std::min(i, ncc - 1)];
@@ -2778,17 +2783,15 @@ int step_impl(dugks_engine *h, const double *dtc, int nsteps)
         HIPCHK(h, hipEventElapsedTime(&tf, h->tev[0], h->tev[2]));
         // the cell side: k_cell, k_cell_fin and k_cell_s1 (one kernel when fused)
         HIPCHK(h, hipEventElapsedTime(&tc, h->tev[3], h->tev[6]));
-        if (i >= 0 && i < nfc && (h->tune_best_face == 0.0 || tf < h->tune_best_face)) {
-            h->tune_best_face = tf;
-            h->tune_face = i;
-        }
-        if (i >= 0 && i < ncc && (h->tune_best_cell == 0.0 || tc < h->tune_best_cell)) {
-            h->tune_best_cell = tc;
-            h->tune_cell = i;
-        }
-        if (++h->tune_i >= std::max(nfc, ncc)) {
-            set_face_shape(h, h->face_cand[h->tune_face].first, h->face_cand[h->tune_face].second);
-            set_cell_shape(h, h->cell_cand[h->tune_cell].first, h->cell_cand[h->tune_cell].second);
+        h->tune_tf.resize(nfc, 0.0);
+        h->tune_tc.resize(ncc, 0.0);
+        if (i >= 0 && i < nfc && (h->tune_tf[i] == 0.0 || tf < h->tune_tf[i])) h->tune_tf[i] = tf;
+        if (i >= 0 && i < ncc && (h->tune_tc[i] == 0.0 || tc < h->tune_tc[i])) h->tune_tc[i] = tc;
+        if (++h->tune_i >= dugks_engine::TUNE_ROUNDS * ncand) {
+            const int bf = (int)(std::min_element(h->tune_tf.begin(), h->tune_tf.end()) - h->tune_tf.begin());
+            const int bc = (int)(std::min_element(h->tune_tc.begin(), h->tune_tc.end()) - h->tune_tc.begin());
+            set_face_shape(h, h->face_cand[bf].first, h->face_cand[bf].second);
+            set_cell_shape(h, h->cell_cand[bc].first, h->cell_cand[bc].second);
             h->tuning = 0;
             drop_graphs(h);
         }
@@ -3279,8 +3282,8 @@ int dugks_set_autotune(dugks_engine *h, int32_t on)
     if (!h) return DUGKS_ERR_INVALID;
     h->tuning = on ? 1 : 0;
     h->tune_i = -1;
-    h->tune_best_face = h->tune_best_cell = 0.0;
-    h->tune_face = h->tune_cell = 0;
+    h->tune_tf.clear();
+    h->tune_tc.clear();
     return 0;
 }
 
@@ -3295,7 +3298,7 @@ int dugks_autotune_steps_left(dugks_engine *h, int32_t *steps)
 {
     if (!h || !steps) return DUGKS_ERR_INVALID;
     const int n = std::max((int)h->face_cand.size(), (int)h->cell_cand.size());
-    *steps = h->tuning ? n - h->tune_i : 0;
+    *steps = h->tuning ? dugks_engine::TUNE_ROUNDS * n - h->tune_i : 0;
     return 0;
 }
 

@@ -194,7 +194,8 @@ int dugks_set_graphs(dugks_engine* h, int32_t on);
 
 /* Online launch-shape tuning (no reference counterpart): with autotune on, the next steps
  * run one candidate work decomposition of the face kernel and of the cell kernel each (after
- * one warm-up step), timed with HIP events, then keep the fastest of each.  The steps are
+ * one warm-up step; every candidate in two rounds), timed with HIP events, then keep the one
+ * with the shortest time of each.  The steps are
  * real steps of the simulation; the candidates differ only in how velocity sums are split,
  * i.e. in round-off.  dugks_get_launch / dugks_get_cell_launch report the shapes in use. */
 int dugks_set_autotune(dugks_engine* h, int32_t on);

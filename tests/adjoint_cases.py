@@ -81,6 +81,8 @@ CASES = [
     ("symmetry_nv7_x0", lambda: cases.symmetry_cavity(3, 6, 7, plane="x0"), 5, 1, 2),  # xi.S == 0: self-mirror
     # 1-D: heat conduction between two plates at 263 K and 283 K (fvDVM.C:198-215 velocity set)
     ("fourier_1d", lambda: cases.fourier_flow(8, nv=9), 6, 2, 2),
+    # polygons with 12 faces, several of them shared with one neighbour
+    ("agglomerated", lambda: cases.agglomerated_cavity(9, nv=6, block=3), 5, 2, 2),
 ]
 
 

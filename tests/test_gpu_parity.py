@@ -49,6 +49,8 @@ def _small_cases():
         "openchannel_12x6": (lambda: cases.open_channel(12, 6, nv=12), 10),
         "pchannel_12x6": (lambda: cases.pressure_channel(12, 6, nv=12), 10),
         "zgchannel_12x6": (lambda: cases.open_channel(12, 6, nv=12, outlet="zeroGradient"), 10),
+        "agglom_b3": (lambda: cases.agglomerated_cavity(12, nv=12, block=3), 8),
+        "agglom_b4": (lambda: cases.agglomerated_cavity(12, nv=9, block=4, K=1), 8),
         "fourier_1d": (lambda: cases.fourier_flow(12, nv=13), 10),
         "shocktube_1d": (lambda: cases.shock_tube(16, nv=12), 8),
         "shocktube_1d_wall": (lambda: cases.shock_tube(12, nv=10, ends="wall", K=1), 8),
@@ -131,7 +133,8 @@ def test_courant(pair):
 
 
 @pytest.mark.parametrize("which", ["cavity2d_12x21", "cavity3d_6x12", "delaunay_300", "channel_20x8", "symcavity_6x12",
-                                   "symplane_6x12", "openchannel_12x6", "zgchannel_12x6", "fourier_1d", "shocktube_1d"])
+                                   "symplane_6x12", "openchannel_12x6", "zgchannel_12x6", "fourier_1d", "shocktube_1d",
+                                   "agglom_b3"])
 def test_fp32_storage(which):
     """FP32 storage of g/h (FP64 arithmetic) against the FP64 oracle: every field within 1e-5."""
     mk, n = _small_cases()[which]

@@ -133,14 +133,17 @@ def test_lsq_uniform_grid_is_central_difference(oracle_lib):
     assert g[cc, 1] == pytest.approx((phi[cc + n] - phi[cc - n]) / (2 * dx), rel=1e-12)
 
 
-@pytest.mark.parametrize("mesh", ["uniform2d", "uniform3d", "delaunay"])
+@pytest.mark.parametrize("mesh", ["uniform2d", "uniform3d", "delaunay", "agglomerated"])
 def test_lsq_exact_for_linear_fields(oracle_lib, mesh):
-    """Least squares with boundary values of the same linear field is exact on any mesh.
+    """Least squares with boundary values of the same linear field is exact on any mesh, the
+    polygonal one with cells of 16 faces (several shared with one neighbour) included.
     The boundary value sits at C_P + delta (the point the fvPatch delta reaches)."""
     if mesh == "uniform2d":
         c = cases.cavity(7, 5)
     elif mesh == "uniform3d":
         c = cases.cavity(4, 5, dim=3)
+    elif mesh == "agglomerated":
+        c = cases.agglomerated_cavity(12, nv=4, block=4)
     else:
         c = cases.delaunay_cavity(400, 16, nv=4)
     o = _oracle(c)
