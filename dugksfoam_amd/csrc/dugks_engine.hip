@@ -96,6 +96,10 @@ constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
 #ifndef KC_PRELOAD
 #define KC_PRELOAD 1
 #endif
+// cells whose faces are all internal take a loop without the boundary branches
+#ifndef KFV_INNER
+#define KFV_INNER 1
+#endif
 #ifndef FM_UNR
 #define FM_UNR 2
 #endif
@@ -166,7 +170,8 @@ struct alignas(64) CellSt {
     double L[MAXS][3];
     double a[MAXS];       // wall entries: 1/deltaCoeffs
     int32_t wmask;        // bit j set: entry j is a wall face
-    int32_t pad[15];
+    int32_t inner;        // 1: every slot is an internal face (KIND_OWN or KIND_NEI)
+    int32_t pad[14];
 };
 static_assert(sizeof(CellSt) % 64 == 0, "cellst size");
 
@@ -775,6 +780,56 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
     const TS *__restrict__ rowC = p.gb_in + (size_t)c * p.nvp;
     const TS *__restrict__ hrowC = H ? p.hb_in + (size_t)c * p.nvp : nullptr;
     const unsigned k0 = (unsigned)b * p.vb + threadIdx.x;
+    if (KFV_INNER && cptr(p.cst)[c].inner) {
+        // every face of the cell is internal: the owner and neighbour branches of the loop below,
+        // unrolled over the stencil, with the same arithmetic
+        for (int it = 0; it < p.nib; ++it) {
+            const unsigned k = k0 + it * p.nt;
+            const Vel v = load_vel<D>(p, k, vt);
+            const double hx[3] = {0.5 * v.x * dt, 0.5 * v.y * dt, 0.5 * v.z * dt};
+            const double gC = ld(rowC, k);
+            const double hC = H ? ld(hrowC, k) : 0.0;
+            double GC[3] = {0, 0, 0}, HC[3] = {0, 0, 0};
+            grad_at<D, H, NS>(p, c, k, gC, hC, GC, HC);
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                if (j >= ns) continue;   // a constant trip count keeps the loop unrollable
+                const CONST_AS Slot &S = SL[j];
+                const double Sv[3] = {S.S[0], S.S[1], S.S[2]};
+                const double s = dotnc(v.x, v.y, v.z, Sv);
+                const double dCs[3] = {S.dCs[0], S.dCs[1], S.dCs[2]};
+                TS *__restrict__ fg = p.fvg + (size_t)S.face * p.nvp;
+                TS *__restrict__ fh = H ? p.fvh + (size_t)S.face * p.nvp : nullptr;
+                if (S.kind == KIND_NEI) {
+                    if (s < -VSMALL) {
+                        st_stream(fg, k, gC + dotg<D>(GC, dCs, hx));
+                        if (H) st_stream(fh, k, hC + dotg<D>(HC, dCs, hx));
+                    }
+                    continue;
+                }
+                // owner: upwind where xi.S >= VSMALL; where |xi.S| < VSMALL the mean of both
+                // reconstructions, which needs the neighbour's gradient
+                const bool mid = !(s >= VSMALL) && !(s < -VSMALL);
+                if (__any(mid)) {
+                    const double dCo[3] = {S.dCo[0], S.dCo[1], S.dCo[2]};
+                    const int o = S.other;
+                    const double gO = ld(p.gb_in + (size_t)o * p.nvp, k);
+                    const double hO = H ? ld(p.hb_in + (size_t)o * p.nvp, k) : 0.0;
+                    double GO[3] = {0, 0, 0}, HO[3] = {0, 0, 0};
+                    grad_at<D, H, NS>(p, o, k, gO, hO, GO, HO);
+                    if (mid) {
+                        st_stream(fg, k, face_value<D>(KIND_OWN, s, gC, GC, dCs, gO, GO, dCo, 0.0, hx));
+                        if (H) st_stream(fh, k, face_value<D>(KIND_OWN, s, hC, HC, dCs, hO, HO, dCo, 0.0, hx));
+                    }
+                }
+                if (s >= VSMALL) {
+                    st_stream(fg, k, gC + dotg<D>(GC, dCs, hx));
+                    if (H) st_stream(fh, k, hC + dotg<D>(HC, dCs, hx));
+                }
+            }
+        }
+        return;
+    }
     for (int it = 0; it < p.nib; ++it) {
         const unsigned k = k0 + it * p.nt;
         const Vel v = load_vel<D>(p, k, vt);
@@ -2477,6 +2532,11 @@ int build(dugks_engine *h, const dugks_case *cs)
         std::memset(&t, 0, sizeof t);
         const int ns = cnt[c + 1] - cnt[c];
         h->ns_max = std::max(h->ns_max, ns);
+        t.inner = ns > 0;
+        for (int j = 0; j < ns; ++j) {
+            const int kd = sl[cnt[c] + j].kind;
+            if (kd != KIND_OWN && kd != KIND_NEI) t.inner = 0;
+        }
         for (int j = 0; j < MAXS; ++j) {
             t.off[j] = (int64_t)c * h->nvp;
             if (j >= ns) continue;
