@@ -1,5 +1,5 @@
-// adjoint_core.h -- per-point work of the discrete adjoint of the DUGKS step (walls and cyclic
-// patches, FP64), written once for two executors:
+// adjoint_core.h -- per-point work of the discrete adjoint of the DUGKS step (every patch kind of
+// the engine, FP64), written once for two executors:
 //   * the HIP kernels of dugks_adjoint.hip: one workgroup per cell or face, the lanes stride over
 //     the velocities, velocity sums are workgroup reductions;
 //   * a host executor (plain loops) used to check the same code on the CPU.
@@ -8,8 +8,9 @@
 //   X::sum<N>(v)       replace the worker's partial sums v[N] by the point's totals,
 //   X::leader()        true for the one worker that writes per-point scalars.
 //
-// The step is fvDVM::evolution() (fvDVM.C:1086-1108) restricted to internal faces, Maxwell walls,
-// cyclic pairs, fixed-Maxwellian and zeroGradient patches (S1-S9 of the oracle).  State n is (g~, h~ [cell][k], the lagged fixedGradient
+// The step is fvDVM::evolution() (fvDVM.C:1086-1108) with internal faces, Maxwell walls, cyclic
+// pairs, fixed-Maxwellian, far-field, pressureIn/pressureOut, zeroGradient, symmetryMod and
+// symmetryPlane patches (S1-S10 of the oracle).  State n is (g~, h~ [cell][k], the lagged fixedGradient
 // rows Gg, Gh [boundary face][k], the cell macros rho, U, T, q).  Its macros are the S9 values of
 // step n-1 (for n = 0 the given initial fields with q = 0).  The objective is
 //   J = sum over steps n >= first of sum_f c_f (rho U)_f . S_f,  (rho U)_f = sum_k w_k xi_k gbar_f,k
@@ -46,13 +47,19 @@ constexpr int FK_SYM = 5;     // symmetry plane: incoming value = outgoing value
 constexpr int FK_FAR = 6;     // far field: incoming rho_b M(U of the owner cell, T_b)
 constexpr int FK_PIN = 7;     // pressureIn: a far field with rho_b = p_b / (R T_b)
 constexpr int FK_POUT = 8;    // pressureOut: a far field whose rho_b, T_b follow the owner cell
+// symmetryPlane (OpenFOAM constraint patch): incoming value as FK_SYM, the gradient takes
+// phi_b = phi_P (no lagged row), S7 relaxes every value once and the outgoing ones once more
+constexpr int FK_SPL = 9;
 
-// Non-coupled boundary faces: a lagged fixedGradient row, only the outgoing half relaxed.
+// Non-coupled boundary faces: a lagged fixedGradient row, only the outgoing half relaxed
+// (FK_SPL is the exception to both).
 ADJ_HD bool open_kind(int fk) { return fk != FK_INTERNAL && fk != FK_CYCLIC; }
 // Boundary faces whose incoming half is a Maxwellian at the boundary temperature (dJ/dT output).
 ADJ_HD bool maxwell_kind(int fk) { return fk == FK_WALL || fk == FK_FIXED || fk == FK_FAR || fk == FK_PIN || fk == FK_POUT; }
 // Far-field kinds: incoming rho_b M(U of the owner cell, T_b).
 ADJ_HD bool far_kind(int fk) { return fk == FK_FAR || fk == FK_PIN || fk == FK_POUT; }
+// Mirror planes: the incoming value of a velocity is the outgoing value of its mirror.
+ADJ_HD bool mirror_kind(int fk) { return fk == FK_SYM || fk == FK_SPL; }
 
 // Mesh, velocity set and gas; every array is indexed by global face / cell numbers.
 struct AP {
@@ -376,6 +383,10 @@ ADJ_HD void grad_cell(X &x, const AP &p, St s, int c, const double *gp, const do
                 L = p.pvec + 3 * f;
                 dg = gp[ix(o, nv, k)] - gc;
                 dh = hp[ix(o, nv, k)] - hc;
+            } else if (p.fkind[f] == FK_SPL) {
+                L = p.pvec + 3 * f;
+                dg = gc - gc;   // phi_b = phi_P
+                dh = hc - hc;
             } else {
                 L = p.pvec + 3 * f;
                 const size_t b = ix(f - nfi, nv, k);
@@ -394,7 +405,7 @@ ADJ_HD void grad_cell(X &x, const AP &p, St s, int c, const double *gp, const do
         if (Gng)
             for (int e = p.cstart[c]; e < p.cstart[c + 1]; ++e) {
                 const int f = p.cface[e];
-                if (f < nfi || p.fkind[f] == FK_CYCLIC) continue;
+                if (f < nfi || p.fkind[f] == FK_CYCLIC || p.fkind[f] == FK_SPL) continue;
                 Gng[ix(f - nfi, nv, k)] = dot3(G, p.nhat + 3 * f);
                 Gnh[ix(f - nfi, nv, k)] = dot3(H, p.nhat + 3 * f);
             }
@@ -418,7 +429,7 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
         h = hp[ix(P, nv, k)];
         return;
     }
-    if (fk == FK_SYM && !(s > 0)) {
+    if (mirror_kind(fk) && !(s > 0)) {
         // incoming: the outgoing reconstruction of the mirrored velocity, at its own index
         // (discreteVelocity.C:746-790); xi.S == 0 mirrors onto itself
         const int km = p.mir[p.sax[f] * nv + k];
@@ -433,7 +444,7 @@ ADJ_HD void face_value(const AP &p, int f, int k, double s, const double *gp, co
         return;
     }
     const double gP = gp[ix(P, nv, k)] + dot3(GP, dO), hP = hp[ix(P, nv, k)] + dot3(HP, dO);
-    if (fk == FK_WALL || fk == FK_FIXED || fk == FK_SYM || far_kind(fk)) {
+    if (fk == FK_WALL || fk == FK_FIXED || mirror_kind(fk) || far_kind(fk)) {
         g = gP;
         h = hP;
         return;
@@ -582,7 +593,8 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, const double *
     x.for_k(nv, [&](int k) {
         const double *xi = p.xi + 3 * k;
         const size_t i = ix(f, nv, k);
-        if (open_kind(fk) && !(sdot(xi, S) > 0)) {
+        const bool out = sdot(xi, S) > 0;
+        if (open_kind(fk) && fk != FK_SPL && !out) {
             gr[i] = gf[i];   // incoming halves are not relaxed (their patch operators are no-ops, *Fvs.H:161-181)
             hr[i] = hf[i];
             return;
@@ -591,6 +603,12 @@ ADJ_HD void face_fwd(X &x, const AP &p, int f, const double *mac, const double *
         shakhov(p, e, xi, geq, heq);
         gr[i] = (1.0 - r) * gf[i] + r * geq;
         hr[i] = (1.0 - r) * hf[i] + r * heq;
+        if (fk == FK_SPL && out) {
+            // symmetryPlane: the field operation relaxed every value (discreteVelocity.C:880-881),
+            // the loop over non-coupled patches relaxes the outgoing ones again (905-920)
+            gr[i] = (1.0 - r) * gr[i] + r * geq;
+            hr[i] = (1.0 - r) * hr[i] + r * heq;
+        }
     });
 }
 
@@ -678,7 +696,17 @@ ADJ_HD void face_adj(X &x, const AP &p, int f, const double *mac, const FaceRec 
             bf += (s * dt) * ah1[ix(N, nv, k)] / VN;
         }
         double ab = af, bb = bf;
-        if (!open_kind(fk) || s > 0) {
+        if (fk == FK_SPL && s > 0) {
+            // two relaxations, g1 = (1 - r) gf + r geq and gr = (1 - r) g1 + r geq
+            double geq, heq;
+            shakhov(p, e, xi, geq, heq);
+            const double g1 = (1.0 - R.r) * gf[i] + R.r * geq, h1 = (1.0 - R.r) * hf[i] + R.r * heq;
+            acc[0] += af * (geq - g1) + bf * (heq - h1) + (1.0 - R.r) * (af * (geq - gf[i]) + bf * (heq - hf[i]));
+            const double we = R.r * (2.0 - R.r);
+            shakhov_vjp(p, e, xi, we * af, we * bf, acc + 1);
+            ab = (1.0 - R.r) * (1.0 - R.r) * af;
+            bb = (1.0 - R.r) * (1.0 - R.r) * bf;
+        } else if (!open_kind(fk) || fk == FK_SPL || s > 0) {
             double geq, heq;
             shakhov(p, e, xi, geq, heq);
             acc[0] += af * (geq - gf[i]) + bf * (heq - hf[i]);
@@ -826,7 +854,7 @@ ADJ_HD void recon_adj(X &x, const AP &p, int c, const double *abf, const double 
                 // outgoing value of this velocity; on a symmetry plane also the incoming value of
                 // its mirror, which is the same reconstruction
                 double wg = s > 0 ? vg : 0.0, wh = s > 0 ? vh : 0.0;
-                if (p.fkind[f] == FK_SYM) {
+                if (mirror_kind(p.fkind[f])) {
                     const int km = p.mir[p.sax[f] * nv + k];
                     if (!(sdot(p.xi + 3 * km, p.S + 3 * f) > 0)) {
                         wg += abf[ix(f, nv, km)];
@@ -839,10 +867,12 @@ ADJ_HD void recon_adj(X &x, const AP &p, int c, const double *abf, const double 
                     G[q] += wg * d[q];
                     H[q] += wh * d[q];
                 }
-                const double bg = AGg[ix(f - nfi, nv, k)], bh = AGh[ix(f - nfi, nv, k)];
-                for (int q = 0; q < 3; ++q) {
-                    G[q] += bg * p.nhat[3 * f + q];
-                    H[q] += bh * p.nhat[3 * f + q];
+                if (p.fkind[f] != FK_SPL) {   // a symmetryPlane has no lagged row
+                    const double bg = AGg[ix(f - nfi, nv, k)], bh = AGh[ix(f - nfi, nv, k)];
+                    for (int q = 0; q < 3; ++q) {
+                        G[q] += bg * p.nhat[3 * f + q];
+                        H[q] += bh * p.nhat[3 * f + q];
+                    }
                 }
             } else {
                 // own face: upwind part (weight o1) and coupled part (o2), w = this face's weight
@@ -904,8 +934,11 @@ ADJ_HD void grad_adj(X &x, const AP &p, int c, const double *agg, const double *
                 Lc = p.pvec + 3 * f;
                 Lo = p.pvec + 3 * p.partner[f];
             } else {
-                AGg[ix(f - nfi, nv, k)] = p.adel[f] * dot3(p.pvec + 3 * f, Gc);
-                AGh[ix(f - nfi, nv, k)] = p.adel[f] * dot3(p.pvec + 3 * f, Hc);
+                // lagged row: phi_b = phi_P + G / deltaCoeffs; a symmetryPlane's phi_b = phi_P
+                // adds L (phi_P - phi_P) = 0 and reads no row
+                const bool spl = p.fkind[f] == FK_SPL;
+                AGg[ix(f - nfi, nv, k)] = spl ? 0.0 : p.adel[f] * dot3(p.pvec + 3 * f, Gc);
+                AGh[ix(f - nfi, nv, k)] = spl ? 0.0 : p.adel[f] * dot3(p.pvec + 3 * f, Hc);
                 continue;
             }
             double Go[3], Ho[3];

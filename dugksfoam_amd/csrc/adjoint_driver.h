@@ -95,14 +95,14 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
         code = DUGKS_ERR_INVALID;
         if (pp.start < nfi || pp.size < 0 || pp.start + pp.size > nf) return "patch face range outside the boundary";
         code = DUGKS_ERR_UNSUPPORTED;
-        if (pp.type == DUGKS_PATCH_SYMMETRY_PLANE) return "symmetryPlane patches are not differentiated (symmetryMod is)";
         if (pp.type != DUGKS_PATCH_WALL && pp.type != DUGKS_PATCH_CYCLIC && pp.type != DUGKS_PATCH_FIXED &&
             pp.type != DUGKS_PATCH_ZEROGRADIENT && pp.type != DUGKS_PATCH_SYMMETRY &&
+            pp.type != DUGKS_PATCH_SYMMETRY_PLANE &&
             pp.type != DUGKS_PATCH_FARFIELD && pp.type != DUGKS_PATCH_PRESSURE_IN &&
             pp.type != DUGKS_PATCH_PRESSURE_OUT)
             return "unknown patch type";
         int axis = 0;
-        if (pp.type == DUGKS_PATCH_SYMMETRY && pp.size) {
+        if ((pp.type == DUGKS_PATCH_SYMMETRY || pp.type == DUGKS_PATCH_SYMMETRY_PLANE) && pp.size) {
             // the mirror axis from the patch's first face normal (discreteVelocity.C:766-768)
             const double *S0 = cs->Sf + 3 * (size_t)pp.start;
             const double mS = dugks_lsq::mag3(S0);
@@ -145,7 +145,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
                 if (!(m.Tw[f] > 0) || !(m.rb[f] > 0)) return "boundary density and temperature must be > 0";
             } else if (pp.type == DUGKS_PATCH_ZEROGRADIENT) {
                 // nothing beyond the lagged gradient row
-            } else if (pp.type == DUGKS_PATCH_SYMMETRY) {
+            } else if (pp.type == DUGKS_PATCH_SYMMETRY || pp.type == DUGKS_PATCH_SYMMETRY_PLANE) {
                 m.sax[f] = axis;
             } else {
                 bpartner[b] = cs->patches[pp.neighbour_patch].start - nfi + i;
@@ -186,6 +186,7 @@ inline const char *prepare(const dugks_case *cs, Mesh &m, int &code)
             m.fkind[f] = bkind[b] == DUGKS_PATCH_WALL    ? FK_WALL
                          : bkind[b] == DUGKS_PATCH_FIXED   ? FK_FIXED
                          : bkind[b] == DUGKS_PATCH_SYMMETRY ? FK_SYM
+                         : bkind[b] == DUGKS_PATCH_SYMMETRY_PLANE ? FK_SPL
                          : bkind[b] == DUGKS_PATCH_FARFIELD ? FK_FAR
                          : bkind[b] == DUGKS_PATCH_PRESSURE_IN ? FK_PIN
                          : bkind[b] == DUGKS_PATCH_PRESSURE_OUT ? FK_POUT

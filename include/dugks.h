@@ -256,7 +256,7 @@ int dugks_abi_version(void);
  * The run starts from the Shakhov equilibrium of rho, U, T with q = 0 (as
  * dugks_init_equilibrium) and advances n_steps FP64 steps of size dt, keeping a checkpoint every
  * checkpoint_every steps (0: ceil(sqrt(n_steps))); the reverse sweep re-runs each segment.
- * Cases: every patch kind except symmetryPlane (DUGKS_ERR_UNSUPPORTED); batches (n_cases > 1) with their per-case muRef and the one dt; FP64 (the case's
+ * Cases: every patch kind; batches (n_cases > 1) with their per-case muRef and the one dt; FP64 (the case's
  * storage field is ignored).  Device memory: about
  * (ceil(n_steps / C) + C + 1) x (2 n_cells + 2 n_boundary_faces) x nv x 8 bytes for the trajectory
  * plus about (8 + 4 dim) n_cells x nv x 8 and 4 n_faces x nv x 8 bytes of work arrays;

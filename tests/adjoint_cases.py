@@ -79,6 +79,9 @@ CASES = [
     # symmetry plane: incoming values are the mirrored velocities' outgoing reconstructions
     ("symmetry_nv6", lambda: cases.symmetry_cavity(4, 6, 6), 6, 2, 0),
     ("symmetry_nv7_x0", lambda: cases.symmetry_cavity(3, 6, 7, plane="x0"), 5, 1, 2),  # xi.S == 0: self-mirror
+    # symmetryPlane: phi_b = phi_P in the gradient (no lagged row), outgoing values relaxed twice
+    ("symplane_nv6", lambda: cases.symmetry_cavity(4, 6, 6, kind="symmetryPlane"), 6, 2, 0),
+    ("symplane_nv7_x0", lambda: cases.symmetry_cavity(3, 6, 7, plane="x0", kind="symmetryPlane"), 5, 1, 2),
     # 1-D: heat conduction between two plates at 263 K and 283 K (fvDVM.C:198-215 velocity set)
     ("fourier_1d", lambda: cases.fourier_flow(8, nv=9), 6, 2, 2),
     # polygons with 12 faces, several of them shared with one neighbour

@@ -83,9 +83,6 @@ def test_unsupported_and_invalid_inputs():
         c.patches[0]["type"] = 99
         Adjoint(c)
     assert e.value.code == -6 and "unknown patch type" in str(e.value)
-    with pytest.raises(DugksError) as e:
-        Adjoint(cases.symmetry_cavity(3, 6, nv=6, kind="symmetryPlane"))
-    assert e.value.code == -6 and "symmetryPlane" in str(e.value)
     c = cases.cavity(3, 5)
     with Adjoint(c) as A:
         with pytest.raises(DugksError) as e:
