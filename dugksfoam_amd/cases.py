@@ -665,7 +665,7 @@ def delaunay_cavity(n_interior: int = 99400, n_side: int = 400, nv: int = 64, kn
                     seed: int = 0, R: float = 208.0, T0: float = 273.0, rho0: float = 1.0,
                     xi_sigma: float = 5.0, lid_mach: float = 0.16, Pr: float = 2.0 / 3.0,
                     omega: float = 0.5, K: int = 0, co: float = 0.5, dz: float = 0.01,
-                    min_sep: float = 0.5, smooth: int = 8) -> Case:
+                    min_sep: float = 0.5, smooth: int = 8, wall_layer: bool = False) -> Case:
     """Cavity on a Delaunay triangulation of seeded uniform-random interior points plus
     n_side uniform points per side of the unit square (one prism layer thick).
 
@@ -677,7 +677,11 @@ def delaunay_cavity(n_interior: int = 99400, n_side: int = 400, nv: int = 64, kn
     a hard-core process still allows.  dt is the stricter of that Courant number and the
     finite-volume bound sum_f |Sf| sqrt(2) xi_max dt / V <= 2 co (at most about half of
     a cell's faces are outflow faces for any one velocity).  Cells are renumbered along a Hilbert curve of
-    their centroids for locality; faces follow OpenFOAM's upper-triangular order."""
+    their centroids for locality; faces follow OpenFOAM's upper-triangular order.
+
+    wall_layer: one row of isosceles triangles along the walls, the apex of each above the
+    midpoint of its wall edge (height sqrt(3)/2 of the edge), so that every wall cell's centroid
+    lies on its wall face's normal through the face centre; the random points fill the rest."""
     from scipy.spatial import Delaunay
     rng = np.random.default_rng(seed)
     t = np.arange(n_side) / n_side
@@ -685,14 +689,26 @@ def delaunay_cavity(n_interior: int = 99400, n_side: int = 400, nv: int = 64, kn
                           np.stack([np.ones_like(t), t], 1),
                           np.stack([1 - t, np.ones_like(t)], 1),
                           np.stack([np.zeros_like(t), 1 - t], 1)])
+    fixed = bnd
+    lo = 0.0
+    if wall_layer:
+        # apexes above the wall-edge midpoints, except on the two edges at each corner
+        hgt = 0.5 * math.sqrt(3.0) / n_side
+        m = (np.arange(1, n_side - 1) + 0.5) / n_side
+        lay = np.concatenate([np.stack([m, np.full_like(m, hgt)], 1),
+                              np.stack([np.full_like(m, 1 - hgt), m], 1),
+                              np.stack([1 - m, np.full_like(m, 1 - hgt)], 1),
+                              np.stack([np.full_like(m, hgt), 1 - m], 1)])
+        fixed = np.concatenate([bnd, lay])
+        lo = hgt
     if min_sep > 0:
         r = min_sep / math.sqrt(n_interior)
-        pin = hard_core_points(n_interior, r, bnd, rng, margin=0.5 * r)
+        pin = hard_core_points(n_interior, r, fixed, rng, margin=lo + 0.5 * r)
     else:
-        margin = 0.25 / n_side
+        margin = lo + 0.25 / n_side
         pin = rng.uniform(margin, 1 - margin, size=(n_interior, 2))
-    pts = np.concatenate([bnd, pin])
-    nb_ = bnd.shape[0]
+    pts = np.concatenate([fixed, pin])
+    nb_ = fixed.shape[0]
     for _ in range(smooth):
         sim = Delaunay(pts).simplices
         e = np.concatenate([sim[:, [0, 1]], sim[:, [1, 2]], sim[:, [2, 0]]])
@@ -888,7 +904,7 @@ def config(name: str, **kw) -> Case:
     if name == "cavity2d":
         return cavity(**{**dict(n=60, nv=41, kn=0.075, dim=2), **kw})
     if name == "delaunay":
-        return delaunay_cavity(**kw)
+        return delaunay_cavity(**{**dict(wall_layer=True), **kw})
     if name == "cavity3d":
         return cavity(**{**dict(n=48, nv=24, kn=0.1, dim=3), **kw})
     if name == "cavity3d_share":
