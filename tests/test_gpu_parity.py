@@ -403,9 +403,11 @@ def test_golden_cavity3d(storage):
 def test_delaunay_full_size_runs_its_steps():
     """The c2 workload (about 2e5 triangles x 64^2 velocities, wall row orthogonal) over its
     1000 steps: every field finite, the temperature within 250-300 K of the 273 K walls and the
-    lid's flow.  The reference scheme is not exactly conservative on this mesh (the lagged
-    wall extrapolation and the averaged reconstruction leak a little mass): the total mass
-    drifts by about 1.2e-6 of itself over the 1000 steps, checked against a 1e-5 bound."""
+    lid's flow.  The reference scheme does not conserve mass exactly (the discrete equilibrium is
+    not moment-exact on the truncated velocity grid, dv.C:393-406, 937; the wall density
+    balances the outgoing flux before relaxation, dv.C:623-624, while the flux uses the relaxed
+    values, dv.C:905-920): the total mass drifts by about 1.2e-6 of itself over the 1000 steps,
+    checked against a 1e-5 bound."""
     c = cases.config("delaunay")
     _say(f"[c2] cells={c.n_cells} nv={c.nv} steps=1000")
     e = Engine(c)
