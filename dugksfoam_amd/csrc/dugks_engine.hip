@@ -761,7 +761,8 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
 // fixedGradient rows n . grad(gbar) it writes as well (so also for symmetry, far-field and fixed
 // faces, whose incoming halves k_face_open writes).  Every face value is written exactly once,
 // with the arithmetic of k_face (same gradient, same face_value), so the stored values equal
-// k_face's.
+// k_face's.  A cell whose faces are all internal (CellSt::inner) takes a copy of the internal
+// branches unrolled over the stencil, without the boundary-kind dispatch of the general loop.
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, int NS, typename TS>
 __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 8 ? FACE_WAVES : FACE_WAVES_FP32, 8))) void k_face_values(P<TS> p)
