@@ -52,7 +52,9 @@ def test_oracle_under_sanitizers():
               cases.channel(6, 4, 1.0, 0.6, nv=8), cases.symmetry_cavity(4, 6, nv=9),
               cases.symmetry_cavity(3, 6, nv=8, plane="x0"), cases.open_channel(6, 4, nv=8),
               cases.pressure_channel(6, 4, nv=8), cases.open_channel(6, 4, nv=8, outlet="zeroGradient"),
-              cases.delaunay_cavity(60, 6, nv=6), cases.batch(n_cases=3, nmin=3, nmax=5, nv=6, seed=2)]
+              cases.delaunay_cavity(60, 6, nv=6), cases.batch(n_cases=3, nmin=3, nmax=5, nv=6, seed=2),
+              cases.symmetry_cavity(4, 6, nv=9, kind="symmetryPlane"), cases.fourier_flow(8, nv=9),
+              cases.agglomerated_cavity(9, nv=6, block=3), cases.delaunay_cavity(60, 6, nv=6, wall_layer=True)]
         for c in cs:
             for threads in (1, 3):
                 o = Oracle(c, threads=threads, variant="san", cache_faces=threads == 3)
@@ -70,7 +72,7 @@ def test_oracle_under_sanitizers():
                 o.close()
             print("ok", c.name, flush=True)
     """)
-    assert out.count("ok") == 11
+    assert out.count("ok") == 15
 
 
 def test_host_adjoint_under_sanitizers():
