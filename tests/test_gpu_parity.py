@@ -402,7 +402,7 @@ def test_golden_cavity3d(storage):
 
 def test_delaunay_full_size_runs_its_steps():
     """The c2 workload (about 2e5 triangles x 64^2 velocities, wall row orthogonal) over its
-    1000 steps: every field finite, the temperature within 250-300 K of the 273 K walls and the
+    1000 steps, in FP64 and in FP32 storage (each field within 1e-5 of FP64): every field finite, the temperature within 250-300 K of the 273 K walls and the
     lid's flow.  The reference scheme does not conserve mass exactly (the discrete equilibrium is
     not moment-exact on the truncated velocity grid, dv.C:393-406, 937; the wall density
     balances the outgoing flux before relaxation, dv.C:623-624, while the flux uses the relaxed
@@ -424,6 +424,16 @@ def test_delaunay_full_size_runs_its_steps():
     _say(f"  mass drift {drift:.2e}")
     assert drift <= 1e-5
     e.close()
+    # the FP32-storage form over the same 1000 steps, within 1e-5 of the FP64 fields
+    e32 = Engine(c, storage="fp32")
+    e32.init_equilibrium()
+    e32.step(nsteps=1000)
+    m32 = e32.macro()
+    e32.close()
+    for k in ("rho", "U", "T", "q"):
+        r = rel_l2(m32[k], m[k])
+        _say(f"  fp32 vs fp64 {k}: {r:.2e}")
+        assert r <= FP32_TOL, (k, r)
 
 
 def test_cavity3d_full_size_invariants():
