@@ -345,6 +345,64 @@ def test_golden_full_size(name):
     np.testing.assert_allclose(np.array(mass), g["mass"], rtol=1e-12)
 
 
+_SMALLEST = {
+    # one cell whose faces are all walls: no internal face, so no face on the matrix-core
+    # moment kernel, and a least-squares stencil of wall rows only
+    "cavity2d_1cell": lambda: cases.cavity(1, 9),
+    "cavity3d_1cell": lambda: cases.cavity(1, 6, dim=3),
+    "cavity2d_2x2": lambda: cases.cavity(2, 7),
+    "fourier_1d_1cell": lambda: cases.fourier_flow(1, nv=9),
+    "shocktube_1d_2cells": lambda: cases.shock_tube(2, nv=8),
+}
+
+
+@pytest.mark.parametrize("which", list(_SMALLEST))
+def test_smallest_meshes(which):
+    """Degenerate meshes (one or two cells, every or all but one face on a boundary) in both
+    storage forms against the FP64 oracle over 5 steps: FP64 cell and face fields and the mass
+    history at the FP64 bar, FP32 storage at 1e-5.  A U or q field that is itself round-off (RMS
+    below 1e-10 of its natural scale, sqrt(R T) or rho (R T)^1.5) is measured against that scale
+    instead: on the two walls of a one-cell Fourier flow the face velocity is the residue of equal
+    in- and outgoing mass fluxes, about 1e-14 sqrt(R T) in both implementations."""
+    c = _SMALLEST[which]()
+    _say(f"[smallest] {which} cells={c.n_cells} faces={c.n_faces} nv={c.nv}")
+    o = _oracle(c)
+    e64 = Engine(c)
+    e32 = Engine(c, storage="fp32")
+    try:
+        for e in (e64, e32):
+            e.init_equilibrium()
+        mo, me = [], []
+        for _ in range(5):
+            o.step()
+            e64.step()
+            e32.step()
+            mo.append(o.total_mass())
+            me.append(seq_mass(c.V, e64.macro()["rho"]))
+        np.testing.assert_allclose(me, mo, rtol=1e-13)
+        co, fo = o.macro(), o.face_macro()
+        c64, f64 = e64.macro(), e64.face_macro()
+        c32 = e32.macro()
+        RT = c.R * float(np.mean(co["T"]))
+        scale = dict(rho=0.0, T=0.0, U=np.sqrt(RT), q=float(np.mean(co["rho"])) * RT ** 1.5)
+
+        def err(a, b, k):
+            a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+            ref = scale[k] * np.sqrt(b.size)
+            nb = np.linalg.norm(b)
+            return float(np.linalg.norm(a - b) / (ref if nb < 1e-10 * ref else nb))
+
+        for k in ("rho", "U", "T", "q"):
+            r_c, r_f, r_32 = err(c64[k], co[k], k), err(f64[k], fo[k], k), err(c32[k], co[k], k)
+            _say(f"  {which} {k}: cell {r_c:.2e} face {r_f:.2e} fp32 {r_32:.2e}")
+            assert r_c <= FP64_TOL and r_f <= FP64_TOL, (which, k, r_c, r_f)
+            assert r_32 <= FP32_TOL, (which, k, r_32)
+    finally:
+        e64.close()
+        e32.close()
+        o.close()
+
+
 def test_delaunay_midsize():
     """An unstructured Delaunay cavity of about 10^4 triangles (irregular owner/neighbour
     gathers, non-orthogonal boundary faces, up to 8 faces per cell after smoothing) against
