@@ -72,8 +72,8 @@ def bytes_model(case, storage_bytes: int, n_blocks: int, n_blocks_cell: int = No
     return kb, step_min
 
 
-def _counters():
-    p = os.path.join(HERE, "profiles", "counters_cavity3d.json")
+def _counters(workload: str = "cavity3d"):
+    p = os.path.join(HERE, "profiles", f"counters_{workload}.json")
     if not os.path.exists(p):
         return {}
     try:
@@ -82,17 +82,17 @@ def _counters():
         return {}
 
 
-def traffic_from_profile(kernel_prefix: str, storage: str):
+def traffic_from_profile(kernel_prefix: str, storage: str, workload: str = "cavity3d"):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, each scaled
     by the factor a plain device copy of known size reads back), recorded in
-    profiles/counters_cavity3d.json, else None."""
-    return _counters().get(storage, {}).get(kernel_prefix)
+    profiles/counters_<workload>.json, else None."""
+    return _counters(workload).get(storage, {}).get(kernel_prefix)
 
 
-def step_traffic_from_profile(storage: str, kernels):
+def step_traffic_from_profile(storage: str, kernels, workload: str = "cavity3d"):
     """Measured HBM bytes of one whole step: the sum over the step's kernels (each launched once
     per step) of their calibrated counter bytes, or None if a kernel was not profiled."""
-    d = _counters().get(storage, {})
+    d = _counters(workload).get(storage, {})
     tot = 0
     for k in kernels:
         if k not in d:
@@ -139,7 +139,7 @@ def host_description(threads: int):
     return info
 
 
-def run_gpu(case, storage: str, steps: int, warmup: int, tune: bool = True):
+def run_gpu(case, storage: str, steps: int, warmup: int, tune: bool = True, workload: str = "cavity3d"):
     from dugksfoam_amd.engine import Engine
     e = Engine(case, storage=storage)
     e.init_equilibrium()
@@ -177,7 +177,7 @@ def run_gpu(case, storage: str, steps: int, warmup: int, tune: bool = True):
     roof = {"bound": "hbm", "kernel": "whole step (" + ", ".join(kern) + ")",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": step_traffic_from_profile(storage, list(kern)),
+            "traffic": step_traffic_from_profile(storage, list(kern), workload),
             "bytes_per_update": bpu, "updates_per_step": ncv, "bytes_per_step": bpu * ncv,
             "step_ms": round(step_ms, 4)}
     if h_elided:
@@ -188,7 +188,7 @@ def run_gpu(case, storage: str, steps: int, warmup: int, tune: bool = True):
     avg_ms, bytes_ = kern[dom]
     roof["dominant_kernel"] = {"name": dom, "avg_launch_ms": round(avg_ms, 4), "design_bytes_per_launch": bytes_,
                                "design_GBs": round(bytes_ / (avg_ms * 1e-3) / 1e9, 1),
-                               "traffic": traffic_from_profile(dom, storage)}
+                               "traffic": traffic_from_profile(dom, storage, workload)}
     # the engine's own byte model (stored face values counted as traffic it must move)
     roof["design_model"] = {
         "kernels": {k: {"avg_launch_ms": round(v[0], 4), "bytes_per_launch": v[1],
@@ -245,6 +245,19 @@ def cpu_baseline(target_s: float = 12.0):
                           "sample": f"FP64 oracle, 60^2 cells x 41^2 velocities, {n1} step(s), {t1:.1f} s"}
     return out
 
+
+# the named workloads of cases.config, as the JSON line's config.workload describes them
+WORKLOADS = {
+    "cavity3d": "cavity3d: 48^3 hexes x 24^3 velocities, Kn 0.1, lid Ma 0.16, diffuse walls 273 K",
+    "cavity2d": "cavity2d: 60^2 quads x 41^2 velocities, Kn 0.075, lid Ma 0.16, diffuse walls 273 K",
+    "delaunay": "delaunay: lid-driven cavity on 203582 Delaunay triangles (orthogonal wall row) x 64^2 "
+                "velocities, Kn 1",
+    "batch": "batch: 256 independent 2-D cavities (sides 20..80, Kn 0.01..10, per-case dt and muRef) "
+             "x 28^2 velocities",
+    "channel": "channel: 200x40 quads x 32^2 velocities, Kn 0.5, cyclic in x, diffuse walls",
+    "cavity3d_share": "cavity3d_share: 48^3 hexes x 12^3 velocities (one GPU's velocity share of "
+                      "cavity3d over 8 GPUs)",
+}
 
 SWEEP = (("cavity2d", 2000), ("delaunay", 1000), ("cavity3d", 200), ("batch", 1000), ("channel", 2000),
          ("cavity3d_share", 200))
@@ -343,6 +356,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-storage form")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no tuning steps)")
+    ap.add_argument("--workload", default="cavity3d", choices=list(WORKLOADS),
+                    help="the workload measured (default cavity3d); another one runs without the "
+                         "CPU-baseline and adjoint legs")
     ap.add_argument("--sweep", action="store_true",
                     help="only run every workload in FP64 and FP32 storage (throughput, FP32 error)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -376,9 +392,10 @@ def main():
               "velocity-space sharding over several GPUs is not built", file=sys.stderr)
         sys.exit(2)
     from dugksfoam_amd import cases
-    case = cases.config("cavity3d")
-    log(f"[bench] cavity3d cells={case.n_cells} nv={case.nv} steps={a.steps} warmup={a.warmup}")
-    r64 = run_gpu(case, "fp64", a.steps, a.warmup, tune=not a.no_tune)
+    wl = a.workload
+    case = cases.config(wl)
+    log(f"[bench] {wl} cells={case.n_cells} nv={case.nv} steps={a.steps} warmup={a.warmup}")
+    r64 = run_gpu(case, "fp64", a.steps, a.warmup, tune=not a.no_tune, workload=wl)
     log(f"[bench] fp64 {r64['value']:.2f} G upd/s, {r64['ms_per_step']:.2f} ms/step")
     out = {
         "metric": "G cell-velocity updates/s (FP64 plain path; FP32-storage form in fp32)",
@@ -393,17 +410,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "cavity3d: 48^3 hexes x 24^3 velocities, Kn 0.1, lid Ma 0.16, diffuse walls 273 K",
-                   "cells": case.n_cells, "velocities": case.nv, "dt": case.dt,
-                   "resident": "g only (h is identically zero for D=3, KInner=0)"},
+        "config": {"workload": WORKLOADS[wl], "cells": case.n_cells, "velocities": case.nv,
+                   "dt": case.dt if case.n_cases == 1 else "per case"},
         "roofline": r64["roofline"],
     }
+    if case.K + 3 - case.dim == 0:
+        out["config"]["resident"] = "g only (h is identically zero for D=3, KInner=0)"
     if not a.no_fp32:
-        r32 = run_gpu(case, "fp32", a.steps, a.warmup, tune=not a.no_tune)
+        r32 = run_gpu(case, "fp32", a.steps, a.warmup, tune=not a.no_tune, workload=wl)
         log(f"[bench] fp32 {r32['value']:.2f} G upd/s")
         out["fp32"] = {"value": round(r32["value"], 3), "ms_per_step": round(r32["ms_per_step"], 3),
                        "speedup_vs_fp64": round(r32["value"] / r64["value"], 3), "roofline": r32["roofline"]}
-    if not a.no_adjoint:
+    if not a.no_adjoint and wl == "cavity3d":
         # the discrete adjoint on the micro-channel (its own workload; not part of value)
         try:
             ad = adjoint_bench(a.adj_steps, min(a.adj_avg, a.adj_steps), min(a.adj_every, a.adj_steps))
@@ -412,7 +430,7 @@ def main():
                                                   "speedup_vs_central_differences")}
         except Exception as exc:   # reported, never allowed to hide the main line
             out["adjoint"] = {"error": repr(exc)}
-    if not a.no_cpu:
+    if not a.no_cpu and wl == "cavity3d":
         cb = cpu_baseline()
         log(f"[bench] cpu baseline {cb['value']:.4f} G upd/s on {cb['cores']} threads")
         out["cpu_baseline"] = cb
