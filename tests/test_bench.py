@@ -47,6 +47,18 @@ def test_committed_traffic_is_above_the_algorithmic_bytes():
     assert t is not None and t >= kb["k_cell"]
 
 
+@pytest.mark.parametrize("wl", ["cavity2d", "delaunay", "batch", "channel"])
+def test_every_workload_has_step_traffic_above_the_contract(wl):
+    """Each workload's committed counter file covers the four step kernels in both storage forms,
+    and the measured bytes of a step are at least the contract's 6 s bytes per update."""
+    c = cases.config(wl)
+    kernels = ["k_face_values", "k_face_moments", "k_face_fin", "k_cell"]
+    for st, s in (("fp64", 8), ("fp32", 4)):
+        t = bench.step_traffic_from_profile(st, kernels, wl)
+        assert t is not None, (wl, st)
+        assert t >= bench.contract_bytes_per_update(s) * c.n_cells * c.nv, (wl, st)
+
+
 def test_contract_bytes_per_update():
     """roofline.achieved counts B = 6 s per update: g and h streamed in twice and out once."""
     assert bench.contract_bytes_per_update(8) == 48
