@@ -47,7 +47,7 @@ def test_committed_traffic_is_above_the_algorithmic_bytes():
     assert t is not None and t >= kb["k_cell"]
 
 
-@pytest.mark.parametrize("wl", ["cavity2d", "delaunay", "batch", "channel"])
+@pytest.mark.parametrize("wl", ["cavity2d", "delaunay", "batch", "channel", "cavity3d_share"])
 def test_every_workload_has_step_traffic_above_the_contract(wl):
     """Each workload's committed counter file covers the four step kernels in both storage forms,
     and the measured bytes of a step are at least the contract's 6 s bytes per update."""
