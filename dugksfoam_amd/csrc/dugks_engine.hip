@@ -87,6 +87,13 @@ constexpr int S1_UNR = 4;      // velocities per thread in flight in k_cell_s1
 #ifndef FACE_WAVES_FP32
 #define FACE_WAVES_FP32 5
 #endif
+// k_face_values asks for 8 waves where that costs no spills: g-only stencils of up to 8 faces and
+// the 2-D and 1-D stencils of 4.  The register allocator then also keeps the kernel under 100
+// SGPRs; at the lower target it takes 106 and the SGPR file caps the SIMD at 7 waves (the
+// 48^3 x 24^3 cavity's face pass: 14.0-14.4 instead of 14.8-15.2 ms in FP64).
+#ifndef KFV_WAVES
+#define KFV_WAVES 8
+#endif
 #ifndef CELL_WAVES
 #define CELL_WAVES 4
 #endif
@@ -750,6 +757,13 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
     if ((int)threadIdx.x < NM) p.fpart[((size_t)f * p.nb + b) * NM + threadIdx.x] = tot[threadIdx.x];
 }
 
+// Wave target of k_face_values (KFV_WAVES where no instantiation of that shape spills).
+template <int D, bool H, int NS, typename TS>
+constexpr int kfv_waves()
+{
+    return (!H && NS <= 8) || (D <= 2 && NS <= 4) ? KFV_WAVES : sizeof(TS) == 8 ? FACE_WAVES : FACE_WAVES_FP32;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Face-value mode, S2 in two passes.
 // k_face_values: cell-wise, over one velocity block.  The least-squares gradient of the cell is
@@ -765,7 +779,7 @@ __global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS
 // branches unrolled over the stencil, without the boundary-kind dispatch of the general loop.
 // ---------------------------------------------------------------------------------------------
 template <int D, bool H, int NS, typename TS>
-__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(sizeof(TS) == 8 ? FACE_WAVES : FACE_WAVES_FP32, 8))) void k_face_values(P<TS> p)
+__global__ __launch_bounds__(MAXNT) __attribute__((amdgpu_waves_per_eu(kfv_waves<D, H, NS, TS>(), 8))) void k_face_values(P<TS> p)
 {
     __shared__ double vt[VT_SIZE];
     int pos, b;
